@@ -1,0 +1,230 @@
+"""Benchmark of the plain quadtree point-in-polygon join on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+
+A step is one pass of the whole path over the gaussian_holes workload with the
+inputs resident in HBM: quadtree_on_points -> polygon_bounding_boxes ->
+join_quadtree_and_bounding_boxes -> quadtree_point_in_polygon, every stage on
+the GPU through libcuspatial_amd's C ABI (outputs included). The metric is
+points joined per second. One JSON line is printed; the roofline of the
+dominant kernel comes from the library's HIP-event timers over the timed steps,
+and the CPU baseline is the oracle (OpenMP) on a bounded prefix of the same
+points with the same polygons.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("points/sec joined on 1 MI355X, learned+batched vs plain quadtree PIP (exact); "
+          "p99 ms")
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def effective_edges(part_offset, ring_offset, px, py):
+    """Edges the crossing test visits per polygon: each ring's vertex pairs
+    (last->first included) minus zero-length ones (closed rings repeat a vertex)."""
+    import numpy as np
+
+    n = px.size
+    nxt = np.arange(n) - 1  # b = previous vertex, a = this vertex
+    starts = ring_offset[:-1].astype(np.int64)
+    ends = ring_offset[1:].astype(np.int64)
+    nxt[starts] = ends - 1  # the first vertex pairs with the ring's last
+    zero = (px == px[nxt]) & (py == py[nxt])
+    ring_id = np.repeat(np.arange(starts.size), ends - starts)
+    ring_edges = np.bincount(ring_id, weights=~zero, minlength=starts.size)
+    poly_of_ring = np.repeat(np.arange(part_offset.size - 1), np.diff(part_offset))
+    return np.bincount(poly_of_ring, weights=ring_edges, minlength=part_offset.size - 1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--points", type=int, default=100_000_000)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="target CPU work of the oracle baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the path has no "
+              "multi-GPU mode in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import cuspatial_amd as cs
+    from cuspatial_amd import _capi, datasets
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    log(f"[bench] generating gaussian_holes with {args.points} points on {dev}")
+    w = datasets.gaussian_holes(num_points=args.points, device=dev)
+    bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
+    pts = cs.Points(w.x, w.y)
+    polys = cs.Polygons(w.part_offset, w.ring_offset, w.px, w.py)
+    torch.cuda.synchronize()
+
+    def step():
+        pi, tree = cs.quadtree_on_points(pts, *bbox, w.scale, w.max_depth, w.max_size)
+        bb = cs.polygon_bounding_boxes(polys)
+        pairs = cs.join_quadtree_and_bounding_boxes(tree, bb, *bbox, w.scale, w.max_depth)
+        res = cs.quadtree_point_in_polygon(pairs, tree, pi, pts, polys)
+        return pi, tree, pairs, res
+
+    for i in range(args.warmup):
+        log(f"[bench] warmup {i}")
+        out = step()
+    del out
+    torch.cuda.synchronize()
+
+    _capi.profile_reset()
+    _capi.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _capi.profile_enable(False)
+    prof = _capi.profile_report()
+    pi, tree, pairs, res = out
+    n = w.num_points
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n * args.steps / (t1 - t0)
+    log(f"[bench] {ms_per_step:.3f} ms/step, {value:.4g} points/s; nodes={len(tree)} "
+        f"pairs={len(pairs)} hits={len(res)}")
+
+    # --- per-kernel algorithmic work ------------------------------------------
+    key_bits = min(32, 2 * math.ceil(math.log2((1 << w.max_depth) + 3)))
+    passes = math.ceil(key_bits / 8)
+    ql = tree["length"].view(torch.int32).to(torch.int64)
+    pq = pairs["quad_offset"].view(torch.int32).to(torch.int64)
+    pp = pairs["bbox_offset"].view(torch.int32).to(torch.int64)
+    edges = torch.as_tensor(effective_edges(w.part_offset, w.ring_offset, w.px, w.py), device=dev)
+    cand = ql[pq]
+    edge_tests = float((cand.to(torch.float64) * edges[pp]).sum())
+    candidates = float(cand.sum())
+    work = {
+        # name: (bound, algorithmic units per launch, unit)
+        "keygen": ("hbm", n * (16 + 4) / 1e9, "GB"),
+        "point_sort_count": ("hbm", n * 4 / 1e9, "GB"),
+        "point_sort_scatter": ("hbm", n * (12 + 16 * (passes - 1)) / passes / 1e9, "GB"),
+        "pip_test": ("fp64", 6.0 * edge_tests / 1e12, "TFLOP"),
+    }
+    kernels = {}
+    for name, (cnt, tot_ms) in prof.items():
+        avg_ms = tot_ms / max(cnt, 1)
+        entry = {"launches_per_step": cnt / args.steps, "avg_ms": round(avg_ms, 6),
+                 "ms_per_step": round(tot_ms / args.steps, 6)}
+        if name in work:
+            bound, units, unit = work[name]
+            rate = units / (avg_ms / 1e3)
+            peak = HBM_PEAK_GBS if bound == "hbm" else FP64_PEAK_TFLOPS
+            entry.update(bound=bound, achieved=round(rate, 3),
+                         unit="GB/s" if bound == "hbm" else "TFLOP/s",
+                         frac=round(rate / peak, 4))
+        kernels[name] = entry
+    dominant = max(prof.items(), key=lambda kv: kv[1][1])[0]
+    dom = kernels[dominant]
+    roofline = None
+    if "achieved" in dom:
+        bound, units, unit = work[dominant]
+        roofline = {
+            "kernel": dominant,
+            "bound": "hbm" if bound == "hbm" else "mfma",
+            "engine": "HBM" if bound == "hbm" else "FP64 VALU (peak equals the FP64 matrix peak)",
+            "achieved": dom["achieved"],
+            "peak": HBM_PEAK_GBS if bound == "hbm" else FP64_PEAK_TFLOPS,
+            "unit": dom["unit"],
+            "frac": dom["frac"],
+            "traffic": None,
+            "avg_ms": dom["avg_ms"],
+            "algorithmic_per_launch": f"{units:.6g} {unit}",
+        }
+
+    # --- CPU baseline: the oracle on a bounded prefix of the same points ---------
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+
+        oracle.build()
+        threads = oracle.max_threads()
+
+        def run_oracle(m):
+            sub = datasets.subsample(w, m)
+            t = time.perf_counter()
+            tr = oracle.quadtree_on_points(sub.x, sub.y, *bbox, w.scale, w.max_depth, w.max_size)
+            obb = oracle.polygon_bounding_boxes(w.part_offset, w.ring_offset, w.px, w.py)
+            jb, jq = oracle.join_quadtree_and_bounding_boxes(tr[1:], obb, *bbox, w.scale,
+                                                             w.max_depth)
+            oracle.quadtree_point_in_polygon((jb, jq), tr[1:], tr[0], sub.x, sub.y,
+                                             w.part_offset, w.ring_offset, w.px, w.py)
+            return time.perf_counter() - t
+
+        m = min(n, 1_000_000)
+        dt = run_oracle(m)
+        m2 = int(min(n, 20_000_000, max(m, m * args.cpu_seconds / max(dt, 1e-3))))
+        if m2 > m:
+            m, dt = m2, run_oracle(m2)
+        cpu = {"value": m / dt, "unit": "points/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} points of the same workload with all "
+                         f"{w.num_polygons} polygons, full path (tree, boxes, join, PIP), "
+                         f"{dt:.2f} s", "seconds": round(dt, 3)}
+        log(f"[bench] cpu baseline {m / dt:.4g} points/s on {threads} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "gaussian_holes: plain quadtree point-in-polygon join "
+                        "(tree + boxes + join + PIP), inputs resident in HBM",
+            "points": n,
+            "polygons": w.num_polygons,
+            "vertices": int(w.px.size),
+            "max_depth": w.max_depth,
+            "max_size": w.max_size,
+            "scale": w.scale,
+            "seed": w.seed,
+            "nodes": len(tree),
+            "pairs": len(pairs),
+            "candidates": candidates,
+            "edge_tests": edge_tests,
+            "hits": len(res),
+        },
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,175 @@
+"""ctypes binding of libcuspatial_amd (the C ABI declared in include/cuspatial_amd.h).
+
+This is the counterpart of the reference's Cython layer
+(python/cuspatial/cuspatial/_lib/quadtree.pyx, spatial_join.pyx,
+polygon_bounding_boxes.pyx): it releases the GIL around each call (ctypes does)
+and maps the library's status codes to the exceptions Cython's ``except +``
+raises for the reference (logic_error and cuda_error -> RuntimeError,
+out_of_memory -> MemoryError).
+
+Outputs are allocated through a ``qpip_resource`` whose callbacks create torch
+tensors, so every output column is a torch tensor on the input's device with
+no copy.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcuspatial_amd.so")
+
+QPIP_SUCCESS = 0
+QPIP_ERROR_LOGIC = 1
+QPIP_ERROR_DEVICE = 2
+QPIP_ERROR_OOM = 3
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+
+_ALLOC = ctypes.CFUNCTYPE(_vp, _vp, ctypes.c_size_t, _vp)
+_FREE = ctypes.CFUNCTYPE(None, _vp, _vp, ctypes.c_size_t, _vp)
+
+
+class qpip_resource(ctypes.Structure):
+    _fields_ = [("allocate", _ALLOC), ("deallocate", _FREE), ("ctx", _vp)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+# Every symbol the C header declares (checked by the CPU test suite).
+EXPORTED = (
+    "qpip_last_error_message",
+    "qpip_version",
+    "qpip_free",
+    "qpip_set_memory_limit",
+    "qpip_trim",
+    "qpip_profile_enable",
+    "qpip_profile_reset",
+    "qpip_profile_report",
+    "qpip_quadtree_on_points",
+    "qpip_polygon_bounding_boxes",
+    "qpip_join_quadtree_and_bounding_boxes",
+    "qpip_quadtree_point_in_polygon",
+)
+
+
+def lib():
+    """Loads the in-tree library. Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m cuspatial_amd._build` "
+                "(the HIP library is required; there is no fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        L.qpip_last_error_message.restype = ctypes.c_char_p
+        L.qpip_version.restype = ctypes.c_char_p
+        L.qpip_free.argtypes = [_vp]
+        L.qpip_set_memory_limit.argtypes = [ctypes.c_size_t]
+        L.qpip_profile_enable.argtypes = [ctypes.c_int]
+        L.qpip_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
+        L.qpip_profile_report.restype = ctypes.c_int64
+        L.qpip_quadtree_on_points.argtypes = [
+            _vp, _vp, _i64, _i64, _d, _d, _d, _d, _d, ctypes.c_int8, ctypes.c_int32,
+            P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]
+        L.qpip_polygon_bounding_boxes.argtypes = [
+            _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, _d, P(qpip_resource), _vp,
+            P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]
+        L.qpip_join_quadtree_and_bounding_boxes.argtypes = [
+            _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp, _i64, _d, _d, _d, _d, _d,
+            ctypes.c_int8, P(qpip_resource), _vp, P(_vp), P(_vp), P(_i64)]
+        L.qpip_quadtree_point_in_polygon.argtypes = [
+            _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
+            _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, P(qpip_resource), _vp,
+            P(_vp), P(_vp), P(_i64)]
+        for name in (
+            "qpip_quadtree_on_points", "qpip_polygon_bounding_boxes",
+            "qpip_join_quadtree_and_bounding_boxes", "qpip_quadtree_point_in_polygon",
+        ):
+            getattr(L, name).restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def version() -> str:
+    return lib().qpip_version().decode()
+
+
+def profile_enable(on: bool = True) -> None:
+    lib().qpip_profile_enable(1 if on else 0)
+
+
+def profile_reset() -> None:
+    lib().qpip_profile_reset()
+
+
+def profile_report() -> dict:
+    """{kernel name: (launches, total ms)} recorded while profiling was enabled."""
+    import json
+
+    L = lib()
+    n = L.qpip_profile_report(None, 0)
+    buf = ctypes.create_string_buffer(int(n) + 1)
+    L.qpip_profile_report(buf, len(buf))
+    return {k: (int(v[0]), float(v[1])) for k, v in json.loads(buf.value.decode()).items()}
+
+
+def check(rc: int) -> None:
+    if rc == QPIP_SUCCESS:
+        return
+    msg = lib().qpip_last_error_message().decode()
+    if rc == QPIP_ERROR_OOM:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+class TorchResource:
+    """A qpip_resource whose buffers are torch uint8 tensors on `device`.
+
+    The library calls ``allocate`` for each output column; the tensors are kept
+    here until ``take`` hands them to the caller (viewed with the right dtype).
+    """
+
+    def __init__(self, device: torch.device):
+        self.device = device
+        self._live: dict[int, torch.Tensor] = {}
+        self._alloc_cb = _ALLOC(self._allocate)
+        self._free_cb = _FREE(self._deallocate)
+        self.struct = qpip_resource(self._alloc_cb, self._free_cb, None)
+
+    def _allocate(self, ctx, nbytes, stream):
+        try:
+            t = torch.empty(int(nbytes), dtype=torch.uint8, device=self.device)
+        except Exception:  # noqa: BLE001 - reported to the library as NULL (out of memory)
+            return None
+        p = t.data_ptr()
+        self._live[p] = t
+        return p
+
+    def _deallocate(self, ctx, ptr, nbytes, stream):
+        self._live.pop(ptr, None)
+
+    def take(self, ptr, n: int, dtype: torch.dtype) -> torch.Tensor:
+        if n == 0 or not ptr:
+            return torch.empty(0, dtype=dtype, device=self.device)
+        t = self._live.pop(ptr)
+        return t.view(dtype)[:n]
+
+
+def stream_handle(device: torch.device):
+    """The HIP stream the library should use: torch's current stream on `device`."""
+    if device.type != "cuda":
+        return None
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)

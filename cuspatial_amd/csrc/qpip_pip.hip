@@ -1,0 +1,432 @@
+// quadtree_point_in_polygon on gfx950.
+//
+// Reference: cpp/src/join/quadtree_point_in_polygon.cu:143-191 (column API) and
+// detail/join/quadtree_point_in_polygon.cuh:104-235: every point of every
+// (polygon, leaf) pair is tested with is_point_in_polygon
+// (detail/algorithm/is_point_in_polygon.cuh:46-100) and the hits are compacted
+// in pair order, then point order within the leaf.
+//
+// Design:
+//   1. Edge table. The crossing test walks each ring from the last vertex, skips
+//      vertices whose edge to the current `b` is zero within 4 ULP (without moving
+//      `b`, :65), and tests every other edge (a, b). Which edges are skipped does
+//      not depend on the test point, so each ring is walked once here into a list
+//      of effective edges {ax, ay, run, rise} + {bx, by}. The per-point test is then
+//      order-independent: inside = parity of crossings over all effective edges,
+//      forced to false if any edge is an on-edge hit (the reference breaks out and
+//      returns false at the first such edge).
+//   2. Chunks. Pair k covers ceil(length / 64) chunks of 64 consecutive point
+//      positions; a scan gives each pair its first chunk.
+//   3. Test: one wave per chunk. The lanes read 64 consecutive point_indices
+//      entries (coalesced) and gather their coordinates; the polygon's edges are
+//      wave-uniform and are read through the scalar cache. The FP64 arithmetic is
+//      the reference's; the 4-ULP collinearity test is preceded by an exact
+//      floating-point filter that rejects pairs of products too far apart to be
+//      within 4 ULP. Each wave stores its 64-bit hit ballot and its popcount.
+//   4. Scan of hit counts -> output positions; one host sync for the total.
+//   5. Write: each wave expands its ballot into (polygon, point position) pairs.
+//   No candidate-sized buffer is ever allocated, so the reference's out-of-memory
+//   fallback (count, then allocate; :220-234) is the only path here.
+#include <algorithm>
+
+#include "qpip_scan.hpp"
+
+namespace qpip {
+
+namespace {
+
+// Hot part of an effective edge (read by every point): 32 bytes, one scalar load.
+struct alignas(32) Edge {
+  double ax, ay, run, rise;
+};
+// Cold part: the edge's `b` vertex (ring start y0 and the on-edge x range).
+struct alignas(16) EdgeB {
+  double bx, by;
+};
+
+// float_equal (detail/utility/floating_point.cuh:97-129): |biased(a) - biased(b)| <= 4.
+__device__ __forceinline__ bool float_equal(double a, double b)
+{
+  if (isnan(a) || isnan(b)) return false;
+  uint64_t const sign = 1ull << 63;
+  uint64_t ua         = static_cast<uint64_t>(__double_as_longlong(a));
+  uint64_t ub         = static_cast<uint64_t>(__double_as_longlong(b));
+  ua                  = (ua & sign) ? (~ua + 1) : (ua | sign);
+  ub                  = (ub & sign) ? (~ub + 1) : (ub | sign);
+  return ua >= ub ? (ua - ub) <= 4 : (ub - ua) <= 4;
+}
+
+struct PipMeta {
+  uint64_t num_chunks;
+  uint64_t num_hits;
+  uint32_t bad;
+};
+
+// --- 1. edge table: one thread per ring -----------------------------------------
+__global__ void __launch_bounds__(256)
+  build_edges_kernel(uint32_t const* __restrict__ ring_offsets,
+                     int64_t num_rings,
+                     double const* __restrict__ vx,
+                     double const* __restrict__ vy,
+                     int64_t nv,
+                     Edge* __restrict__ edges,
+                     EdgeB* __restrict__ edges_b,
+                     uint32_t* __restrict__ ring_edges)
+{
+  int64_t const r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= num_rings) return;
+  int64_t vb = ring_offsets[r];
+  int64_t ve = ring_offsets[r + 1];
+  if (vb > nv) vb = nv;
+  if (ve > nv) ve = nv;
+  uint32_t ne = 0;
+  if (ve > vb) {
+    double bx = vx[ve - 1], by = vy[ve - 1];
+    for (int64_t v = vb; v < ve; ++v) {
+      double const ax = vx[v], ay = vy[v];
+      double const run = bx - ax, rise = by - ay;
+      if (float_equal(run, 0.0) && float_equal(rise, 0.0)) continue;
+      edges[vb + ne]   = Edge{ax, ay, run, rise};
+      edges_b[vb + ne] = EdgeB{bx, by};
+      ++ne;
+      bx = ax;
+      by = ay;
+    }
+  }
+  ring_edges[r] = ne;
+}
+
+// --- 2. chunk enumeration ---------------------------------------------------------
+struct ChunkOp {
+  uint32_t const* pair_quad;
+  uint32_t const* length;
+  uint64_t m;
+  uint32_t* chunk_start;
+  uint32_t* bad;
+  __device__ uint32_t count(uint64_t k) const
+  {
+    uint32_t const q = pair_quad[k];
+    if (q >= m) {
+      atomicOr(bad, 1u);
+      return 0;
+    }
+    return (length[q] + kWave - 1) / kWave;
+  }
+  __device__ void emit(uint64_t k, uint32_t prefix) const { chunk_start[k] = prefix; }
+};
+
+struct PipArgs {
+  uint32_t const* pair_poly;
+  uint32_t const* pair_quad;
+  uint64_t num_pairs;
+  uint32_t const* length;
+  uint32_t const* offset;
+  uint32_t const* point_indices;
+  uint64_t num_points;
+  double const* x;
+  double const* y;
+  uint32_t const* poly_offsets;
+  uint64_t num_polys;
+  uint64_t num_rings;
+  uint32_t const* ring_offsets;
+  uint32_t const* ring_edges;
+  Edge const* edges;
+  EdgeB const* edges_b;
+  uint32_t const* chunk_start;
+  uint64_t num_chunks;
+};
+
+// Pair owning chunk c: the last k with chunk_start[k] <= c (get_quad_and_local_point_indices.cuh:26-45).
+__device__ __forceinline__ uint32_t pair_of_chunk(uint32_t const* chunk_start, uint64_t K, uint32_t c)
+{
+  uint32_t lo = 0, hi = static_cast<uint32_t>(K);
+  while (lo < hi) {
+    uint32_t const mid = lo + ((hi - lo) >> 1);
+    if (chunk_start[mid] <= c)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo - 1;
+}
+
+// --- 3. test ------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+  pip_test_kernel(PipArgs a, uint64_t* __restrict__ masks, uint32_t* __restrict__ hit_counts,
+                  uint32_t* __restrict__ bad)
+{
+  uint32_t const c = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  if (c >= a.num_chunks) return;
+  uint32_t const k     = uniform(pair_of_chunk(a.chunk_start, a.num_pairs, c));
+  uint32_t const poly  = uniform(a.pair_poly[k]);
+  uint32_t const quad  = uniform(a.pair_quad[k]);
+  uint32_t const len   = uniform(a.length[quad]);
+  uint32_t const first = uniform(a.offset[quad]);
+  uint32_t const j     = c - a.chunk_start[k];
+  uint32_t const local = j * kWave + lane_id();
+  bool valid           = local < len;
+  // Errors are recorded at the end: a global store before the edge loads would
+  // keep them off the scalar unit (the compiler must assume it may alias).
+  bool err = false;
+  double px = 0.0, py = 0.0;
+  uint32_t pos = first + local;
+  if (valid) {
+    if (pos >= a.num_points) {
+      err   = true;
+      valid = false;
+    } else {
+      uint32_t const idx = a.point_indices[pos];
+      if (idx >= a.num_points) {
+        err   = true;
+        valid = false;
+      } else {
+        px = a.x[idx];
+        py = a.y[idx];
+      }
+    }
+  }
+  // Collinearity filter. If float_equal(rhs, lhs) (4 ULP), then for normal
+  // values |rhs - lhs| is exact (Sterbenz) and below 2^-50 * max(|rhs|, |lhs|)
+  // <= 2^-48 * |rhs|, and near zero it is below 8 * 2^-1074. So
+  // fma(|rhs|, -2^-48, |rhs - lhs|) <= 2^-1020 holds for every float_equal pair
+  // (NaN compares as "maybe"); lanes where some edge passes are re-examined
+  // exactly in a second, rare pass.
+  //
+  // Lane state is kept as wave-uniform 64-bit masks (one bit per lane), so the
+  // loop's bookkeeping is a handful of scalar ops per edge.
+  uint64_t inside_m = 0, suspect_m = 0;
+  uint32_t r0 = 0, r1 = 0;
+  if (poly < a.num_polys) {
+    r0 = uniform(a.poly_offsets[poly]);
+    r1 = uniform(a.poly_offsets[poly + 1]);
+    if (r1 > a.num_rings) r1 = static_cast<uint32_t>(a.num_rings);
+  } else if (valid) {
+    err = true;
+  }
+  for (uint32_t r = r0; r < r1; ++r) {
+    uint32_t const e0 = uniform(a.ring_offsets[r]);
+    uint32_t const ne = uniform(a.ring_edges[r]);
+    if (ne == 0) continue;
+    Edge const* E = a.edges + e0;
+    // y0 of the first edge is its `b` (the ring's last vertex); afterwards the
+    // previous edge's `a` (is_point_in_polygon.cuh:339, :375-376).
+    uint64_t y0 = ballot(a.edges_b[e0].by > py);
+    auto edge = [&](Edge const& ed) {
+      double const lhs  = (px - ed.ax) * ed.rise;  // run_to_point * rise
+      double const rhs  = ed.run * (py - ed.ay);   // run * rise_to_point
+      double const gap  = fma(fabs(rhs), -0x1p-48, fabs(rhs - lhs));
+      uint64_t const y1 = ballot(ed.ay > py);
+      uint64_t const lt = ballot(lhs < rhs);
+      suspect_m |= ballot(!(gap > 0x1p-1020));
+      inside_m ^= (y1 ^ y0) & (lt ^ y1);
+      y0 = y1;
+    };
+    // Unrolled by 4 so that the loop's scalar bookkeeping is amortised.
+    uint32_t e = 0;
+    for (; e + 4 <= ne; e += 4) {
+      edge(E[e]);
+      edge(E[e + 1]);
+      edge(E[e + 2]);
+      edge(E[e + 3]);
+    }
+    for (; e < ne; ++e) edge(E[e]);
+  }
+  int const lane     = lane_id();
+  bool const inside  = (inside_m >> lane) & 1ull;
+  bool const suspect = ((suspect_m >> lane) & 1ull) && valid;
+  bool on_edge       = false;
+  if (ballot(suspect)) {
+    for (uint32_t r = r0; r < r1; ++r) {
+      uint32_t const e0 = uniform(a.ring_offsets[r]);
+      uint32_t const ne = uniform(a.ring_edges[r]);
+      for (uint32_t e = 0; e < ne; ++e) {
+        Edge const E    = a.edges[e0 + e];
+        double const bx = a.edges_b[e0 + e].bx;
+        // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
+        if (suspect && float_equal(E.run * (py - E.ay), (px - E.ax) * E.rise)) {
+          double const minx = E.ax < bx ? E.ax : bx;
+          double const maxx = E.ax < bx ? bx : E.ax;
+          if (minx <= px && px <= maxx) on_edge = true;
+        }
+      }
+    }
+  }
+  bool const hit       = valid && inside && !on_edge;
+  uint64_t const mask  = ballot(hit);
+  if (lane_id() == 0) {
+    masks[c]      = mask;
+    hit_counts[c] = static_cast<uint32_t>(__popcll(mask));
+  }
+  if (err) atomicOr(bad, 1u);
+}
+
+// --- 4. scan of hit counts ----------------------------------------------------------
+struct HitScanOp {
+  uint32_t const* hit_counts;
+  uint64_t* hit_start;
+  __device__ uint64_t count(uint64_t c) const { return hit_counts[c]; }
+  __device__ void emit(uint64_t c, uint64_t prefix) const { hit_start[c] = prefix; }
+};
+
+// --- 5. write -------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+  pip_write_kernel(PipArgs a, uint64_t const* __restrict__ masks, uint64_t const* __restrict__ hit_start,
+                   uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point)
+{
+  uint32_t const c = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  if (c >= a.num_chunks) return;
+  uint64_t const mask = masks[c];
+  if (mask == 0) return;
+  uint32_t const k     = uniform(pair_of_chunk(a.chunk_start, a.num_pairs, c));
+  uint32_t const poly  = a.pair_poly[k];
+  uint32_t const quad  = a.pair_quad[k];
+  uint32_t const first = a.offset[quad];
+  uint32_t const j     = c - a.chunk_start[k];
+  int const l          = lane_id();
+  if ((mask >> l) & 1ull) {
+    uint64_t const o = hit_start[c] + static_cast<uint64_t>(__popcll(mask & lanemask_lt()));
+    out_poly[o]      = poly;
+    out_point[o]     = first + j * kWave + static_cast<uint32_t>(l);
+  }
+}
+
+}  // namespace
+
+}  // namespace qpip
+
+extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
+                                              const uint32_t* quad_index,
+                                              int64_t num_pairs,
+                                              const uint32_t* key,
+                                              const uint8_t* level,
+                                              const uint8_t* is_internal_node,
+                                              const uint32_t* length,
+                                              const uint32_t* offset,
+                                              int64_t num_nodes,
+                                              const uint32_t* point_indices,
+                                              int64_t num_point_indices,
+                                              const double* x,
+                                              const double* y,
+                                              int64_t num_x,
+                                              int64_t num_y,
+                                              const uint32_t* poly_offsets,
+                                              int64_t num_poly_offsets,
+                                              const uint32_t* ring_offsets,
+                                              int64_t num_ring_offsets,
+                                              const double* poly_x,
+                                              const double* poly_y,
+                                              int64_t num_poly_x,
+                                              int64_t num_poly_y,
+                                              const qpip_resource* mr,
+                                              void* stream_,
+                                              uint32_t** polygon_index,
+                                              uint32_t** point_index,
+                                              int64_t* num_hits)
+{
+  using namespace qpip;
+  (void)key;
+  (void)level;
+  (void)is_internal_node;
+  try {
+    QPIP_EXPECTS(polygon_index && point_index && num_hits, "output pointers must not be null");
+    *polygon_index = *point_index = nullptr;
+    *num_hits                     = 0;
+    hipStream_t stream            = static_cast<hipStream_t>(stream_);
+    // quadtree_point_in_polygon.cu:154-178 (dtype checks are the binding's)
+    QPIP_EXPECTS(num_point_indices == num_x && num_x == num_y,
+                 "number of points must be the same for both x and y columns");
+    QPIP_EXPECTS(num_poly_x == num_poly_y,
+                 "numbers of vertices must be the same for both x and y columns");
+    if (num_pairs == 0 || num_nodes == 0 || num_point_indices == 0 || num_poly_offsets == 0)
+      return QPIP_SUCCESS;
+    QPIP_EXPECTS(num_pairs < (int64_t{1} << 32) && num_x < (int64_t{1} << 32),
+                 "inputs must have fewer than 2^32 rows");
+    uint64_t const K         = static_cast<uint64_t>(num_pairs);
+    uint64_t const num_rings = num_ring_offsets > 0 ? static_cast<uint64_t>(num_ring_offsets - 1) : 0;
+
+    dbuf<PipMeta> meta(1);
+    dbuf<uint32_t> bad(1);
+    QPIP_HIP(hipMemsetAsync(meta.get(), 0, sizeof(PipMeta), stream));
+    QPIP_HIP(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), stream));
+
+    // 1. edges
+    dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
+    dbuf<EdgeB> edges_b(std::max<int64_t>(num_poly_x, 1));
+    dbuf<uint32_t> ring_edges(std::max<uint64_t>(num_rings, 1));
+    if (num_rings > 0) {
+      KernelTimer kt("pip_edges", stream);
+      build_edges_kernel<<<grid_for(num_rings, 256), 256, 0, stream>>>(
+        ring_offsets, static_cast<int64_t>(num_rings), poly_x, poly_y, num_poly_x, edges.get(),
+        edges_b.get(), ring_edges.get());
+      QPIP_LAUNCH_CHECK();
+    }
+    // 2. chunks
+    dbuf<uint32_t> chunk_start(K + 1);
+    dbuf<uint32_t> partials32(kScanBlocks + 1);
+    ChunkOp cop{quad_index, length, static_cast<uint64_t>(num_nodes), chunk_start.get(), bad.get()};
+    count_scan_emit<ChunkOp, uint32_t, uint64_t>(cop, nullptr, K, partials32.get(),
+                                                 &meta.get()->num_chunks, stream, scan_blocks_for(K),
+                                                 "pip_chunks");
+    PipMeta h_meta;
+    QPIP_HIP(hipMemcpyAsync(&h_meta, meta.get(), sizeof(PipMeta), hipMemcpyDeviceToHost, stream));
+    uint32_t h_bad = 0;
+    QPIP_HIP(hipMemcpyAsync(&h_bad, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    QPIP_HIP(hipStreamSynchronize(stream));
+    if (h_bad) throw logic_error("cuSpatial failure: quadrant index out of range of the quadtree");
+    uint64_t const T = h_meta.num_chunks;
+    if (T == 0) return QPIP_SUCCESS;
+    QPIP_EXPECTS(T < (uint64_t{1} << 31), "too many candidate points");
+
+    PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
+                 static_cast<uint64_t>(num_x), x, y, poly_offsets,
+                 static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
+                 ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), T};
+    // 3. test
+    dbuf<uint64_t> masks(T);
+    dbuf<uint32_t> hit_counts(T);
+    unsigned const blocks = static_cast<unsigned>((T + 3) / 4);
+    {
+      KernelTimer kt("pip_test", stream);
+      pip_test_kernel<<<blocks, 256, 0, stream>>>(args, masks.get(), hit_counts.get(), bad.get());
+      QPIP_LAUNCH_CHECK();
+    }
+    // 4. scan
+    dbuf<uint64_t> hit_start(T);
+    dbuf<uint64_t> partials64(kScanBlocks + 1);
+    HitScanOp hop{hit_counts.get(), hit_start.get()};
+    count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, nullptr, T, partials64.get(),
+                                                   &meta.get()->num_hits, stream, scan_blocks_for(T),
+                                                   "pip_scan");
+    QPIP_HIP(hipMemcpyAsync(&h_meta, meta.get(), sizeof(PipMeta), hipMemcpyDeviceToHost, stream));
+    QPIP_HIP(hipMemcpyAsync(&h_bad, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    QPIP_HIP(hipStreamSynchronize(stream));
+    if (h_bad) throw logic_error("cuSpatial failure: point or polygon index out of range");
+    uint64_t const H = h_meta.num_hits;
+    if (H == 0) return QPIP_SUCCESS;
+    QPIP_EXPECTS(H < (uint64_t{1} << 32), "more than 2^32 hits");
+    // 5. write
+    OutputAllocator oa(mr, stream);
+    uint32_t* o_poly  = oa.make<uint32_t>(H);
+    uint32_t* o_point = oa.make<uint32_t>(H);
+    {
+      KernelTimer kt("pip_write", stream);
+      pip_write_kernel<<<blocks, 256, 0, stream>>>(args, masks.get(), hit_start.get(), o_poly,
+                                                   o_point);
+      QPIP_LAUNCH_CHECK();
+    }
+    QPIP_HIP(hipStreamSynchronize(stream));
+    *polygon_index = o_poly;
+    *point_index   = o_point;
+    *num_hits      = static_cast<int64_t>(H);
+    return QPIP_SUCCESS;
+  } catch (logic_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_LOGIC;
+  } catch (oom_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_OOM;
+  } catch (std::exception const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_DEVICE;
+  }
+}

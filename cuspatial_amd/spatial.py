@@ -1,0 +1,277 @@
+"""Python entry points of the quadtree point-in-polygon path.
+
+Mirrors the reference's Python API (same names, argument order, clamping,
+warnings and column names):
+  quadtree_on_points                python/cuspatial/cuspatial/core/spatial/indexing.py:15-199
+  join_quadtree_and_bounding_boxes  python/cuspatial/cuspatial/core/spatial/join.py:105-175
+  quadtree_point_in_polygon         python/cuspatial/cuspatial/core/spatial/join.py:178-262
+  polygon_bounding_boxes            python/cuspatial/cuspatial/core/spatial/bounding.py:19-80
+
+cuDF is not used: points and polygons are given as :class:`Points` /
+:class:`Polygons` (GeoArrow offset + coordinate arrays, as the reference's
+GeoSeries holds them), and results are :class:`Table` objects whose columns are
+torch tensors on the GPU. Everything runs through libcuspatial_amd's C ABI.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import warnings
+from collections import OrderedDict
+
+import numpy as np
+import torch
+
+from . import _capi
+
+_vp = ctypes.c_void_p
+
+
+# ---------------------------------------------------------------------------
+# Containers
+# ---------------------------------------------------------------------------
+def _device(device=None) -> torch.device:
+    if device is not None:
+        return torch.device(device)
+    if not torch.cuda.is_available():
+        raise RuntimeError("cuspatial_amd needs a ROCm GPU (torch.cuda.is_available() is False)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _as_tensor(a, dtype, device, name) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        t = a
+    else:
+        arr = np.asarray(a)
+        if arr.dtype == np.uint32:
+            arr = arr.astype(np.int64)  # torch.from_numpy lacks uint32 on some builds
+        t = torch.as_tensor(arr)
+    if dtype == torch.float64 and t.dtype not in (torch.float64,):
+        if t.is_floating_point():
+            raise TypeError(f"{name}: cuspatial_amd supports float64 coordinates only "
+                            f"(got {t.dtype})")
+        raise RuntimeError("cuSpatial failure: Only floating-point types are supported")
+    if t.dtype != dtype:
+        t = t.to(dtype)
+    return t.to(device).contiguous()
+
+
+class Points:
+    """Point coordinates (the x / y columns of a point GeoSeries)."""
+
+    def __init__(self, x, y, device=None):
+        dev = _device(device if device is not None else
+                      (x.device if isinstance(x, torch.Tensor) and x.is_cuda else None))
+        self.x = _as_tensor(x, torch.float64, dev, "x")
+        self.y = _as_tensor(y, torch.float64, dev, "y")
+        self.device = dev
+
+    @classmethod
+    def from_points_xy(cls, xy, device=None) -> "Points":
+        """Interleaved [x0, y0, x1, y1, ...] (GeoSeries.from_points_xy)."""
+        t = xy if isinstance(xy, torch.Tensor) else torch.as_tensor(np.asarray(xy, np.float64))
+        t = t.reshape(-1, 2)
+        return cls(t[:, 0].contiguous(), t[:, 1].contiguous(), device=device)
+
+    def __len__(self):
+        return int(self.x.numel())
+
+
+class Polygons:
+    """Polygons as GeoArrow offsets (the part_offset / ring_offset / x / y of a
+    polygon GeoSeries, python/cuspatial/cuspatial/core/spatial/join.py:238-246)."""
+
+    def __init__(self, part_offset, ring_offset, x, y, geometry_offset=None, device=None):
+        dev = _device(device if device is not None else
+                      (x.device if isinstance(x, torch.Tensor) and x.is_cuda else None))
+        self.part_offset = _as_tensor(part_offset, torch.int32, dev, "part_offset")
+        self.ring_offset = _as_tensor(ring_offset, torch.int32, dev, "ring_offset")
+        self.x = _as_tensor(x, torch.float64, dev, "x")
+        self.y = _as_tensor(y, torch.float64, dev, "y")
+        self.geometry_offset = (None if geometry_offset is None else
+                                _as_tensor(geometry_offset, torch.int32, dev, "geometry_offset"))
+        self.device = dev
+
+    @classmethod
+    def from_polygons_xy(cls, xy, ring_offset, part_offset, geometry_offset=None,
+                         device=None) -> "Polygons":
+        """GeoSeries.from_polygons_xy(xy, ring_offset, part_offset, geometry_offset)."""
+        t = xy if isinstance(xy, torch.Tensor) else torch.as_tensor(np.asarray(xy, np.float64))
+        t = t.reshape(-1, 2)
+        return cls(part_offset, ring_offset, t[:, 0].contiguous(), t[:, 1].contiguous(),
+                   geometry_offset=geometry_offset, device=device)
+
+    def __len__(self):
+        return max(int(self.part_offset.numel()) - 1, 0)
+
+
+class Table(OrderedDict):
+    """An ordered set of equal-length device columns (what the reference returns
+    as a cudf.DataFrame)."""
+
+    def __len__(self):  # number of rows, like DataFrame.__len__
+        for v in self.values():
+            return int(v.numel())
+        return 0
+
+    @property
+    def columns(self):
+        return list(self.keys())
+
+    @property
+    def num_columns(self):
+        return OrderedDict.__len__(self)
+
+    def to_numpy(self) -> "OrderedDict[str, np.ndarray]":
+        return OrderedDict((k, v.cpu().numpy()) for k, v in self.items())
+
+    def to_pandas(self):
+        import pandas as pd
+
+        return pd.DataFrame(self.to_numpy())
+
+
+def _p(t: torch.Tensor | None):
+    return _vp(t.data_ptr()) if t is not None and t.numel() > 0 else None
+
+
+def _outs(n):
+    return [_vp() for _ in range(n)]
+
+
+# ---------------------------------------------------------------------------
+# Entry points
+# ---------------------------------------------------------------------------
+def quadtree_on_points(points: Points, x_min, x_max, y_min, y_max, scale, max_depth, max_size):
+    """Construct a quadtree from a set of points for a given area-of-interest
+    bounding box (indexing.py:15-199). Returns ``(point_indices, quadtree)``
+    where quadtree has columns key, level, is_internal_node, length, offset."""
+    if not isinstance(points, Points):
+        raise ValueError("GeoSeries must contain only points.")
+    # indexing.py:165-189: order the bbox, clamp the scale with a warning
+    x_min, x_max, y_min, y_max = (min(x_min, x_max), max(x_min, x_max),
+                                  min(y_min, y_max), max(y_min, y_max))
+    min_scale = max(x_max - x_min, y_max - y_min) / ((1 << max_depth) + 2)
+    if scale < min_scale:
+        warnings.warn(
+            "scale {} is less than required minimum ".format(scale)
+            + "scale {}. Clamping to minimum scale".format(min_scale))
+    dev = points.device
+    res = _capi.TorchResource(dev)
+    pi, key, level, internal, length, offset = _outs(6)
+    m = ctypes.c_int64()
+    L = _capi.lib()
+    _capi.check(L.qpip_quadtree_on_points(
+        _p(points.x), _p(points.y), points.x.numel(), points.y.numel(), float(x_min),
+        float(x_max), float(y_min), float(y_max), float(max(scale, min_scale)),
+        ctypes.c_int8(int(max_depth)), ctypes.c_int32(int(max_size)), ctypes.byref(res.struct),
+        _capi.stream_handle(dev), ctypes.byref(pi), ctypes.byref(key), ctypes.byref(level),
+        ctypes.byref(internal), ctypes.byref(length), ctypes.byref(offset), ctypes.byref(m)))
+    n = len(points)
+    M = m.value
+    point_indices = res.take(pi.value, n if M > 0 else 0, torch.uint32)
+    tree = Table()
+    tree["key"] = res.take(key.value, M, torch.uint32)
+    tree["level"] = res.take(level.value, M, torch.uint8)
+    tree["is_internal_node"] = res.take(internal.value, M, torch.bool)
+    tree["length"] = res.take(length.value, M, torch.uint32)
+    tree["offset"] = res.take(offset.value, M, torch.uint32)
+    return point_indices, tree
+
+
+def polygon_bounding_boxes(polygons: Polygons):
+    """Minimum bounding box of each polygon (bounding.py:19-80). Columns minx,
+    miny, maxx, maxy."""
+    names = ["minx", "miny", "maxx", "maxy"]
+    if len(polygons) == 0:
+        dev = polygons.device if isinstance(polygons, Polygons) else _device()
+        return Table((k, torch.empty(0, dtype=torch.float64, device=dev)) for k in names)
+    if not isinstance(polygons, Polygons):
+        raise ValueError("Geoseries must contain only polygons.")
+    dev = polygons.device
+    res = _capi.TorchResource(dev)
+    outs = _outs(4)
+    nb = ctypes.c_int64()
+    _capi.check(_capi.lib().qpip_polygon_bounding_boxes(
+        _p(polygons.part_offset), polygons.part_offset.numel(), _p(polygons.ring_offset),
+        polygons.ring_offset.numel(), _p(polygons.x), _p(polygons.y), polygons.x.numel(),
+        polygons.y.numel(), 0.0, ctypes.byref(res.struct), _capi.stream_handle(dev),
+        *[ctypes.byref(o) for o in outs], ctypes.byref(nb)))
+    return Table((k, res.take(o.value, nb.value, torch.float64)) for k, o in zip(names, outs))
+
+
+def _tree_columns(quadtree):
+    cols = [quadtree[k] for k in ("key", "level", "is_internal_node", "length", "offset")]
+    if len(cols) != 5:
+        raise RuntimeError("cuSpatial failure: quadtree table must have 5 columns")
+    key, level, internal, length, offset = cols
+    return (key.contiguous(), level.contiguous(), internal.to(torch.uint8).contiguous(),
+            length.contiguous(), offset.contiguous())
+
+
+def join_quadtree_and_bounding_boxes(quadtree, bounding_boxes, x_min, x_max, y_min, y_max,
+                                     scale, max_depth):
+    """Search a quadtree for bounding box intersections (join.py:105-175).
+    Returns columns bbox_offset, quad_offset."""
+    x_min, x_max, y_min, y_max = (min(x_min, x_max), max(x_min, x_max),
+                                  min(y_min, y_max), max(y_min, y_max))
+    min_scale = max(x_max - x_min, y_max - y_min) / ((1 << max_depth) + 2)
+    if scale < min_scale:
+        warnings.warn(
+            "scale {} is less than required minimum ".format(scale)
+            + "scale {}. Clamping to minimum scale".format(min_scale))
+    ncols = quadtree.num_columns if isinstance(quadtree, Table) else len(quadtree)
+    if ncols != 5:
+        raise RuntimeError("cuSpatial failure: quadtree table must have 5 columns")
+    bcols = list(bounding_boxes.values()) if isinstance(bounding_boxes, dict) else list(bounding_boxes)
+    if len(bcols) != 4:
+        raise RuntimeError("cuSpatial failure: bbox table must have 4 columns")
+    key, level, internal, length, offset = _tree_columns(quadtree)
+    dev = key.device
+    bcols = [_as_tensor(c, torch.float64, dev, "bbox") for c in bcols]
+    res = _capi.TorchResource(dev)
+    bo, qo = _outs(2)
+    k = ctypes.c_int64()
+    _capi.check(_capi.lib().qpip_join_quadtree_and_bounding_boxes(
+        _p(key), _p(level), _p(internal), _p(length), _p(offset), key.numel(),
+        *[_p(c) for c in bcols], bcols[0].numel(), float(x_min), float(x_max), float(y_min),
+        float(y_max), float(max(scale, min_scale)), ctypes.c_int8(int(max_depth)),
+        ctypes.byref(res.struct), _capi.stream_handle(dev), ctypes.byref(bo), ctypes.byref(qo),
+        ctypes.byref(k)))
+    out = Table()
+    out["bbox_offset"] = res.take(bo.value, k.value, torch.uint32)
+    out["quad_offset"] = res.take(qo.value, k.value, torch.uint32)
+    return out
+
+
+def quadtree_point_in_polygon(poly_quad_pairs, quadtree, point_indices, points: Points,
+                              polygons: Polygons):
+    """Test whether the specified points are inside any of the specified
+    polygons (join.py:178-262). Returns columns polygon_index, point_index;
+    point_index is a position in ``point_indices``."""
+    if not isinstance(points, Points):
+        raise ValueError("`point` Geoseries must contains only point geometries.")
+    if not isinstance(polygons, Polygons):
+        raise ValueError("`polygons` Geoseries must contains only polygons geometries.")
+    pcols = list(poly_quad_pairs.values()) if isinstance(poly_quad_pairs, dict) else list(poly_quad_pairs)
+    if len(pcols) != 2:
+        raise RuntimeError("cuSpatial failure: a quadrant-polygon table must have 2 columns")
+    key, level, internal, length, offset = _tree_columns(quadtree)
+    dev = points.device
+    pp = pcols[0].contiguous()
+    pq = pcols[1].contiguous()
+    pi = point_indices.contiguous()
+    res = _capi.TorchResource(dev)
+    po, pt = _outs(2)
+    h = ctypes.c_int64()
+    _capi.check(_capi.lib().qpip_quadtree_point_in_polygon(
+        _p(pp), _p(pq), pp.numel(), _p(key), _p(level), _p(internal), _p(length), _p(offset),
+        key.numel(), _p(pi), pi.numel(), _p(points.x), _p(points.y), points.x.numel(),
+        points.y.numel(), _p(polygons.part_offset), polygons.part_offset.numel(),
+        _p(polygons.ring_offset), polygons.ring_offset.numel(), _p(polygons.x),
+        _p(polygons.y), polygons.x.numel(), polygons.y.numel(), ctypes.byref(res.struct),
+        _capi.stream_handle(dev), ctypes.byref(po), ctypes.byref(pt), ctypes.byref(h)))
+    out = Table()
+    out["polygon_index"] = res.take(po.value, h.value, torch.uint32)
+    out["point_index"] = res.take(pt.value, h.value, torch.uint32)
+    return out

@@ -1,0 +1,215 @@
+/*
+ * cuspatial_amd — C ABI of the MI355X-native quadtree point-in-polygon join.
+ *
+ * Each entry point replaces one function of the reference's column API
+ * (rapidsai/cuspatial 25.06, cited as file:line relative to the reference root):
+ * the same inputs, in the same order and meaning, as plain device pointers and
+ * element counts instead of cudf::column_view, and the same output columns as
+ * device buffers instead of cudf::column.
+ *
+ * Conventions
+ *  - All array arguments are device pointers (HBM). Coordinates are float64;
+ *    the reference's float32 instantiation is not provided.
+ *  - `stream` is a hipStream_t (NULL: the default stream). Every entry point
+ *    returns after its work on `stream` is complete, as the reference's column
+ *    API does.
+ *  - Outputs are allocated from `mr` when it is non-NULL (the counterpart of the
+ *    reference's `rmm::device_async_resource_ref mr` argument); with mr == NULL
+ *    they come from the library's pool and are released with qpip_free().
+ *    Zero-length outputs are returned as NULL with a count of 0.
+ *  - Return value: QPIP_SUCCESS, or an error code with a message available from
+ *    qpip_last_error_message() on the calling thread. QPIP_ERROR_LOGIC
+ *    corresponds to cuspatial::logic_error (same reason strings),
+ *    QPIP_ERROR_DEVICE to cuspatial::cuda_error, QPIP_ERROR_OOM to
+ *    rmm::out_of_memory.
+ */
+#ifndef CUSPATIAL_AMD_H
+#define CUSPATIAL_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QPIP_SUCCESS      0
+#define QPIP_ERROR_LOGIC  1
+#define QPIP_ERROR_DEVICE 2
+#define QPIP_ERROR_OOM    3
+
+/* Caller-supplied device memory resource for outputs. */
+typedef struct qpip_resource {
+  void* (*allocate)(void* ctx, size_t bytes, void* stream);
+  void (*deallocate)(void* ctx, void* ptr, size_t bytes, void* stream);
+  void* ctx;
+} qpip_resource;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* qpip_last_error_message(void);
+
+/* Library version string. */
+const char* qpip_version(void);
+
+/* Releases a buffer returned with mr == NULL. NULL is ignored. */
+void qpip_free(void* ptr);
+
+/* Caps the bytes the library's pool may hold in live buffers (0: no cap).
+ * Allocations beyond the cap fail with QPIP_ERROR_OOM. */
+void qpip_set_memory_limit(size_t bytes);
+
+/* Returns cached, unused pool memory to the device. */
+void qpip_trim(void);
+
+/* Kernel timing (no counterpart in the reference; for benchmarks). While
+ * enabled, every kernel launch is bracketed by HIP events on its stream.
+ * qpip_profile_report writes {"kernel name": [launches, total ms], ...} as JSON
+ * into buf (truncated to len) and returns the full length. */
+void qpip_profile_enable(int on);
+void qpip_profile_reset(void);
+int64_t qpip_profile_report(char* buf, size_t len);
+
+/*
+ * quadtree_on_points
+ * Replaces cuspatial::quadtree_on_points
+ *   (cpp/include/cuspatial/point_quadtree.hpp:68-78, cpp/src/indexing/point_quadtree.cu:154-180;
+ *    algorithm detail/point_quadtree.cuh:238-272).
+ * Inputs x, y (num_x, num_y points). Clamps: bbox ordered, max_size >= 1,
+ * 0 <= max_depth <= 15, scale >= max(dx, dy) / ((1 << max_depth) + 2).
+ * Outputs: point_indices (UINT32, num_x), and the five quadtree columns
+ * key (UINT32), level (UINT8), is_internal_node (BOOL8 as uint8), length (UINT32),
+ * offset (UINT32), each *num_nodes long.
+ * Errors: "x and y columns must have the same length".
+ */
+int qpip_quadtree_on_points(const double* x,
+                            const double* y,
+                            int64_t num_x,
+                            int64_t num_y,
+                            double x_min,
+                            double x_max,
+                            double y_min,
+                            double y_max,
+                            double scale,
+                            int8_t max_depth,
+                            int32_t max_size,
+                            const qpip_resource* mr,
+                            void* stream,
+                            uint32_t** point_indices,
+                            uint32_t** key,
+                            uint8_t** level,
+                            uint8_t** is_internal_node,
+                            uint32_t** length,
+                            uint32_t** offset,
+                            int64_t* num_nodes);
+
+/*
+ * polygon_bounding_boxes
+ * Replaces cuspatial::polygon_bounding_boxes
+ *   (cpp/include/cuspatial/bounding_boxes.hpp:79-85,
+ *    cpp/src/bounding_boxes/polygon_bounding_boxes.cu:132-159;
+ *    algorithm detail/bounding_boxes.cuh:62-90,139-184).
+ * poly_offsets / ring_offsets are GeoArrow INT32 offsets (num_polygons + 1 and
+ * num_rings + 1 entries). Outputs minx, miny, maxx, maxy, *num_boxes =
+ * num_poly_offsets - 1 rows (0 when num_poly_offsets <= 1).
+ * Errors: "x and y must be the same size".
+ */
+int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
+                                int64_t num_poly_offsets,
+                                const int32_t* ring_offsets,
+                                int64_t num_ring_offsets,
+                                const double* x,
+                                const double* y,
+                                int64_t num_x,
+                                int64_t num_y,
+                                double expansion_radius,
+                                const qpip_resource* mr,
+                                void* stream,
+                                double** minx,
+                                double** miny,
+                                double** maxx,
+                                double** maxy,
+                                int64_t* num_boxes);
+
+/*
+ * join_quadtree_and_bounding_boxes
+ * Replaces cuspatial::join_quadtree_and_bounding_boxes
+ *   (cpp/include/cuspatial/spatial_join.hpp:66-75, cpp/src/join/quadtree_bbox_filtering.cu:90-126;
+ *    algorithm detail/join/quadtree_bbox_filtering.cuh:37-188).
+ * The quadtree is the five columns returned by qpip_quadtree_on_points; the
+ * bounding boxes are four columns of num_bboxes rows.
+ * Outputs bbox_offset, quad_offset (UINT32, *num_pairs), ordered by the leaf's
+ * first point position, then by traversal order.
+ * Errors: "scale must be positive", "invalid bounding box (x_min, x_max, y_min, y_max)",
+ * "maximum depth must be positive and less than 16".
+ */
+int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
+                                          const uint8_t* level,
+                                          const uint8_t* is_internal_node,
+                                          const uint32_t* length,
+                                          const uint32_t* offset,
+                                          int64_t num_nodes,
+                                          const double* bbox_minx,
+                                          const double* bbox_miny,
+                                          const double* bbox_maxx,
+                                          const double* bbox_maxy,
+                                          int64_t num_bboxes,
+                                          double x_min,
+                                          double x_max,
+                                          double y_min,
+                                          double y_max,
+                                          double scale,
+                                          int8_t max_depth,
+                                          const qpip_resource* mr,
+                                          void* stream,
+                                          uint32_t** bbox_offset,
+                                          uint32_t** quad_offset,
+                                          int64_t* num_pairs);
+
+/*
+ * quadtree_point_in_polygon
+ * Replaces cuspatial::quadtree_point_in_polygon
+ *   (cpp/include/cuspatial/spatial_join.hpp:116-126, cpp/src/join/quadtree_point_in_polygon.cu:143-191;
+ *    algorithm detail/join/quadtree_point_in_polygon.cuh:104-235).
+ * poly_quad_pairs = (poly_index, quad_index) from the join; the quadtree's
+ * length/offset columns (the other three are not read and may be NULL);
+ * point_indices from qpip_quadtree_on_points; points x, y; polygons as
+ * GeoArrow offsets read as uint32 (as the reference does) and vertices.
+ * Outputs polygon_index, point_index (UINT32, *num_hits); point_index is a
+ * position in point_indices. Order: pair order, then point order in the leaf.
+ * Errors: "number of points must be the same for both x and y columns",
+ * "numbers of vertices must be the same for both x and y columns".
+ */
+int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
+                                   const uint32_t* quad_index,
+                                   int64_t num_pairs,
+                                   const uint32_t* key,
+                                   const uint8_t* level,
+                                   const uint8_t* is_internal_node,
+                                   const uint32_t* length,
+                                   const uint32_t* offset,
+                                   int64_t num_nodes,
+                                   const uint32_t* point_indices,
+                                   int64_t num_point_indices,
+                                   const double* x,
+                                   const double* y,
+                                   int64_t num_x,
+                                   int64_t num_y,
+                                   const uint32_t* poly_offsets,
+                                   int64_t num_poly_offsets,
+                                   const uint32_t* ring_offsets,
+                                   int64_t num_ring_offsets,
+                                   const double* poly_x,
+                                   const double* poly_y,
+                                   int64_t num_poly_x,
+                                   int64_t num_poly_y,
+                                   const qpip_resource* mr,
+                                   void* stream,
+                                   uint32_t** polygon_index,
+                                   uint32_t** point_index,
+                                   int64_t* num_hits);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUSPATIAL_AMD_H */

@@ -1,0 +1,172 @@
+"""The C ABI without a GPU: the library loads, exports every symbol the header
+declares, and reproduces the reference's precondition errors and empty-input
+behaviour (both are decided before any device work)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "cuspatial_amd.h")
+
+
+@pytest.fixture(scope="module")
+def capi():
+    from cuspatial_amd import _capi
+
+    if not os.path.exists(_capi.LIB_PATH):
+        from cuspatial_amd import _build
+
+        _build.build()
+    return _capi
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qpip_[a-z_]+)\s*\(", text)))
+
+
+def test_exports_every_declared_symbol(capi):
+    L = capi.lib()
+    names = declared_symbols()
+    assert len(names) == 12
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(capi.EXPORTED)
+    assert "gfx950" in capi.version()
+
+
+def _outs(n):
+    return [ctypes.c_void_p() for _ in range(n)]
+
+
+def _qt(capi, nx, ny, **kw):
+    L = capi.lib()
+    outs = _outs(6)
+    m = ctypes.c_int64(-1)
+    rc = L.qpip_quadtree_on_points(None, None, nx, ny, kw.get("x_min", 0.0), 1.0, 0.0, 1.0, 1.0,
+                                   ctypes.c_int8(3), ctypes.c_int32(4), None, None,
+                                   *[ctypes.byref(o) for o in outs], ctypes.byref(m))
+    return rc, m.value, outs
+
+
+def test_quadtree_size_mismatch(capi):
+    rc, _, _ = _qt(capi, 3, 4)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    msg = capi.lib().qpip_last_error_message().decode()
+    assert msg.startswith("cuSpatial failure at: ")
+    assert msg.endswith("x and y columns must have the same length")
+    with pytest.raises(RuntimeError, match="x and y columns must have the same length"):
+        capi.check(rc)
+
+
+def test_quadtree_empty(capi):
+    rc, m, outs = _qt(capi, 0, 0)
+    assert rc == capi.QPIP_SUCCESS and m == 0
+    assert all(o.value is None for o in outs)
+
+
+def _join(capi, scale=1.0, xmin=0.0, xmax=1.0, ymin=0.0, ymax=1.0, depth=3, m=0, b=0):
+    L = capi.lib()
+    outs = _outs(2)
+    k = ctypes.c_int64(-1)
+    rc = L.qpip_join_quadtree_and_bounding_boxes(
+        None, None, None, None, None, m, None, None, None, None, b, xmin, xmax, ymin, ymax, scale,
+        ctypes.c_int8(depth), None, None, *[ctypes.byref(o) for o in outs], ctypes.byref(k))
+    return rc, k.value
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(scale=0.0), "scale must be positive"),
+    (dict(scale=-1.0), "scale must be positive"),
+    (dict(xmin=2.0), "invalid bounding box (x_min, x_max, y_min, y_max)"),
+    (dict(ymax=0.0), "invalid bounding box (x_min, x_max, y_min, y_max)"),
+    (dict(depth=0), "maximum depth must be positive and less than 16"),
+    (dict(depth=16), "maximum depth must be positive and less than 16"),
+])
+def test_join_preconditions(capi, kw, msg):
+    # quadtree_bbox_filtering.cu:102-106, in the reference's order
+    rc, _ = _join(capi, **kw)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert capi.lib().qpip_last_error_message().decode().endswith(msg)
+
+
+def test_join_empty(capi):
+    assert _join(capi, m=0, b=5) == (capi.QPIP_SUCCESS, 0)
+    assert _join(capi, m=5, b=0) == (capi.QPIP_SUCCESS, 0)
+
+
+def _pip(capi, pairs=0, nodes=0, npi=0, nx=0, ny=0, npo=0, nvx=0, nvy=0):
+    L = capi.lib()
+    outs = _outs(2)
+    h = ctypes.c_int64(-1)
+    rc = L.qpip_quadtree_point_in_polygon(
+        None, None, pairs, None, None, None, None, None, nodes, None, npi, None, None, nx, ny,
+        None, npo, None, 0, None, None, nvx, nvy, None, None, *[ctypes.byref(o) for o in outs],
+        ctypes.byref(h))
+    return rc, h.value
+
+
+def test_pip_preconditions(capi):
+    # quadtree_point_in_polygon.cu:158-161
+    rc, _ = _pip(capi, npi=3, nx=3, ny=4)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert capi.lib().qpip_last_error_message().decode().endswith(
+        "number of points must be the same for both x and y columns")
+    rc, _ = _pip(capi, npi=2, nx=3, ny=3)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    rc, _ = _pip(capi, nvx=5, nvy=4)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert capi.lib().qpip_last_error_message().decode().endswith(
+        "numbers of vertices must be the same for both x and y columns")
+
+
+@pytest.mark.parametrize("kw", [dict(pairs=0, nodes=3, npi=3, nx=3, ny=3, npo=2),
+                                dict(pairs=3, nodes=0, npi=3, nx=3, ny=3, npo=2),
+                                dict(pairs=3, nodes=3, npi=0, nx=0, ny=0, npo=2),
+                                dict(pairs=3, nodes=3, npi=3, nx=3, ny=3, npo=0)])
+def test_pip_empty(capi, kw):
+    # quadtree_point_in_polygon.cu:171-178
+    assert _pip(capi, **kw) == (capi.QPIP_SUCCESS, 0)
+
+
+def test_bbox_preconditions_and_empty(capi):
+    L = capi.lib()
+    outs = _outs(4)
+    nb = ctypes.c_int64(-1)
+    rc = L.qpip_polygon_bounding_boxes(None, 3, None, 3, None, None, 4, 5, 0.0, None, None,
+                                       *[ctypes.byref(o) for o in outs], ctypes.byref(nb))
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert capi.lib().qpip_last_error_message().decode().endswith("x and y must be the same size")
+    for npo in (0, 1):
+        rc = L.qpip_polygon_bounding_boxes(None, npo, None, 3, None, None, 4, 4, 0.0, None, None,
+                                           *[ctypes.byref(o) for o in outs], ctypes.byref(nb))
+        assert rc == capi.QPIP_SUCCESS and nb.value == 0
+    rc = L.qpip_polygon_bounding_boxes(None, 3, None, 0, None, None, 4, 4, 0.0, None, None,
+                                       *[ctypes.byref(o) for o in outs], ctypes.byref(nb))
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert capi.lib().qpip_last_error_message().decode().endswith(
+        "Polygon ring offsets must contain at least one (1) value")
+
+
+def test_python_scale_clamp_warning():
+    """indexing.py:172-177: the warning text the reference emits."""
+    import warnings
+
+    import cuspatial_amd.spatial as sp
+
+    x_min, x_max, y_min, y_max, scale, d = 0.0, 8.0, 0.0, 8.0, 0.01, 3
+    min_scale = max(x_max - x_min, y_max - y_min) / ((1 << d) + 2)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        try:
+            sp.join_quadtree_and_bounding_boxes(sp.Table(), [], x_min, x_max, y_min, y_max,
+                                                scale, d)
+        except RuntimeError:
+            pass
+    assert any(str(m.message) == "scale {} is less than required minimum scale {}. Clamping to "
+               "minimum scale".format(scale, min_scale) for m in w)
