@@ -219,6 +219,7 @@ struct ExpandOp {
   // Child quadrant boundaries of node `node`: b[q]..b[q+1] is quadrant q.
   __device__ __forceinline__ uint32_t split(uint64_t node, uint32_t b[5]) const
   {
+    if (node >= cap) return 0;  // only after an overflow (reported to the host)
     uint32_t const c = ncount[node];
     if (c <= max_size) return 0;
     uint32_t const k  = nkey[node];
@@ -246,6 +247,7 @@ struct ExpandOp {
   {
     uint64_t const b0   = begin();
     uint64_t const node = b0 + i;
+    if (node >= cap) return;
     uint32_t b[5];
     uint32_t const nc   = split(node, b);
     uint64_t const next = b0 + meta->sizes[level] + prefix;  // first child slot
@@ -278,6 +280,7 @@ __global__ void __launch_bounds__(64)
                      uint32_t const* __restrict__ ncount,
                      int d,
                      uint32_t max_size,
+                     uint64_t cap,
                      TreeMeta* __restrict__ meta)
 {
   if (threadIdx.x != 0) return;
@@ -288,6 +291,10 @@ __global__ void __launch_bounds__(64)
     total += meta->sizes[l];
   }
   meta->num_nodes = total;
+  if (total > cap) {
+    meta->overflow = 1;
+    return;
+  }
   uint32_t const lo0 = static_cast<uint32_t>(bottom_begin);
   uint32_t const hi0 = static_cast<uint32_t>(total);
   for (uint32_t e = 0; e < meta->num_marked; ++e) {
@@ -437,7 +444,7 @@ static int64_t quadtree_on_points_impl(double const* x,
                                                   stream, kScanBlocks, "tree_level_scan");
   }
   post_levels_kernel<<<1, 64, 0, stream>>>(nkey.get(), nstart.get(), ncount.get(), d,
-                                           static_cast<uint32_t>(max_size), meta.get());
+                                           static_cast<uint32_t>(max_size), cap, meta.get());
   QPIP_LAUNCH_CHECK();
   }
 
