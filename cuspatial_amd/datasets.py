@@ -45,7 +45,7 @@ class Workload:
 
     @property
     def num_points(self) -> int:
-        return int(self.x.size)
+        return int(self.x.shape[0])
 
     @property
     def num_polygons(self) -> int:

@@ -178,9 +178,14 @@ def test_random_parity(cs, oracle_mod, seed, n, d, max_size):
     bbox = (0.0, 100.0, 0.0, 100.0)
     scale = 100.0 / (1 << d)
     print(f"[parity] seed={seed} n={n} d={d} max_size={max_size}", flush=True)
-    g = run_gpu(cs, x, y, bbox, scale, d, max_size, polys)
-    o = run_oracle(oracle_mod, x, y, bbox, scale, d, max_size, polys)
+    # the join requires 0 < max_depth < 16 (quadtree_bbox_filtering.cu:104-105)
+    p = polys if d > 0 else None
+    g = run_gpu(cs, x, y, bbox, scale, d, max_size, p)
+    o = run_oracle(oracle_mod, x, y, bbox, scale, d, max_size, p)
     assert_same(g, o)
+    if d == 0:
+        with pytest.raises(RuntimeError, match="maximum depth must be positive"):
+            run_gpu(cs, x, y, bbox, scale, d, max_size, polys)
 
 
 @pytest.mark.parametrize("seed,d,max_size", [(21, 2, 1), (22, 3, 2), (23, 1, 1), (24, 6, 4),
