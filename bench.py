@@ -60,6 +60,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (the known "
+                         "traffic that scales the HBM counters) and exit")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -75,6 +78,14 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration << 20, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {args.copy_calibration} MiB three times")
+        return
     log(f"[bench] generating gaussian_holes with {args.points} points on {dev}")
     w = datasets.gaussian_holes(num_points=args.points, device=dev)
     bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
@@ -124,8 +135,11 @@ def main():
     candidates = float(cand.sum())
     work = {
         # name: (bound, algorithmic units per launch, unit)
+        # keygen reads x, y and writes the key
         "keygen": ("hbm", n * (16 + 4) / 1e9, "GB"),
+        # the digit count of passes 1.. reads the keys
         "point_sort_count": ("hbm", n * 4 / 1e9, "GB"),
+        # pass 0 reads keys and writes keys + indices; later passes move both
         "point_sort_scatter": ("hbm", n * (12 + 16 * (passes - 1)) / passes / 1e9, "GB"),
         "pip_test": ("fp64", 6.0 * edge_tests / 1e12, "TFLOP"),
     }
@@ -181,7 +195,7 @@ def main():
 
         m = min(n, 1_000_000)
         dt = run_oracle(m)
-        m2 = int(min(n, 20_000_000, max(m, m * args.cpu_seconds / max(dt, 1e-3))))
+        m2 = int(min(n, 60_000_000, max(m, m * args.cpu_seconds / max(dt, 1e-3))))
         if m2 > m:
             m, dt = m2, run_oracle(m2)
         cpu = {"value": m / dt, "unit": "points/s", "cores": threads, "kind": "port",

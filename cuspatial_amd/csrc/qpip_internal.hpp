@@ -254,27 +254,16 @@ __device__ __forceinline__ T block_excl_scan(T v, T* lds, T* total)
 
 // ----------------------------------------------------------------------------
 // Radix sort (qpip_sort.hip).
-// Stable LSD sort of (key, value) pairs on bits [0, bits) of the key.
-// values_in == nullptr means values are the identity 0..n-1.
-// Results are in *keys_out / *values_out (which point into the given buffers).
+// Stable LSD sort of (key, value) pairs on bits [0, bits) of the key, 8-bit digits.
+// keys_a holds the input keys and, with keys_b, is overwritten; values_in == nullptr
+// means the values are the identity 0..n-1; the sorted values always end in
+// values_out (values_tmp is scratch).
+//
+// Every pass works on tiles of kSortTile consecutive pairs (one workgroup each).
+// The first pass's digit counts per tile (tiles x 256) may come from the producer
+// of the keys (`first_counts`, e.g. the keygen kernel; they are overwritten); they
+// are counted here otherwise.
 // ----------------------------------------------------------------------------
-struct SortResult {
-  uint32_t* keys;
-  uint32_t* values;
-};
-SortResult radix_sort_pairs(uint32_t* keys_a,
-                            uint32_t* keys_b,
-                            uint32_t const* values_in,
-                            uint32_t* values_a,
-                            uint32_t* values_b,
-                            uint64_t n,
-                            int bits,
-                            hipStream_t stream,
-                            uint32_t* pass0_counts = nullptr,
-                            char const* tag        = "sort");
-
-// Tile geometry shared by the keygen kernel (which emits the first pass's tile
-// histograms) and the sort.
 constexpr int kSortThreads = 256;
 constexpr int kSortItems   = 16;
 constexpr int kSortTile    = kSortThreads * kSortItems;
@@ -282,6 +271,23 @@ constexpr int kRadixBits   = 8;
 constexpr int kRadix       = 1 << kRadixBits;
 
 inline uint64_t sort_tiles(uint64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+struct SortResult {
+  uint32_t* keys;
+  uint32_t* values;
+};
+int radix_passes(int bits);
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
+SortResult radix_sort_pairs(uint32_t* keys_a,
+                            uint32_t* keys_b,
+                            uint32_t const* values_in,
+                            uint32_t* values_out,
+                            uint32_t* values_tmp,
+                            uint64_t n,
+                            int bits,
+                            hipStream_t stream,
+                            uint32_t* first_counts       = nullptr,
+                            char const* tag              = "sort");
 
 inline unsigned grid_for(uint64_t n, int threads, unsigned cap = 1u << 20)
 {

@@ -392,8 +392,8 @@ extern "C" int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
     QPIP_LAUNCH_CHECK();
     int bits = 1;
     while (bits < 32 && (h_meta.max_leaf_offset >> bits) != 0) ++bits;
-    SortResult s = radix_sort_pairs(ka.get(), kb.get(), nullptr, va.get(), vb.get(), K, bits, stream,
-                                    nullptr, "join_sort");
+    SortResult s = radix_sort_pairs(ka.get(), kb.get(), nullptr, va.get(), vb.get(), K, bits,
+                                    stream, nullptr, "join_sort");
     OutputAllocator oa(mr, stream);
     uint32_t* o_bbox = oa.make<uint32_t>(K);
     uint32_t* o_quad = oa.make<uint32_t>(K);

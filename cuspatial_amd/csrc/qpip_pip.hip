@@ -16,10 +16,13 @@
 //      forced to false if any edge is an on-edge hit (the reference breaks out and
 //      returns false at the first such edge).
 //   2. Chunks. Pair k covers ceil(length / 64) chunks of 64 consecutive point
-//      positions; a scan gives each pair its first chunk.
-//   3. Test: one wave per chunk. The lanes read 64 consecutive point_indices
-//      entries (coalesced) and gather their coordinates; the polygon's edges are
-//      wave-uniform and are read through the scalar cache. The FP64 arithmetic is
+//      positions; a scan gives each pair its first chunk, and a map gives each
+//      chunk its pair (get_quad_and_local_point_indices.cuh:26-45 searches for it).
+//   3. Test: one wave per leaf, for all the pairs of that leaf (they are adjacent
+//      in pair order), up to 256 points (4 chunks) at a time. The lanes read
+//      consecutive point_indices entries (coalesced) and gather the coordinates
+//      once; each polygon's edges are wave-uniform and are read through the
+//      scalar cache. The FP64 arithmetic is
 //      the reference's; the 4-ULP collinearity test is preceded by an exact
 //      floating-point filter that rejects pairs of products too far apart to be
 //      within 4 ULP. Each wave stores its 64-bit hit ballot and its popcount.
@@ -133,74 +136,53 @@ struct PipArgs {
   Edge const* edges;
   EdgeB const* edges_b;
   uint32_t const* chunk_start;
+  uint32_t const* chunk_pair;
   uint64_t num_chunks;
+  uint64_t num_nodes;
 };
 
-// Pair owning chunk c: the last k with chunk_start[k] <= c (get_quad_and_local_point_indices.cuh:26-45).
-__device__ __forceinline__ uint32_t pair_of_chunk(uint32_t const* chunk_start, uint64_t K, uint32_t c)
+// chunk -> owning pair, so that each wave of the test finds its pair with one load
+__global__ void __launch_bounds__(256)
+  chunk_pair_kernel(uint32_t const* __restrict__ chunk_start, uint64_t K, uint64_t T,
+                    uint32_t* __restrict__ chunk_pair)
 {
-  uint32_t lo = 0, hi = static_cast<uint32_t>(K);
-  while (lo < hi) {
-    uint32_t const mid = lo + ((hi - lo) >> 1);
-    if (chunk_start[mid] <= c)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo - 1;
+  uint64_t const k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= K) return;
+  uint32_t const c0 = chunk_start[k];
+  uint64_t const c1 = k + 1 < K ? chunk_start[k + 1] : T;
+  for (uint64_t c = c0; c < c1; ++c) chunk_pair[c] = static_cast<uint32_t>(k);
 }
 
 // --- 3. test ------------------------------------------------------------------------
-__global__ void __launch_bounds__(256)
-  pip_test_kernel(PipArgs a, uint64_t* __restrict__ masks, uint32_t* __restrict__ hit_counts,
-                  uint32_t* __restrict__ bad)
+// One wave per run of pairs that share a leaf (the join orders pairs by leaf offset,
+// so such pairs are adjacent): the wave of the run's first pair gathers the leaf's
+// points up to 4 chunks (256 points, U per lane) at a time and tests them against
+// every polygon of the run; the waves of the run's other pairs return at once.
+// Masks are stored per (pair, chunk) in pair order: chunk j of the run's i-th pair
+// is chunk_start[first pair] + i * chunks + j.
+//
+// Collinearity filter. If float_equal(rhs, lhs) (4 ULP), then for normal values
+// |rhs - lhs| is exact (Sterbenz) and below 2^-50 * max(|rhs|, |lhs|) <= 2^-48 *
+// |rhs|, and near zero it is below 8 * 2^-1074. So fma(|rhs|, -2^-48, |rhs - lhs|)
+// <= 2^-1020 holds for every float_equal pair (NaN compares as "maybe"); lanes
+// where some edge passes are re-examined exactly in a second, rare pass.
+//
+// Lane state is kept as wave-uniform 64-bit masks (one bit per lane and point
+// slot), so the loop's bookkeeping is a handful of scalar ops per edge.
+template <int U>
+__device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, double const (&px)[4],
+                                             double const (&py)[4], uint64_t const (&valid_m)[4],
+                                             uint64_t (&hit_m)[4], bool& err)
 {
-  uint32_t const c = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-  if (c >= a.num_chunks) return;
-  uint32_t const k     = uniform(pair_of_chunk(a.chunk_start, a.num_pairs, c));
-  uint32_t const poly  = uniform(a.pair_poly[k]);
-  uint32_t const quad  = uniform(a.pair_quad[k]);
-  uint32_t const len   = uniform(a.length[quad]);
-  uint32_t const first = uniform(a.offset[quad]);
-  uint32_t const j     = c - a.chunk_start[k];
-  uint32_t const local = j * kWave + lane_id();
-  bool valid           = local < len;
-  // Errors are recorded at the end: a global store before the edge loads would
-  // keep them off the scalar unit (the compiler must assume it may alias).
-  bool err = false;
-  double px = 0.0, py = 0.0;
-  uint32_t pos = first + local;
-  if (valid) {
-    if (pos >= a.num_points) {
-      err   = true;
-      valid = false;
-    } else {
-      uint32_t const idx = a.point_indices[pos];
-      if (idx >= a.num_points) {
-        err   = true;
-        valid = false;
-      } else {
-        px = a.x[idx];
-        py = a.y[idx];
-      }
-    }
-  }
-  // Collinearity filter. If float_equal(rhs, lhs) (4 ULP), then for normal
-  // values |rhs - lhs| is exact (Sterbenz) and below 2^-50 * max(|rhs|, |lhs|)
-  // <= 2^-48 * |rhs|, and near zero it is below 8 * 2^-1074. So
-  // fma(|rhs|, -2^-48, |rhs - lhs|) <= 2^-1020 holds for every float_equal pair
-  // (NaN compares as "maybe"); lanes where some edge passes are re-examined
-  // exactly in a second, rare pass.
-  //
-  // Lane state is kept as wave-uniform 64-bit masks (one bit per lane), so the
-  // loop's bookkeeping is a handful of scalar ops per edge.
-  uint64_t inside_m = 0, suspect_m = 0;
+  uint64_t inside_m[U], suspect_m[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) inside_m[u] = suspect_m[u] = 0;
   uint32_t r0 = 0, r1 = 0;
   if (poly < a.num_polys) {
     r0 = uniform(a.poly_offsets[poly]);
     r1 = uniform(a.poly_offsets[poly + 1]);
     if (r1 > a.num_rings) r1 = static_cast<uint32_t>(a.num_rings);
-  } else if (valid) {
+  } else {
     err = true;
   }
   for (uint32_t r = r0; r < r1; ++r) {
@@ -210,52 +192,123 @@ __global__ void __launch_bounds__(256)
     Edge const* E = a.edges + e0;
     // y0 of the first edge is its `b` (the ring's last vertex); afterwards the
     // previous edge's `a` (is_point_in_polygon.cuh:339, :375-376).
-    uint64_t y0 = ballot(a.edges_b[e0].by > py);
+    double const by0 = a.edges_b[e0].by;
+    uint64_t y0[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) y0[u] = ballot(by0 > py[u]);
     auto edge = [&](Edge const& ed) {
-      double const lhs  = (px - ed.ax) * ed.rise;  // run_to_point * rise
-      double const rhs  = ed.run * (py - ed.ay);   // run * rise_to_point
-      double const gap  = fma(fabs(rhs), -0x1p-48, fabs(rhs - lhs));
-      uint64_t const y1 = ballot(ed.ay > py);
-      uint64_t const lt = ballot(lhs < rhs);
-      suspect_m |= ballot(!(gap > 0x1p-1020));
-      inside_m ^= (y1 ^ y0) & (lt ^ y1);
-      y0 = y1;
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        double const lhs  = (px[u] - ed.ax) * ed.rise;  // run_to_point * rise
+        double const rhs  = ed.run * (py[u] - ed.ay);   // run * rise_to_point
+        double const gap  = fma(fabs(rhs), -0x1p-48, fabs(rhs - lhs));
+        uint64_t const y1 = ballot(ed.ay > py[u]);
+        uint64_t const lt = ballot(lhs < rhs);
+        suspect_m[u] |= ballot(!(gap > 0x1p-1020));
+        inside_m[u] ^= (y1 ^ y0[u]) & (lt ^ y1);
+        y0[u] = y1;
+      }
     };
-    // Unrolled by 4 so that the loop's scalar bookkeeping is amortised.
     uint32_t e = 0;
-    for (; e + 4 <= ne; e += 4) {
-      edge(E[e]);
-      edge(E[e + 1]);
-      edge(E[e + 2]);
-      edge(E[e + 3]);
+    if (U <= 2) {  // unrolled so that the loop's scalar bookkeeping is amortised
+      for (; e + 2 <= ne; e += 2) {
+        edge(E[e]);
+        edge(E[e + 1]);
+      }
     }
     for (; e < ne; ++e) edge(E[e]);
   }
-  int const lane     = lane_id();
-  bool const inside  = (inside_m >> lane) & 1ull;
-  bool const suspect = ((suspect_m >> lane) & 1ull) && valid;
-  bool on_edge       = false;
-  if (ballot(suspect)) {
-    for (uint32_t r = r0; r < r1; ++r) {
-      uint32_t const e0 = uniform(a.ring_offsets[r]);
-      uint32_t const ne = uniform(a.ring_edges[r]);
-      for (uint32_t e = 0; e < ne; ++e) {
-        Edge const E    = a.edges[e0 + e];
-        double const bx = a.edges_b[e0 + e].bx;
-        // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
-        if (suspect && float_equal(E.run * (py - E.ay), (px - E.ax) * E.rise)) {
-          double const minx = E.ax < bx ? E.ax : bx;
-          double const maxx = E.ax < bx ? bx : E.ax;
-          if (minx <= px && px <= maxx) on_edge = true;
+  int const lane = lane_id();
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    uint64_t const sus = suspect_m[u] & valid_m[u];
+    uint64_t on_m      = 0;
+    if (sus) {
+      bool const suspect = (sus >> lane) & 1ull;
+      bool on_edge       = false;
+      for (uint32_t r = r0; r < r1; ++r) {
+        uint32_t const e0 = uniform(a.ring_offsets[r]);
+        uint32_t const ne = uniform(a.ring_edges[r]);
+        for (uint32_t e = 0; e < ne; ++e) {
+          Edge const E    = a.edges[e0 + e];
+          double const bx = a.edges_b[e0 + e].bx;
+          // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
+          if (suspect && float_equal(E.run * (py[u] - E.ay), (px[u] - E.ax) * E.rise)) {
+            double const minx = E.ax < bx ? E.ax : bx;
+            double const maxx = E.ax < bx ? bx : E.ax;
+            if (minx <= px[u] && px[u] <= maxx) on_edge = true;
+          }
         }
       }
+      on_m = ballot(on_edge);
     }
+    hit_m[u] = inside_m[u] & ~on_m & valid_m[u];
   }
-  bool const hit       = valid && inside && !on_edge;
-  uint64_t const mask  = ballot(hit);
-  if (lane_id() == 0) {
-    masks[c]      = mask;
-    hit_counts[c] = static_cast<uint32_t>(__popcll(mask));
+}
+
+__global__ void __launch_bounds__(256)
+  pip_test_kernel(PipArgs a, uint64_t* __restrict__ masks, uint32_t* __restrict__ hit_counts,
+                  uint32_t* __restrict__ bad)
+{
+  uint32_t const k0 = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  if (k0 >= a.num_pairs) return;
+  uint32_t const quad = uniform(a.pair_quad[k0]);
+  if (k0 > 0 && uniform(a.pair_quad[k0 - 1]) == quad) return;  // not the run's first pair
+  // Errors are recorded at the end: a global store before the edge loads would
+  // keep them off the scalar unit (the compiler must assume it may alias).
+  bool err = false;
+  if (quad >= a.num_nodes) {
+    if (lane_id() == 0) atomicOr(bad, 1u);
+    return;
+  }
+  uint32_t const len    = uniform(a.length[quad]);
+  uint32_t const first  = uniform(a.offset[quad]);
+  uint32_t const chunks = (len + kWave - 1) / kWave;
+  uint64_t const slot0  = uniform(a.chunk_start[k0]);
+  int const lane        = lane_id();
+  for (uint32_t j0 = 0; j0 < chunks; j0 += 4) {
+    uint32_t const nu = chunks - j0 < 4 ? chunks - j0 : 4;
+    double px[4] = {0.0, 0.0, 0.0, 0.0}, py[4] = {0.0, 0.0, 0.0, 0.0};
+    uint64_t valid_m[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      uint32_t const local = (j0 + u) * kWave + lane;
+      bool valid           = local < len;
+      uint32_t const pos   = first + local;
+      if (valid) {
+        if (pos >= a.num_points) {
+          err   = true;
+          valid = false;
+        } else {
+          uint32_t const idx = a.point_indices[pos];
+          if (idx >= a.num_points) {
+            err   = true;
+            valid = false;
+          } else {
+            px[u] = a.x[idx];
+            py[u] = a.y[idx];
+          }
+        }
+      }
+      valid_m[u] = ballot(valid);
+    }
+    for (uint32_t k = k0; k < a.num_pairs; ++k) {
+      if (k > k0 && uniform(a.pair_quad[k]) != quad) break;
+      uint32_t const poly = uniform(a.pair_poly[k]);
+      uint64_t hit_m[4]   = {0, 0, 0, 0};
+      switch (nu) {
+        case 1: test_polygon<1>(a, poly, px, py, valid_m, hit_m, err); break;
+        case 2: test_polygon<2>(a, poly, px, py, valid_m, hit_m, err); break;
+        case 3: test_polygon<3>(a, poly, px, py, valid_m, hit_m, err); break;
+        default: test_polygon<4>(a, poly, px, py, valid_m, hit_m, err); break;
+      }
+      uint64_t const slot = slot0 + static_cast<uint64_t>(k - k0) * chunks + j0;
+      if (lane < static_cast<int>(nu)) {
+        uint64_t const m = lane == 0 ? hit_m[0] : lane == 1 ? hit_m[1] : lane == 2 ? hit_m[2] : hit_m[3];
+        masks[slot + lane]      = m;
+        hit_counts[slot + lane] = static_cast<uint32_t>(__popcll(m));
+      }
+    }
   }
   if (err) atomicOr(bad, 1u);
 }
@@ -277,7 +330,7 @@ __global__ void __launch_bounds__(256)
   if (c >= a.num_chunks) return;
   uint64_t const mask = masks[c];
   if (mask == 0) return;
-  uint32_t const k     = uniform(pair_of_chunk(a.chunk_start, a.num_pairs, c));
+  uint32_t const k     = uniform(a.chunk_pair[c]);
   uint32_t const poly  = a.pair_poly[k];
   uint32_t const quad  = a.pair_quad[k];
   uint32_t const first = a.offset[quad];
@@ -377,17 +430,26 @@ extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
     if (T == 0) return QPIP_SUCCESS;
     QPIP_EXPECTS(T < (uint64_t{1} << 31), "too many candidate points");
 
+    dbuf<uint32_t> chunk_pair(T);
+    {
+      KernelTimer kt("pip_chunks", stream);
+      chunk_pair_kernel<<<grid_for(K, 256), 256, 0, stream>>>(chunk_start.get(), K, T,
+                                                              chunk_pair.get());
+      QPIP_LAUNCH_CHECK();
+    }
     PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
                  static_cast<uint64_t>(num_x), x, y, poly_offsets,
                  static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
-                 ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), T};
+                 ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(),
+                 chunk_pair.get(), T, static_cast<uint64_t>(num_nodes)};
     // 3. test
     dbuf<uint64_t> masks(T);
     dbuf<uint32_t> hit_counts(T);
     unsigned const blocks = static_cast<unsigned>((T + 3) / 4);
     {
       KernelTimer kt("pip_test", stream);
-      pip_test_kernel<<<blocks, 256, 0, stream>>>(args, masks.get(), hit_counts.get(), bad.get());
+      pip_test_kernel<<<static_cast<unsigned>((K + 3) / 4), 256, 0, stream>>>(
+        args, masks.get(), hit_counts.get(), bad.get());
       QPIP_LAUNCH_CHECK();
     }
     // 4. scan

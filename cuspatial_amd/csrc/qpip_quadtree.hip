@@ -10,7 +10,7 @@
 // sorts. This implementation computes the same five columns top-down from the
 // sorted keys and never materialises pruned nodes:
 //
-//   1. keygen (fused with pass 0's tile histogram of the radix sort);
+//   1. keygen (fused with the radix sort's first digit count);
 //   2. stable LSD radix sort of (key, point index) over the 2d+2 (at most 32)
 //      key bits that can be non-zero;
 //   3. level 0: the distinct values of key >> 2(d-1), found by binary search;
@@ -95,9 +95,21 @@ __device__ __forceinline__ uint32_t lower_bound_u32(uint32_t const* K, uint32_t 
 }
 
 // --- 1. keygen: compute_point_keys_and_sorted_indices (phase_1.cuh:62-95) -------
-// Tile layout is the radix sort's (256 threads x 16 items, wave-striped) so the
-// digit-0 histogram of each tile can be written here.
-__global__ void __launch_bounds__(kSortThreads)
+// One block per radix-sort tile (4096 points). The block also counts the sort's
+// first digit (key bits 0-7) into per-wave LDS histograms, so that the sort skips
+// its first counting pass. Full tiles of 16-byte aligned inputs are read as
+// double2 and written as uint4 (thread t owns points 4t..4t+3 of each quarter).
+__device__ __forceinline__ uint32_t point_key(double x, double y, double min_x, double min_y,
+                                              double max_x, double max_y, double scale,
+                                              uint32_t outside_key)
+{
+  if (x < min_x || x > max_x || y < min_y || y > max_y) return outside_key;
+  uint32_t const cx = to_u16((x - min_x) / scale);
+  uint32_t const cy = to_u16((y - min_y) / scale);
+  return (dilate_2(cy) << 1) | dilate_2(cx);
+}
+
+__global__ void __launch_bounds__(256)
   keygen_kernel(double const* __restrict__ xs,
                 double const* __restrict__ ys,
                 uint64_t n,
@@ -107,38 +119,67 @@ __global__ void __launch_bounds__(kSortThreads)
                 double max_y,
                 double scale,
                 uint32_t outside_key,
-                uint32_t digit_mask,
+                bool vec,
                 uint32_t* __restrict__ keys,
-                uint32_t* __restrict__ counts,
-                uint64_t tiles)
+                uint32_t* __restrict__ counts)
 {
-  __shared__ uint32_t hist[kRadix];
-  for (int i = threadIdx.x; i < kRadix; i += kSortThreads) hist[i] = 0;
+  constexpr int NW = 256 / kWave;
+  __shared__ uint32_t h[NW][kRadix];
+  int const w = threadIdx.x / kWave;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) h[ww][threadIdx.x] = 0;
   __syncthreads();
-  uint64_t const tile = blockIdx.x;
-  int const w         = threadIdx.x / kWave;
-  uint64_t const base = tile * kSortTile + static_cast<uint64_t>(w) * (kSortItems * kWave);
-#pragma unroll 4
-  for (int j = 0; j < kSortItems; ++j) {
-    uint64_t const idx = base + static_cast<uint64_t>(j) * kWave + lane_id();
-    bool const valid   = idx < n;
-    uint32_t key       = 0;
-    if (valid) {
-      double const x = xs[idx];
-      double const y = ys[idx];
-      if (x < min_x || x > max_x || y < min_y || y > max_y) {
-        key = outside_key;
-      } else {
-        uint32_t const cx = to_u16((x - min_x) / scale);
-        uint32_t const cy = to_u16((y - min_y) / scale);
-        key               = (dilate_2(cy) << 1) | dilate_2(cx);
-      }
-      keys[idx] = key;
+  uint64_t const lo = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+  if (vec && lo + kSortTile <= n) {
+    constexpr int Q = kSortItems / 4;
+    double2 const* x2 = reinterpret_cast<double2 const*>(xs + lo);
+    double2 const* y2 = reinterpret_cast<double2 const*>(ys + lo);
+    double2 xv[2 * Q], yv[2 * Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      int const e   = (j * 256 + threadIdx.x) * 2;  // double2 index of point 4 * (j * 256 + t)
+      xv[2 * j]     = x2[e];
+      xv[2 * j + 1] = x2[e + 1];
+      yv[2 * j]     = y2[e];
+      yv[2 * j + 1] = y2[e + 1];
     }
-    wave_hist_add(hist, key & digit_mask, valid);
+    uint4* k4 = reinterpret_cast<uint4*>(keys + lo);
+#pragma unroll
+    for (int j = 0; j < Q; ++j) {
+      uint4 k;
+      k.x = point_key(xv[2 * j].x, yv[2 * j].x, min_x, min_y, max_x, max_y, scale, outside_key);
+      k.y = point_key(xv[2 * j].y, yv[2 * j].y, min_x, min_y, max_x, max_y, scale, outside_key);
+      k.z = point_key(xv[2 * j + 1].x, yv[2 * j + 1].x, min_x, min_y, max_x, max_y, scale,
+                      outside_key);
+      k.w = point_key(xv[2 * j + 1].y, yv[2 * j + 1].y, min_x, min_y, max_x, max_y, scale,
+                      outside_key);
+      k4[j * 256 + threadIdx.x] = k;
+      wave_count_digit(h[w], k.x & 0xFFu);
+      wave_count_digit(h[w], k.y & 0xFFu);
+      wave_count_digit(h[w], k.z & 0xFFu);
+      wave_count_digit(h[w], k.w & 0xFFu);
+    }
+  } else {
+    double x[kSortItems], y[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+      uint64_t const i = lo + static_cast<uint64_t>(j) * 256 + threadIdx.x;
+      x[j]             = i < n ? xs[i] : 0.0;
+      y[j]             = i < n ? ys[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+      uint64_t const i   = lo + static_cast<uint64_t>(j) * 256 + threadIdx.x;
+      uint32_t const key = point_key(x[j], y[j], min_x, min_y, max_x, max_y, scale, outside_key);
+      if (i < n) keys[i] = key;
+      wave_count_digit(h[w], i < n ? (key & 0xFFu) : static_cast<uint32_t>(kRadix));
+    }
   }
   __syncthreads();
-  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[d * tiles + tile] = hist[d];
+  uint32_t s = 0;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) s += h[ww][threadIdx.x];
+  counts[static_cast<uint64_t>(blockIdx.x) * kRadix + threadIdx.x] = s;
 }
 
 // --- 3. level 0 and the root-level quirk ----------------------------------------
@@ -194,8 +235,23 @@ __global__ void __launch_bounds__(64)
   }
 }
 
-// --- 4. level expansion (one level, count/scan/emit) ----------------------------
-struct ExpandOp {
+// --- 4. level expansion: two launches per level --------------------------------
+// level_split_kernel finds, for every node of level l with more than max_size
+// points (l <= d-2: the reference's internal nodes), the boundaries of its four
+// child quadrants inside its sorted point range, stores them with the number of
+// non-empty quadrants, and sums those per block. level_emit_kernel gives each
+// block the sum of the blocks before it, scans its nodes' child counts and appends
+// the children in (parent, quadrant) order: level-major and key order within a
+// level, the order of reverse_tree_levels.
+//
+// Node-to-block assignment (both kernels): a level of at most 4 * kLevelBlocks
+// nodes is searched one wave per node (block b: nodes 4b .. 4b+3), 21 lanes per
+// boundary, each probing one position, so the interval shrinks 22-fold per round
+// trip to memory. Larger levels give each block a contiguous chunk and each thread
+// one node at a time (three interleaved binary searches).
+constexpr int kLevelBlocks = 2048;
+
+struct LevelArgs {
   uint32_t const* K;
   uint32_t* nkey;
   uint32_t* nstart;
@@ -203,75 +259,218 @@ struct ExpandOp {
   uint8_t* nlevel;
   uint32_t* cfirst;
   uint32_t* ccount;
+  uint4* split;  // {b1, b2, b3, non-empty quadrants} per node
+  uint32_t* partials;
   TreeMeta* meta;
   uint64_t cap;
   int level;
   int d;
   uint32_t max_size;
+};
 
-  __device__ __forceinline__ uint64_t begin() const
-  {
-    uint64_t b = 0;
-    for (int l = 0; l < level; ++l) b += meta->sizes[l];
-    return b;
+__device__ __forceinline__ uint64_t level_begin(TreeMeta const* meta, int level)
+{
+  uint64_t b = 0;
+  for (int l = 0; l < level; ++l) b += meta->sizes[l];
+  return b;
+}
+
+__device__ __forceinline__ bool wave_mode(uint64_t S) { return S <= 4ull * kLevelBlocks; }
+
+__device__ __forceinline__ void node_chunk(uint64_t S, uint64_t b, uint64_t& lo, uint64_t& hi)
+{
+  if (wave_mode(S)) {
+    lo = 4 * b;
+    hi = lo + 4;
+    if (lo > S) lo = S;
+    if (hi > S) hi = S;
+  } else {
+    chunk_range(S, b, kLevelBlocks, lo, hi);
   }
+}
 
-  // Child quadrant boundaries of node `node`: b[q]..b[q+1] is quadrant q.
-  __device__ __forceinline__ uint32_t split(uint64_t node, uint32_t b[5]) const
-  {
-    if (node >= cap) return 0;  // only after an overflow (reported to the host)
-    uint32_t const c = ncount[node];
-    if (c <= max_size) return 0;
-    uint32_t const k  = nkey[node];
-    uint32_t const s  = nstart[node];
-    uint32_t const e  = s + c;
-    int const sh      = 2 * (d - 2 - level);
-    b[0]              = s;
-    b[4]              = e;
-    uint32_t nonempty = 0;
-#pragma unroll
-    for (int q = 1; q < 4; ++q)
-      b[q] = lower_bound_u32(K, b[q - 1], e, (static_cast<uint64_t>(k) * 4 + q) << sh);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) nonempty += b[q + 1] > b[q];
-    return nonempty;
-  }
+__device__ __forceinline__ uint32_t nonempty(uint32_t s, uint32_t e, uint32_t b1, uint32_t b2,
+                                             uint32_t b3)
+{
+  return (b1 > s) + (b2 > b1) + (b3 > b2) + (e > b3);
+}
 
-  __device__ uint32_t count(uint64_t i) const
-  {
-    uint32_t b[5];
-    return split(begin() + i, b);
-  }
-
-  __device__ void emit(uint64_t i, uint32_t prefix) const
-  {
-    uint64_t const b0   = begin();
-    uint64_t const node = b0 + i;
-    if (node >= cap) return;
-    uint32_t b[5];
-    uint32_t const nc   = split(node, b);
-    uint64_t const next = b0 + meta->sizes[level] + prefix;  // first child slot
-    cfirst[node]        = static_cast<uint32_t>(next);
-    ccount[node]        = nc;
-    if (nc == 0) return;
-    if (next + nc > cap) {
-      atomicOr(&meta->overflow, 1u);
-      return;
+// Child boundaries b1..b3 of the node (key, [s, e)) with a whole wave; the result
+// is returned to every lane.
+__device__ __forceinline__ void wave_split(uint32_t const* __restrict__ K, uint32_t s, uint32_t e,
+                                           uint32_t key, int sh, uint32_t& b1, uint32_t& b2,
+                                           uint32_t& b3)
+{
+  constexpr int P   = 21;  // probes per boundary
+  int const lane    = lane_id();
+  int const grp     = lane / P;  // boundary 1..3 -> group 0..2; lane 63 idles
+  int const pj      = lane - grp * P;
+  bool const active = grp < 3;
+  uint64_t const target =
+    active ? ((static_cast<uint64_t>(key) * 4 + static_cast<uint64_t>(grp + 1)) << sh) : 0;
+  uint64_t const gmask = active ? (((1ull << P) - 1) << (P * grp)) : 0;
+  uint32_t lo = s, hi = e;
+  while (true) {
+    bool const open = active && lo < hi;
+    if (ballot(open) == 0) break;
+    uint64_t const len = hi - lo;
+    uint32_t const pos = lo + static_cast<uint32_t>((static_cast<uint64_t>(pj + 1) * len) / (P + 1));
+    bool const below   = open && static_cast<uint64_t>(K[pos]) < target;
+    uint64_t const bm  = ballot(below);
+    if (open) {
+      // probes are monotone: the first `ft` of this group are below the target
+      int const ft = __popcll(bm & gmask);
+      uint32_t const nlo =
+        ft > 0 ? lo + static_cast<uint32_t>((static_cast<uint64_t>(ft) * len) / (P + 1)) + 1 : lo;
+      uint32_t const nhi =
+        ft < P ? lo + static_cast<uint32_t>((static_cast<uint64_t>(ft + 1) * len) / (P + 1)) : hi;
+      lo = nlo;
+      hi = nhi;
     }
-    uint32_t const k = nkey[node];
-    uint64_t slot    = next;
+  }
+  b1 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo), 0));
+  b2 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo), P));
+  b3 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(lo), 2 * P));
+}
+
+// Child boundaries with one thread: three interleaved binary searches.
+__device__ __forceinline__ void thread_split(uint32_t const* __restrict__ K, uint32_t s,
+                                             uint32_t e, uint32_t key, int sh, uint32_t& b1,
+                                             uint32_t& b2, uint32_t& b3)
+{
+  uint64_t t[3];
+  uint32_t lo[3], hi[3];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      if (b[q + 1] > b[q]) {
-        nkey[slot]   = k * 4 + q;
-        nstart[slot] = b[q];
-        ncount[slot] = b[q + 1] - b[q];
-        nlevel[slot] = static_cast<uint8_t>(level + 1);
-        ++slot;
+  for (int q = 0; q < 3; ++q) {
+    t[q]  = (static_cast<uint64_t>(key) * 4 + static_cast<uint64_t>(q + 1)) << sh;
+    lo[q] = s;
+    hi[q] = e;
+  }
+  while (lo[0] < hi[0] || lo[1] < hi[1] || lo[2] < hi[2]) {
+    uint32_t mid[3], v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      mid[q] = lo[q] + ((hi[q] - lo[q]) >> 1);
+      v[q]   = lo[q] < hi[q] ? K[mid[q]] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (lo[q] < hi[q]) {
+        if (static_cast<uint64_t>(v[q]) < t[q])
+          lo[q] = mid[q] + 1;
+        else
+          hi[q] = mid[q];
       }
     }
   }
-};
+  b1 = lo[0];
+  b2 = lo[1];
+  b3 = lo[2];
+}
+
+__global__ void __launch_bounds__(256) level_split_kernel(LevelArgs a)
+{
+  __shared__ uint32_t lds[256 / kWave + 1];
+  uint64_t const S     = a.meta->sizes[a.level];
+  uint64_t const begin = level_begin(a.meta, a.level);
+  int const sh         = 2 * (a.d - 2 - a.level);
+  uint64_t lo, hi;
+  node_chunk(S, blockIdx.x, lo, hi);
+  uint32_t acc = 0;
+  if (wave_mode(S)) {
+    uint64_t const i    = lo + threadIdx.x / kWave;
+    uint64_t const node = begin + i;
+    if (i < hi && node < a.cap) {
+      uint32_t const c = a.ncount[node];
+      uint4 sp         = make_uint4(0, 0, 0, 0);
+      if (c > a.max_size) {
+        uint32_t const s = a.nstart[node];
+        uint32_t b1, b2, b3;
+        wave_split(a.K, s, s + c, a.nkey[node], sh, b1, b2, b3);
+        sp = make_uint4(b1, b2, b3, nonempty(s, s + c, b1, b2, b3));
+      }
+      if (lane_id() == 0) {
+        a.split[node] = sp;
+        acc           = sp.w;
+      }
+    }
+  } else {
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += 256) {
+      uint64_t const node = begin + i;
+      if (node >= a.cap) break;  // only after an overflow (reported to the host)
+      uint32_t const c = a.ncount[node];
+      uint4 sp         = make_uint4(0, 0, 0, 0);
+      if (c > a.max_size) {
+        uint32_t const s = a.nstart[node];
+        uint32_t b1, b2, b3;
+        thread_split(a.K, s, s + c, a.nkey[node], sh, b1, b2, b3);
+        sp = make_uint4(b1, b2, b3, nonempty(s, s + c, b1, b2, b3));
+      }
+      a.split[node] = sp;
+      acc += sp.w;
+    }
+  }
+  uint32_t tot;
+  block_excl_scan<256>(acc, lds, &tot);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = tot;
+}
+
+__global__ void __launch_bounds__(256) level_emit_kernel(LevelArgs a)
+{
+  __shared__ uint32_t lds[256 / kWave + 1];
+  uint64_t const S     = a.meta->sizes[a.level];
+  uint64_t const begin = level_begin(a.meta, a.level);
+  // this block's base (children of earlier blocks) and the level's total
+  uint32_t before = 0, all = 0;
+  for (int i = threadIdx.x; i < kLevelBlocks; i += 256) {
+    uint32_t const p = a.partials[i];
+    all += p;
+    if (i < static_cast<int>(blockIdx.x)) before += p;
+  }
+  uint32_t base, total;
+  block_excl_scan<256>(before, lds, &base);
+  block_excl_scan<256>(all, lds, &total);
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.meta->sizes[a.level + 1] = total;
+  uint64_t lo, hi;
+  node_chunk(S, blockIdx.x, lo, hi);
+  uint64_t const next0 = begin + S + base;
+  uint64_t run         = 0;
+  for (uint64_t t0 = lo; t0 < hi; t0 += 256) {
+    uint64_t const i    = t0 + threadIdx.x;
+    uint64_t const node = begin + i;
+    bool const live     = i < hi && node < a.cap;
+    uint4 const sp      = live ? a.split[node] : make_uint4(0, 0, 0, 0);
+    uint32_t tile_total;
+    uint32_t const ex = block_excl_scan<256>(sp.w, lds, &tile_total);
+    if (live) {
+      uint64_t const next = next0 + run + ex;  // first child slot
+      a.cfirst[node]      = static_cast<uint32_t>(next);
+      a.ccount[node]      = sp.w;
+      if (sp.w > 0) {
+        if (next + sp.w > a.cap) {
+          atomicOr(&a.meta->overflow, 1u);
+        } else {
+          uint32_t const k = a.nkey[node];
+          uint32_t const s = a.nstart[node];
+          uint32_t const e = s + a.ncount[node];
+          uint32_t const b[5] = {s, sp.x, sp.y, sp.z, e};
+          uint64_t slot       = next;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (b[q + 1] > b[q]) {
+              a.nkey[slot]   = k * 4 + q;
+              a.nstart[slot] = b[q];
+              a.ncount[slot] = b[q + 1] - b[q];
+              a.nlevel[slot] = static_cast<uint8_t>(a.level + 1);
+              ++slot;
+            }
+          }
+        }
+      }
+    }
+    run += tile_total;
+  }
+}
 
 // After all levels: total node count and the kept extra "parents".
 __global__ void __launch_bounds__(64)
@@ -396,22 +595,24 @@ static int64_t quadtree_on_points_impl(double const* x,
   int const key_bits = std::min(32, 2 * cbits);
   uint32_t const outside_key = static_cast<uint32_t>((1u << (2 * d)) - 1u);
 
-  uint64_t const un    = static_cast<uint64_t>(n);
+  uint64_t const un = static_cast<uint64_t>(n);
   uint64_t const tiles = sort_tiles(un);
-  dbuf<uint32_t> keys_a(un), keys_b(un), vals_a(un), vals_b(un);
-  dbuf<uint32_t> counts(static_cast<size_t>(kRadix) * tiles);
-  int const nb0 = std::min(kRadixBits, key_bits);
+  // point_indices is written by the sort's last pass straight into the output
+  uint32_t* pi = oa.make<uint32_t>(un);
+  dbuf<uint32_t> keys_a(un), keys_b(un), vals_tmp(un);
+  dbuf<uint32_t> counts0(static_cast<size_t>(tiles) * kRadix);
   {
     KernelTimer kt("keygen", stream);
-    keygen_kernel<<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>(
-    x, y, un, min_x, min_y, max_x, max_y, scale, outside_key, (1u << nb0) - 1u, keys_a.get(),
-    counts.get(), tiles);
+    bool const vec = reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(y) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(keys_a.get()) % 16 == 0;
+    keygen_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
+      x, y, un, min_x, min_y, max_x, max_y, scale, outside_key, vec, keys_a.get(), counts0.get());
     QPIP_LAUNCH_CHECK();
   }
-  SortResult sorted = radix_sort_pairs(keys_a.get(), keys_b.get(), nullptr, vals_a.get(),
-                                       vals_b.get(), un, key_bits, stream, counts.get(),
-                                       "point_sort");
-  counts.release();
+  SortResult sorted = radix_sort_pairs(keys_a.get(), keys_b.get(), nullptr, pi, vals_tmp.get(),
+                                       un, key_bits, stream, counts0.get(), "point_sort");
+  vals_tmp.release();
 
   // Node capacity: level 0 holds <= 64 nodes; level l >= 1 holds <= 4 children
   // of each internal node of level l-1 (internal nodes of a level are disjoint
@@ -426,7 +627,8 @@ static int64_t quadtree_on_points_impl(double const* x,
   dbuf<uint32_t> nkey(cap), nstart(cap), ncount(cap), cfirst(cap), ccount(cap);
   dbuf<uint8_t> nlevel(cap);
   dbuf<TreeMeta> meta(1);
-  dbuf<uint32_t> partials(kScanBlocks + 1);
+  dbuf<uint4> split(cap);
+  dbuf<uint32_t> partials(kLevelBlocks);
 
   {
   KernelTimer kt_levels("tree_levels", stream);
@@ -435,13 +637,13 @@ static int64_t quadtree_on_points_impl(double const* x,
   QPIP_LAUNCH_CHECK();
   // Levels 0..d-2 may have children (level d-1 is the bottom).
   for (int l = 0; l + 2 <= d; ++l) {
-    ExpandOp op{sorted.keys, nkey.get(), nstart.get(), ncount.get(), nlevel.get(),
-                cfirst.get(), ccount.get(), meta.get(), cap, l, d,
-                static_cast<uint32_t>(max_size)};
-    // sizes[l] is this level's length; sizes[l+1] receives the number of children
-    uint64_t* sizes = meta.get()->sizes;
-    count_scan_emit<ExpandOp, uint32_t, uint64_t>(op, sizes + l, 0, partials.get(), sizes + l + 1,
-                                                  stream, kScanBlocks, "tree_level_scan");
+    LevelArgs la{sorted.keys, nkey.get(), nstart.get(), ncount.get(), nlevel.get(),
+                 cfirst.get(), ccount.get(), split.get(), partials.get(), meta.get(), cap, l, d,
+                 static_cast<uint32_t>(max_size)};
+    level_split_kernel<<<kLevelBlocks, 256, 0, stream>>>(la);
+    QPIP_LAUNCH_CHECK();
+    level_emit_kernel<<<kLevelBlocks, 256, 0, stream>>>(la);
+    QPIP_LAUNCH_CHECK();
   }
   post_levels_kernel<<<1, 64, 0, stream>>>(nkey.get(), nstart.get(), ncount.get(), d,
                                            static_cast<uint32_t>(max_size), cap, meta.get());
@@ -467,12 +669,6 @@ static int64_t quadtree_on_points_impl(double const* x,
       o_offset);
     QPIP_LAUNCH_CHECK();
   }
-  // point_indices: the sorted values buffer
-  uint32_t* pi = nullptr;
-  if (sorted.values == vals_a.get())
-    pi = oa.adopt(vals_a, un);
-  else
-    pi = oa.adopt(vals_b, un);
   QPIP_HIP(hipStreamSynchronize(stream));
   *point_indices = pi;
   *key           = o_key;

@@ -3,13 +3,21 @@
 // (phase_1.cuh:92, quadtree_bbox_filtering.cuh:161-165), which sort all 32 bits;
 // only the bits that can be non-zero are sorted here.
 //
-// One pass per 8-bit digit, reduce-then-scan:
-//   count   : per-tile digit histogram (skipped for pass 0 when the producer
-//             kernel already wrote it, as the fused keygen does);
-//   scan    : one workgroup per digit scans that digit's row of tile counts;
-//   scatter : tiles of 4096 pairs are ranked in registers with wave64 ballots
-//             (stable: wave-striped order is index order), staged through LDS in
-//             digit order, and written out as runs of equal digits.
+// One 8-bit digit per pass, reduce-then-scan over tiles of 4096 pairs. No block
+// ever waits for another, so nothing depends on the order in which workgroups are
+// dispatched:
+//   count:    one block per tile counts its digits (LDS atomics into per-wave
+//             histograms) -> counts[tile][256];
+//   colscan:  counts[tile][d] becomes the first output position of the tile's
+//             digit-d pairs: digit base (exclusive scan of the digit totals) + the
+//             digit's count in earlier tiles. Three small kernels over groups of
+//             kScanRows tiles (reduce, scan of the group sums, rescan);
+//   scatter:  one block per tile (4096 pairs, 16 per thread, wave-striped so that
+//             register order is index order) ranks the tile with wave64 ballot peer
+//             matching, stages it through LDS in digit order, and writes each
+//             digit's run at its position.
+// The first pass's counts usually come from the producer of the keys (the keygen
+// kernel counts digit 0 while it writes the keys).
 #include <string>
 
 #include "qpip_sort.hpp"
@@ -23,43 +31,185 @@ __device__ __forceinline__ uint32_t digit_of(uint32_t key, int shift, uint32_t m
   return (key >> shift) & mask;
 }
 
-__global__ void __launch_bounds__(kSortThreads)
-  radix_count_kernel(uint32_t const* __restrict__ keys, uint64_t n, uint64_t tiles, int shift,
-                     uint32_t mask, uint32_t* __restrict__ counts)
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint64_t n, uint32_t v)
 {
-  __shared__ uint32_t hist[kRadix];
-  for (int i = threadIdx.x; i < kRadix; i += kSortThreads) hist[i] = 0;
-  __syncthreads();
-  uint64_t const tile = blockIdx.x;
-  int const w         = threadIdx.x / kWave;
-  uint64_t const base = tile * kSortTile + static_cast<uint64_t>(w) * (kSortItems * kWave);
-#pragma unroll 4
-  for (int j = 0; j < kSortItems; ++j) {
-    uint64_t const idx = base + static_cast<uint64_t>(j) * kWave + lane_id();
-    bool const valid   = idx < n;
-    uint32_t const d   = valid ? digit_of(keys[idx], shift, mask) : 0u;
-    wave_hist_add(hist, d, valid);
-  }
-  __syncthreads();
-  for (int d = threadIdx.x; d < kRadix; d += kSortThreads) counts[d * tiles + tile] = hist[d];
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
 }
 
-// Block d scans row d of counts (exclusive, in place) and writes its total.
-__global__ void __launch_bounds__(256)
-  radix_scan_kernel(uint32_t* __restrict__ counts, uint64_t tiles, uint32_t* __restrict__ row_total)
+constexpr int kScanRows = 64;  // tiles per column-scan group
+constexpr int kBatch    = 8;   // loads in flight per thread in the column scans
+
+// Digit counts of one tile. `vec`: the keys are 16-byte aligned (full tiles are
+// then read as uint4).
+__global__ void __launch_bounds__(kSortThreads)
+  radix_count_kernel(uint32_t const* __restrict__ keys, uint64_t n, int shift, uint32_t mask,
+                     bool vec, uint32_t* __restrict__ counts)
 {
-  __shared__ uint32_t lds[256 / kWave + 1];
-  uint32_t* row = counts + static_cast<uint64_t>(blockIdx.x) * tiles;
-  uint32_t base = 0;
-  for (uint64_t t0 = 0; t0 < tiles; t0 += 256) {
-    uint64_t const i = t0 + threadIdx.x;
-    uint32_t const c = i < tiles ? row[i] : 0u;
-    uint32_t tot;
-    uint32_t const ex = block_excl_scan<256>(c, lds, &tot);
-    if (i < tiles) row[i] = base + ex;
-    base += tot;
+  constexpr int NW = kSortThreads / kWave;
+  __shared__ uint32_t h[NW][kRadix];
+  int const w = threadIdx.x / kWave;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) h[ww][threadIdx.x] = 0;
+  __syncthreads();
+  uint64_t const lo = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+  if (vec && lo + kSortTile <= n) {
+    uint4 const* k4 = reinterpret_cast<uint4 const*>(keys + lo);
+    uint4 v[kSortItems / 4];
+#pragma unroll
+    for (int j = 0; j < kSortItems / 4; ++j) v[j] = k4[j * kSortThreads + threadIdx.x];
+#pragma unroll
+    for (int j = 0; j < kSortItems / 4; ++j) {
+      wave_count_digit(h[w], digit_of(v[j].x, shift, mask));
+      wave_count_digit(h[w], digit_of(v[j].y, shift, mask));
+      wave_count_digit(h[w], digit_of(v[j].z, shift, mask));
+      wave_count_digit(h[w], digit_of(v[j].w, shift, mask));
+    }
+  } else {
+    uint32_t d[kSortItems];
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) {
+      uint64_t const idx = lo + static_cast<uint64_t>(j) * kSortThreads + threadIdx.x;
+      d[j]               = idx < n ? digit_of(keys[idx], shift, mask) : kRadix;
+    }
+#pragma unroll
+    for (int j = 0; j < kSortItems; ++j) wave_count_digit(h[w], d[j]);
   }
-  if (threadIdx.x == 0) row_total[blockIdx.x] = base;
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) s += h[ww][threadIdx.x];
+  counts[static_cast<uint64_t>(blockIdx.x) * kRadix + threadIdx.x] = s;
+}
+
+// Column sums of counts over the tiles of group g (rows [g * kScanRows, ...)).
+__global__ void __launch_bounds__(kRadix)
+  colscan_reduce_kernel(uint32_t const* __restrict__ counts, uint64_t tiles,
+                        uint32_t* __restrict__ sums)
+{
+  int const d       = threadIdx.x;
+  uint64_t const r0 = static_cast<uint64_t>(blockIdx.x) * kScanRows;
+  uint64_t const r1 = r0 + kScanRows < tiles ? r0 + kScanRows : tiles;
+  uint32_t s        = 0;
+  for (uint64_t r = r0; r < r1; r += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = r + u < r1 ? counts[(r + u) * kRadix + d] : 0u;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) s += v[u];
+  }
+  sums[static_cast<uint64_t>(blockIdx.x) * kRadix + d] = s;
+}
+
+// sums[g][d] (in place) -> first output position of digit d's pairs of group g:
+// digit base (exclusive scan of the digit totals, in digit order) + the digit's
+// count in earlier groups. 1024 threads: digit d = threadIdx % 256 over quarter
+// q = threadIdx / 256 of the groups.
+__global__ void __launch_bounds__(1024)
+  colscan_top_kernel(uint32_t* __restrict__ sums, uint64_t groups)
+{
+  __shared__ uint32_t qsum[4][kRadix];
+  __shared__ uint32_t dbase[kRadix];
+  __shared__ uint32_t lds[1024 / kWave + 1];
+  int const d        = threadIdx.x % kRadix;
+  int const q        = threadIdx.x / kRadix;
+  uint64_t const per = (groups + 3) / 4;
+  uint64_t g0        = static_cast<uint64_t>(q) * per;
+  uint64_t g1        = g0 + per;
+  if (g0 > groups) g0 = groups;
+  if (g1 > groups) g1 = groups;
+  uint32_t s = 0;
+  for (uint64_t g = g0; g < g1; g += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = g + u < g1 ? sums[(g + u) * kRadix + d] : 0u;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) s += v[u];
+  }
+  qsum[q][d] = s;
+  __syncthreads();
+  uint32_t total = 0;
+  if (q == 0) total = qsum[0][d] + qsum[1][d] + qsum[2][d] + qsum[3][d];
+  uint32_t tot;
+  uint32_t const base = block_excl_scan<1024>(total, lds, &tot);
+  if (q == 0) dbase[d] = base;
+  __syncthreads();
+  uint32_t run = dbase[d];
+  for (int k = 0; k < q; ++k) run += qsum[k][d];
+  for (uint64_t g = g0; g < g1; g += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = g + u < g1 ? sums[(g + u) * kRadix + d] : 0u;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (g + u < g1) sums[(g + u) * kRadix + d] = run;
+      run += v[u];
+    }
+  }
+}
+
+// counts[t][d] (in place) -> first output position of tile t's digit-d pairs.
+__global__ void __launch_bounds__(kRadix)
+  colscan_down_kernel(uint32_t* __restrict__ counts, uint64_t tiles,
+                      uint32_t const* __restrict__ sums)
+{
+  int const d       = threadIdx.x;
+  uint64_t const r0 = static_cast<uint64_t>(blockIdx.x) * kScanRows;
+  uint64_t const r1 = r0 + kScanRows < tiles ? r0 + kScanRows : tiles;
+  uint32_t run      = sums[static_cast<uint64_t>(blockIdx.x) * kRadix + d];
+  for (uint64_t r = r0; r < r1; r += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) v[u] = r + u < r1 ? counts[(r + u) * kRadix + d] : 0u;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (r + u < r1) counts[(r + u) * kRadix + d] = run;
+      run += v[u];
+    }
+  }
+}
+
+// Ranks one tile: keys/values in registers, per-wave digit counts in s_wcount_w,
+// rank[j] = position of item j among equal digits within its wave (stable).
+template <int NB>
+__device__ __forceinline__ void rank_tile(uint32_t const* __restrict__ keys_in,
+                                          uint32_t const* __restrict__ vals_in, uint64_t n,
+                                          uint64_t base, int shift, uint32_t (&key)[kSortItems],
+                                          uint32_t (&val)[kSortItems],
+                                          uint32_t (&rank)[kSortItems], uint32_t* s_wcount_w)
+{
+  uint32_t const mask = (1u << NB) - 1u;
+  int const lane      = lane_id();
+  uint64_t const lt   = lanemask_lt();
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    uint64_t const idx = base + static_cast<uint64_t>(j) * kWave + lane;
+    bool const valid   = idx < n;
+    key[j]             = valid ? keys_in[idx] : 0xFFFFFFFFu;
+    val[j]             = valid ? (vals_in ? vals_in[idx] : static_cast<uint32_t>(idx)) : 0u;
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    uint64_t const idx   = base + static_cast<uint64_t>(j) * kWave + lane;
+    bool const valid     = idx < n;
+    uint32_t const d     = digit_of(key[j], shift, mask);
+    uint64_t const act   = ballot(valid);
+    uint64_t const peers = wave_match_digit<NB>(d, act);
+    uint32_t const prior = s_wcount_w[d];
+    rank[j]              = prior + static_cast<uint32_t>(__popcll(peers & lt));
+    if (valid && lane == __builtin_ctzll(peers))
+      s_wcount_w[d] = prior + static_cast<uint32_t>(__popcll(peers));
+  }
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8). Giving
+// XCD x the x-th eighth of the tiles, in order, makes the tiles that write the
+// neighbouring pieces of each digit's runs share an L2, where partial lines merge.
+__device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t tiles)
+{
+  uint64_t const per = tiles / 8, rem = tiles % 8, x = b % 8, r = b / 8;
+  return x * per + (x < rem ? x : rem) + r;
 }
 
 template <int NB>
@@ -69,10 +219,8 @@ __global__ void __launch_bounds__(kSortThreads)
                        uint32_t* __restrict__ keys_out,
                        uint32_t* __restrict__ vals_out,
                        uint64_t n,
-                       uint64_t tiles,
                        int shift,
-                       uint32_t const* __restrict__ counts,     // scanned rows
-                       uint32_t const* __restrict__ row_total)  // per-digit totals
+                       uint32_t const* __restrict__ offsets)  // tiles x 256, from the colscan
 {
   constexpr int NW = kSortThreads / kWave;
   __shared__ uint32_t s_keys[kSortTile];
@@ -83,65 +231,40 @@ __global__ void __launch_bounds__(kSortThreads)
   __shared__ uint32_t s_scan[kSortThreads / kWave + 1];
 
   uint32_t const mask = (1u << NB) - 1u;
-  uint64_t const tile = blockIdx.x;
   int const w         = threadIdx.x / kWave;
   int const lane      = lane_id();
-
-  for (int i = threadIdx.x; i < NW * kRadix; i += kSortThreads) (&s_wcount[0][0])[i] = 0;
+  int const d         = threadIdx.x;  // kSortThreads == kRadix: one digit per thread
+  uint64_t const tile = xcd_tile(blockIdx.x, gridDim.x);
+  s_global[d]         = offsets[tile * kRadix + d];
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) s_wcount[ww][d] = 0;
   __syncthreads();
-
   uint64_t const base = tile * kSortTile + static_cast<uint64_t>(w) * (kSortItems * kWave);
-  uint32_t key[kSortItems];
-  uint32_t val[kSortItems];
-  uint32_t rank[kSortItems];
-  uint64_t const lt = lanemask_lt();
-#pragma unroll
-  for (int j = 0; j < kSortItems; ++j) {
-    uint64_t const idx = base + static_cast<uint64_t>(j) * kWave + lane;
-    bool const valid   = idx < n;
-    key[j]             = valid ? keys_in[idx] : 0xFFFFFFFFu;
-    val[j]             = valid ? (vals_in ? vals_in[idx] : static_cast<uint32_t>(idx)) : 0u;
-    uint32_t const d   = digit_of(key[j], shift, mask);
-    uint64_t const act = ballot(valid);
-    uint64_t const peers = wave_match_digit<NB>(d, act);
-    uint32_t const prior = s_wcount[w][d];
-    rank[j]              = prior + static_cast<uint32_t>(__popcll(peers & lt));
-    if (valid && lane == __builtin_ctzll(peers))
-      s_wcount[w][d] = prior + static_cast<uint32_t>(__popcll(peers));
-  }
+  uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
+  rank_tile<NB>(keys_in, vals_in, n, base, shift, key, val, rank, s_wcount[w]);
   __syncthreads();
-
-  // Per digit: exclusive prefix over waves, tile-local digit starts, global base.
-  {
-    int const d = threadIdx.x;  // kSortThreads == kRadix
-    uint32_t acc = 0;
+  // per digit: exclusive prefix over waves and the tile's first slot of the digit
+  uint32_t count = 0;
 #pragma unroll
-    for (int ww = 0; ww < NW; ++ww) {
-      uint32_t const c = s_wcount[ww][d];
-      s_wcount[ww][d]  = acc;
-      acc += c;
-    }
-    uint32_t tot;
-    uint32_t const start = block_excl_scan<kSortThreads>(acc, s_scan, &tot);
-    s_tile_start[d]      = start;
-    uint32_t const rt    = row_total[d];
-    uint32_t const dbase = block_excl_scan<kSortThreads>(rt, s_scan, &tot);
-    s_global[d]          = dbase + counts[static_cast<uint64_t>(d) * tiles + tile];
+  for (int ww = 0; ww < NW; ++ww) {
+    uint32_t const c = s_wcount[ww][d];
+    s_wcount[ww][d]  = count;
+    count += c;
   }
+  uint32_t tot;
+  s_tile_start[d] = block_excl_scan<kSortThreads>(count, s_scan, &tot);
   __syncthreads();
-
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     uint64_t const idx = base + static_cast<uint64_t>(j) * kWave + lane;
     if (idx < n) {
-      uint32_t const d   = digit_of(key[j], shift, mask);
-      uint32_t const loc = s_tile_start[d] + s_wcount[w][d] + rank[j];
+      uint32_t const dj  = digit_of(key[j], shift, mask);
+      uint32_t const loc = s_tile_start[dj] + s_wcount[w][dj] + rank[j];
       s_keys[loc]        = key[j];
       s_vals[loc]        = val[j];
     }
   }
   __syncthreads();
-
   uint64_t const tile_lo = tile * kSortTile;
   uint32_t const valid_n =
     static_cast<uint32_t>((n - tile_lo) < static_cast<uint64_t>(kSortTile) ? (n - tile_lo)
@@ -151,90 +274,112 @@ __global__ void __launch_bounds__(kSortThreads)
     uint32_t const i = static_cast<uint32_t>(k * kSortThreads + threadIdx.x);
     if (i < valid_n) {
       uint32_t const kk  = s_keys[i];
-      uint32_t const d   = digit_of(kk, shift, mask);
-      uint64_t const pos = static_cast<uint64_t>(s_global[d]) + (i - s_tile_start[d]);
+      uint32_t const dk  = digit_of(kk, shift, mask);
+      uint64_t const pos = static_cast<uint64_t>(s_global[dk]) + (i - s_tile_start[dk]);
       keys_out[pos]      = kk;
       vals_out[pos]      = s_vals[i];
     }
   }
 }
 
-template <int NB>
-void launch_scatter(uint32_t const* ki, uint32_t const* vi, uint32_t* ko, uint32_t* vo, uint64_t n,
-                    uint64_t tiles, int shift, uint32_t const* counts, uint32_t const* row_total,
-                    hipStream_t stream)
-{
-  radix_scatter_kernel<NB><<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>(
-    ki, vi, ko, vo, n, tiles, shift, counts, row_total);
-  QPIP_LAUNCH_CHECK();
-}
+#define QPIP_DISPATCH_NB(nb, CALL) \
+  switch (nb) {                    \
+    case 1: CALL(1); break;        \
+    case 2: CALL(2); break;        \
+    case 3: CALL(3); break;        \
+    case 4: CALL(4); break;        \
+    case 5: CALL(5); break;        \
+    case 6: CALL(6); break;        \
+    case 7: CALL(7); break;        \
+    default: CALL(8); break;       \
+  }
 
 }  // namespace
+
+int radix_passes(int bits)
+{
+  if (bits < 1) bits = 1;
+  if (bits > 32) bits = 32;
+  return (bits + kRadixBits - 1) / kRadixBits;
+}
+
+void fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream)
+{
+  if (n == 0) return;
+  fill_u32_kernel<<<grid_for(n, 256, 2048), 256, 0, stream>>>(p, n, v);
+  QPIP_LAUNCH_CHECK();
+}
 
 SortResult radix_sort_pairs(uint32_t* keys_a,
                             uint32_t* keys_b,
                             uint32_t const* values_in,
-                            uint32_t* values_a,
-                            uint32_t* values_b,
+                            uint32_t* values_out,
+                            uint32_t* values_tmp,
                             uint64_t n,
                             int bits,
                             hipStream_t stream,
-                            uint32_t* pass0_counts,
+                            uint32_t* first_counts,
                             char const* tag)
 {
-  std::string const t_count   = std::string(tag) + "_count";
-  std::string const t_scan    = std::string(tag) + "_scan";
-  std::string const t_scatter = std::string(tag) + "_scatter";
   static_assert(kSortThreads == kRadix, "one thread per digit in the scatter prologue");
-  if (n == 0) return {keys_a, values_in ? const_cast<uint32_t*>(values_in) : values_a};
   if (bits < 1) bits = 1;
   if (bits > 32) bits = 32;
-  uint64_t const tiles = sort_tiles(n);
+  int const passes = radix_passes(bits);
+  if (n == 0) return {keys_a, values_out};
+  if (n >= (uint64_t{1} << 32)) throw logic_error("cuSpatial failure: sort input too large");
+  uint64_t const tiles         = sort_tiles(n);
+  uint64_t const groups        = (tiles + kScanRows - 1) / kScanRows;
   if (tiles > 0x7FFFFFFFull) throw logic_error("cuSpatial failure: sort input too large");
-  // pass0_counts, when given, already holds pass 0's tile histograms (written by
-  // the producer of the keys) and serves as the histogram buffer of every pass.
-  dbuf<uint32_t> own_counts;
-  if (!pass0_counts) own_counts.reset(static_cast<size_t>(kRadix) * tiles);
-  uint32_t* const cnt = pass0_counts ? pass0_counts : own_counts.get();
-  dbuf<uint32_t> row_total(kRadix);
+  std::string const t_count   = std::string(tag) + "_count";
+  std::string const t_scan    = std::string(tag) + "_colscan";
+  std::string const t_scatter = std::string(tag) + "_scatter";
 
+  // the last pass writes values_out
+  auto vdst_of = [&](int p) { return ((passes - 1 - p) % 2 == 0) ? values_out : values_tmp; };
   uint32_t* ksrc       = keys_a;
   uint32_t* kdst       = keys_b;
   uint32_t const* vsrc = values_in;  // null on the first pass: identity
-  uint32_t* vdst       = values_a;
-  uint32_t* vother     = values_b;
-  int const passes     = (bits + kRadixBits - 1) / kRadixBits;
+  bool const vec = (reinterpret_cast<uintptr_t>(keys_a) % 16 == 0) &&
+                   (reinterpret_cast<uintptr_t>(keys_b) % 16 == 0);
+
+  dbuf<uint32_t> own_counts(first_counts ? 0 : static_cast<size_t>(tiles) * kRadix);
+  uint32_t* counts = first_counts ? first_counts : own_counts.get();
+  dbuf<uint32_t> sums(static_cast<size_t>(groups) * kRadix);
   for (int p = 0; p < passes; ++p) {
-    int const shift = p * kRadixBits;
-    int const nb    = (bits - shift) < kRadixBits ? (bits - shift) : kRadixBits;
-    if (!(p == 0 && pass0_counts)) {
+    int const shift     = p * kRadixBits;
+    int const nb        = (bits - shift) < kRadixBits ? (bits - shift) : kRadixBits;
+    uint32_t const mask = (1u << nb) - 1u;
+    uint32_t* vdst      = vdst_of(p);
+    if (!(p == 0 && first_counts)) {
       KernelTimer kt(t_count.c_str(), stream);
       radix_count_kernel<<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>(
-        ksrc, n, tiles, shift, (1u << nb) - 1u, cnt);
+        ksrc, n, shift, mask, vec, counts);
       QPIP_LAUNCH_CHECK();
     }
     {
       KernelTimer kt(t_scan.c_str(), stream);
-      radix_scan_kernel<<<kRadix, 256, 0, stream>>>(cnt, tiles, row_total.get());
+      colscan_reduce_kernel<<<static_cast<unsigned>(groups), kRadix, 0, stream>>>(counts, tiles,
+                                                                               sums.get());
+      QPIP_LAUNCH_CHECK();
+      colscan_top_kernel<<<1, 1024, 0, stream>>>(sums.get(), groups);
+      QPIP_LAUNCH_CHECK();
+      colscan_down_kernel<<<static_cast<unsigned>(groups), kRadix, 0, stream>>>(counts, tiles,
+                                                                             sums.get());
       QPIP_LAUNCH_CHECK();
     }
-    KernelTimer kt(t_scatter.c_str(), stream);
-    switch (nb) {
-      case 1: launch_scatter<1>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
-      case 2: launch_scatter<2>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
-      case 3: launch_scatter<3>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
-      case 4: launch_scatter<4>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
-      case 5: launch_scatter<5>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
-      case 6: launch_scatter<6>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
-      case 7: launch_scatter<7>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
-      default: launch_scatter<8>(ksrc, vsrc, kdst, vdst, n, tiles, shift, cnt, row_total.get(), stream); break;
+    {
+      KernelTimer kt(t_scatter.c_str(), stream);
+#define QPIP_SCATTER(NB)                                                               \
+  radix_scatter_kernel<NB><<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>( \
+    ksrc, vsrc, kdst, vdst, n, shift, counts)
+      QPIP_DISPATCH_NB(nb, QPIP_SCATTER);
+#undef QPIP_SCATTER
+      QPIP_LAUNCH_CHECK();
     }
-    // next pass reads what this one wrote
     std::swap(ksrc, kdst);
     vsrc = vdst;
-    std::swap(vdst, vother);
   }
-  return {ksrc, const_cast<uint32_t*>(vsrc)};
+  return {ksrc, values_out};
 }
 
 }  // namespace qpip

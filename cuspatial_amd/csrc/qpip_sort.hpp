@@ -20,15 +20,19 @@ __device__ __forceinline__ uint64_t wave_match_digit(uint32_t d, uint64_t active
   return peers;
 }
 
-// Adds the digits of one wave-wide slot into an LDS histogram, one LDS atomic per
-// distinct digit (the lowest lane of each peer group adds the group's size).
-__device__ __forceinline__ void wave_hist_add(uint32_t* hist, uint32_t d, bool valid)
+// Adds one digit per lane (kRadix: no digit) into an LDS histogram: one LDS atomic
+// when every counted lane has the same digit (skewed data), else one per lane.
+__device__ __forceinline__ void wave_count_digit(uint32_t* hist, uint32_t d)
 {
-  uint64_t const active = ballot(valid);
-  uint64_t const peers  = wave_match_digit<kRadixBits>(d, active);
-  if (valid) {
-    int const leader = __builtin_ctzll(peers);
-    if (lane_id() == leader) atomicAdd(&hist[d], static_cast<uint32_t>(__popcll(peers)));
+  bool const valid   = d < static_cast<uint32_t>(kRadix);
+  uint64_t const act = ballot(valid);
+  if (act == 0) return;
+  int const first   = __builtin_ctzll(act);
+  uint32_t const d0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(d), first));
+  if (ballot(d == d0) == act) {
+    if (lane_id() == first) atomicAdd(&hist[d0], static_cast<uint32_t>(__popcll(act)));
+  } else if (valid) {
+    atomicAdd(&hist[d], 1u);
   }
 }
 
