@@ -89,7 +89,8 @@ def main():
     log(f"[bench] generating gaussian_holes with {args.points} points on {dev}")
     w = datasets.gaussian_holes(num_points=args.points, device=dev)
     bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
-    pts = cs.Points(w.x, w.y)
+    # points as a GeoSeries holds them: one interleaved x, y array (vec_2d)
+    pts = cs.Points.from_points_xy(torch.stack([w.x, w.y], dim=1).reshape(-1))
     polys = cs.Polygons(w.part_offset, w.ring_offset, w.px, w.py)
     torch.cuda.synchronize()
 
@@ -221,6 +222,7 @@ def main():
             "workload": "gaussian_holes: plain quadtree point-in-polygon join "
                         "(tree + boxes + join + PIP), inputs resident in HBM",
             "points": n,
+            "point_layout": "interleaved x, y (vec_2d, the GeoSeries storage)",
             "polygons": w.num_polygons,
             "vertices": int(w.px.size),
             "max_depth": w.max_depth,

@@ -30,6 +30,7 @@ COMMON = [
     "-fPIC",
     "-ffp-contract=off",  # FP64 predicate parity: no implicit FMA contraction
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fvisibility=hidden",  # only the QPIP_API entry points are exported
     "-Wall",
     "-Wno-unused-function",
 ]

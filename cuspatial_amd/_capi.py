@@ -54,9 +54,11 @@ EXPORTED = (
     "qpip_profile_reset",
     "qpip_profile_report",
     "qpip_quadtree_on_points",
+    "qpip_quadtree_on_points_xy",
     "qpip_polygon_bounding_boxes",
     "qpip_join_quadtree_and_bounding_boxes",
     "qpip_quadtree_point_in_polygon",
+    "qpip_quadtree_point_in_polygon_xy",
 )
 
 
@@ -84,6 +86,9 @@ def lib():
         L.qpip_quadtree_on_points.argtypes = [
             _vp, _vp, _i64, _i64, _d, _d, _d, _d, _d, ctypes.c_int8, ctypes.c_int32,
             P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]
+        L.qpip_quadtree_on_points_xy.argtypes = [
+            _vp, _i64, _d, _d, _d, _d, _d, ctypes.c_int8, ctypes.c_int32,
+            P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]
         L.qpip_polygon_bounding_boxes.argtypes = [
             _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, _d, P(qpip_resource), _vp,
             P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]
@@ -94,9 +99,14 @@ def lib():
             _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
             _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, P(qpip_resource), _vp,
             P(_vp), P(_vp), P(_i64)]
+        L.qpip_quadtree_point_in_polygon_xy.argtypes = [
+            _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64,
+            _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, P(qpip_resource), _vp,
+            P(_vp), P(_vp), P(_i64)]
         for name in (
-            "qpip_quadtree_on_points", "qpip_polygon_bounding_boxes",
+            "qpip_quadtree_on_points", "qpip_quadtree_on_points_xy", "qpip_polygon_bounding_boxes",
             "qpip_join_quadtree_and_bounding_boxes", "qpip_quadtree_point_in_polygon",
+            "qpip_quadtree_point_in_polygon_xy",
         ):
             getattr(L, name).restype = ctypes.c_int
         _lib = L

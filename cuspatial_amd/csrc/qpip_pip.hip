@@ -139,6 +139,8 @@ struct PipArgs {
   uint32_t const* chunk_pair;
   uint64_t num_chunks;
   uint64_t num_nodes;
+  double const* xy;  // interleaved x, y pairs instead of x / y (null: two arrays)
+  bool xy_vec;       // xy is 16-byte aligned
 };
 
 // chunk -> owning pair, so that each wave of the test finds its pair with one load
@@ -284,6 +286,15 @@ __global__ void __launch_bounds__(256)
           if (idx >= a.num_points) {
             err   = true;
             valid = false;
+          } else if (a.xy) {
+            if (a.xy_vec) {
+              double2 const p = reinterpret_cast<double2 const*>(a.xy)[idx];
+              px[u]           = p.x;
+              py[u]           = p.y;
+            } else {
+              px[u] = a.xy[2 * static_cast<uint64_t>(idx)];
+              py[u] = a.xy[2 * static_cast<uint64_t>(idx) + 1];
+            }
           } else {
             px[u] = a.x[idx];
             py[u] = a.y[idx];
@@ -347,7 +358,9 @@ __global__ void __launch_bounds__(256)
 
 }  // namespace qpip
 
-extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
+namespace {
+
+int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                                               const uint32_t* quad_index,
                                               int64_t num_pairs,
                                               const uint32_t* key,
@@ -360,6 +373,7 @@ extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
                                               int64_t num_point_indices,
                                               const double* x,
                                               const double* y,
+                                              const double* xy,
                                               int64_t num_x,
                                               int64_t num_y,
                                               const uint32_t* poly_offsets,
@@ -441,7 +455,8 @@ extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
                  static_cast<uint64_t>(num_x), x, y, poly_offsets,
                  static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
                  ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(),
-                 chunk_pair.get(), T, static_cast<uint64_t>(num_nodes)};
+                 chunk_pair.get(), T, static_cast<uint64_t>(num_nodes), xy,
+                 xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0};
     // 3. test
     dbuf<uint64_t> masks(T);
     dbuf<uint32_t> hit_counts(T);
@@ -491,4 +506,76 @@ extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
     set_last_error(e.what());
     return QPIP_ERROR_DEVICE;
   }
+}
+
+}  // namespace
+
+extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
+                                              const uint32_t* quad_index,
+                                              int64_t num_pairs,
+                                              const uint32_t* key,
+                                              const uint8_t* level,
+                                              const uint8_t* is_internal_node,
+                                              const uint32_t* length,
+                                              const uint32_t* offset,
+                                              int64_t num_nodes,
+                                              const uint32_t* point_indices,
+                                              int64_t num_point_indices,
+                                              const double* x,
+                                              const double* y,
+                                              int64_t num_x,
+                                              int64_t num_y,
+                                              const uint32_t* poly_offsets,
+                                              int64_t num_poly_offsets,
+                                              const uint32_t* ring_offsets,
+                                              int64_t num_ring_offsets,
+                                              const double* poly_x,
+                                              const double* poly_y,
+                                              int64_t num_poly_x,
+                                              int64_t num_poly_y,
+                                              const qpip_resource* mr,
+                                              void* stream,
+                                              uint32_t** polygon_index,
+                                              uint32_t** point_index,
+                                              int64_t* num_hits)
+{
+  return quadtree_point_in_polygon_entry(
+    poly_index, quad_index, num_pairs, key, level, is_internal_node, length, offset, num_nodes,
+    point_indices, num_point_indices, x, y, nullptr, num_x, num_y, poly_offsets,
+    num_poly_offsets, ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x, num_poly_y, mr,
+    stream, polygon_index, point_index, num_hits);
+}
+
+extern "C" int qpip_quadtree_point_in_polygon_xy(const uint32_t* poly_index,
+                                                 const uint32_t* quad_index,
+                                                 int64_t num_pairs,
+                                                 const uint32_t* key,
+                                                 const uint8_t* level,
+                                                 const uint8_t* is_internal_node,
+                                                 const uint32_t* length,
+                                                 const uint32_t* offset,
+                                                 int64_t num_nodes,
+                                                 const uint32_t* point_indices,
+                                                 int64_t num_point_indices,
+                                                 const double* points_xy,
+                                                 int64_t num_points,
+                                                 const uint32_t* poly_offsets,
+                                                 int64_t num_poly_offsets,
+                                                 const uint32_t* ring_offsets,
+                                                 int64_t num_ring_offsets,
+                                                 const double* poly_x,
+                                                 const double* poly_y,
+                                                 int64_t num_poly_x,
+                                                 int64_t num_poly_y,
+                                                 const qpip_resource* mr,
+                                                 void* stream,
+                                                 uint32_t** polygon_index,
+                                                 uint32_t** point_index,
+                                                 int64_t* num_hits)
+{
+  return quadtree_point_in_polygon_entry(
+    poly_index, quad_index, num_pairs, key, level, is_internal_node, length, offset, num_nodes,
+    point_indices, num_point_indices, nullptr, nullptr, points_xy, num_points, num_points,
+    poly_offsets, num_poly_offsets, ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
+    num_poly_y, mr, stream, polygon_index, point_index, num_hits);
 }

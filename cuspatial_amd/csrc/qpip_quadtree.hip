@@ -97,8 +97,10 @@ __device__ __forceinline__ uint32_t lower_bound_u32(uint32_t const* K, uint32_t 
 // --- 1. keygen: compute_point_keys_and_sorted_indices (phase_1.cuh:62-95) -------
 // One block per radix-sort tile (4096 points). The block also counts the sort's
 // first digit (key bits 0-7) into per-wave LDS histograms, so that the sort skips
-// its first counting pass. Full tiles of 16-byte aligned inputs are read as
-// double2 and written as uint4 (thread t owns points 4t..4t+3 of each quarter).
+// its first counting pass. Full tiles of 16-byte aligned inputs are read with
+// 16-byte loads and the keys written as uint4 (thread t owns points 4t..4t+3 of
+// each quarter of the tile). Points are either two coordinate arrays (the column
+// API) or interleaved x, y pairs (vec_2d, the header-only API's layout).
 __device__ __forceinline__ uint32_t point_key(double x, double y, double min_x, double min_y,
                                               double max_x, double max_y, double scale,
                                               uint32_t outside_key)
@@ -109,9 +111,10 @@ __device__ __forceinline__ uint32_t point_key(double x, double y, double min_x, 
   return (dilate_2(cy) << 1) | dilate_2(cx);
 }
 
+template <bool XY>
 __global__ void __launch_bounds__(256)
-  keygen_kernel(double const* __restrict__ xs,
-                double const* __restrict__ ys,
+  keygen_kernel(double const* __restrict__ xs,  // XY: interleaved x, y pairs
+                double const* __restrict__ ys,  // XY: unused
                 uint64_t n,
                 double min_x,
                 double min_y,
@@ -132,40 +135,54 @@ __global__ void __launch_bounds__(256)
   uint64_t const lo = static_cast<uint64_t>(blockIdx.x) * kSortTile;
   if (vec && lo + kSortTile <= n) {
     constexpr int Q = kSortItems / 4;
-    double2 const* x2 = reinterpret_cast<double2 const*>(xs + lo);
-    double2 const* y2 = reinterpret_cast<double2 const*>(ys + lo);
-    double2 xv[2 * Q], yv[2 * Q];
+    double px[kSortItems], py[kSortItems];
+    if (XY) {
+      double2 const* p2 = reinterpret_cast<double2 const*>(xs) + lo;
 #pragma unroll
-    for (int j = 0; j < Q; ++j) {
-      int const e   = (j * 256 + threadIdx.x) * 2;  // double2 index of point 4 * (j * 256 + t)
-      xv[2 * j]     = x2[e];
-      xv[2 * j + 1] = x2[e + 1];
-      yv[2 * j]     = y2[e];
-      yv[2 * j + 1] = y2[e + 1];
+      for (int j = 0; j < Q; ++j) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double2 const v   = p2[(j * 256 + threadIdx.x) * 4 + c];
+          px[4 * j + c]     = v.x;
+          py[4 * j + c]     = v.y;
+        }
+      }
+    } else {
+      double2 const* x2 = reinterpret_cast<double2 const*>(xs + lo);
+      double2 const* y2 = reinterpret_cast<double2 const*>(ys + lo);
+#pragma unroll
+      for (int j = 0; j < Q; ++j) {
+        int const e = (j * 256 + threadIdx.x) * 2;  // double2 index of point 4 * (j * 256 + t)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          double2 const xv  = x2[e + c];
+          double2 const yv  = y2[e + c];
+          px[4 * j + 2 * c] = xv.x;
+          px[4 * j + 2 * c + 1] = xv.y;
+          py[4 * j + 2 * c] = yv.x;
+          py[4 * j + 2 * c + 1] = yv.y;
+        }
+      }
     }
     uint4* k4 = reinterpret_cast<uint4*>(keys + lo);
 #pragma unroll
     for (int j = 0; j < Q; ++j) {
-      uint4 k;
-      k.x = point_key(xv[2 * j].x, yv[2 * j].x, min_x, min_y, max_x, max_y, scale, outside_key);
-      k.y = point_key(xv[2 * j].y, yv[2 * j].y, min_x, min_y, max_x, max_y, scale, outside_key);
-      k.z = point_key(xv[2 * j + 1].x, yv[2 * j + 1].x, min_x, min_y, max_x, max_y, scale,
-                      outside_key);
-      k.w = point_key(xv[2 * j + 1].y, yv[2 * j + 1].y, min_x, min_y, max_x, max_y, scale,
-                      outside_key);
-      k4[j * 256 + threadIdx.x] = k;
-      wave_count_digit(h[w], k.x & 0xFFu);
-      wave_count_digit(h[w], k.y & 0xFFu);
-      wave_count_digit(h[w], k.z & 0xFFu);
-      wave_count_digit(h[w], k.w & 0xFFu);
+      uint32_t k[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        k[c] = point_key(px[4 * j + c], py[4 * j + c], min_x, min_y, max_x, max_y, scale,
+                         outside_key);
+        wave_count_digit(h[w], k[c] & 0xFFu);
+      }
+      k4[j * 256 + threadIdx.x] = make_uint4(k[0], k[1], k[2], k[3]);
     }
   } else {
     double x[kSortItems], y[kSortItems];
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
       uint64_t const i = lo + static_cast<uint64_t>(j) * 256 + threadIdx.x;
-      x[j]             = i < n ? xs[i] : 0.0;
-      y[j]             = i < n ? ys[i] : 0.0;
+      x[j]             = i < n ? (XY ? xs[2 * i] : xs[i]) : 0.0;
+      y[j]             = i < n ? (XY ? xs[2 * i + 1] : ys[i]) : 0.0;
     }
 #pragma unroll
     for (int j = 0; j < kSortItems; ++j) {
@@ -563,6 +580,7 @@ __global__ void __launch_bounds__(256)
 // Host orchestration. Returns the number of nodes; outputs via `oa`.
 static int64_t quadtree_on_points_impl(double const* x,
                                        double const* y,
+                                       bool interleaved,  // x holds x, y pairs; y unused
                                        int64_t n,
                                        double x_min,
                                        double x_max,
@@ -604,10 +622,16 @@ static int64_t quadtree_on_points_impl(double const* x,
   {
     KernelTimer kt("keygen", stream);
     bool const vec = reinterpret_cast<uintptr_t>(x) % 16 == 0 &&
-                     reinterpret_cast<uintptr_t>(y) % 16 == 0 &&
+                     (interleaved || reinterpret_cast<uintptr_t>(y) % 16 == 0) &&
                      reinterpret_cast<uintptr_t>(keys_a.get()) % 16 == 0;
-    keygen_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
-      x, y, un, min_x, min_y, max_x, max_y, scale, outside_key, vec, keys_a.get(), counts0.get());
+    if (interleaved)
+      keygen_kernel<true><<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
+        x, nullptr, un, min_x, min_y, max_x, max_y, scale, outside_key, vec, keys_a.get(),
+        counts0.get());
+    else
+      keygen_kernel<false><<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
+        x, y, un, min_x, min_y, max_x, max_y, scale, outside_key, vec, keys_a.get(),
+        counts0.get());
     QPIP_LAUNCH_CHECK();
   }
   SortResult sorted = radix_sort_pairs(keys_a.get(), keys_b.get(), nullptr, pi, vals_tmp.get(),
@@ -681,6 +705,61 @@ static int64_t quadtree_on_points_impl(double const* x,
 
 }  // namespace qpip
 
+namespace {
+
+int quadtree_on_points_entry(const double* x,
+                             const double* y,
+                             bool interleaved,
+                             int64_t num_points,
+                             int64_t num_y,
+                             double x_min,
+                             double x_max,
+                             double y_min,
+                             double y_max,
+                             double scale,
+                             int8_t max_depth,
+                             int32_t max_size,
+                             const qpip_resource* mr,
+                             void* stream,
+                             uint32_t** point_indices,
+                             uint32_t** key,
+                             uint8_t** level,
+                             uint8_t** is_internal_node,
+                             uint32_t** length,
+                             uint32_t** offset,
+                             int64_t* num_nodes)
+{
+  using namespace qpip;
+  try {
+    QPIP_EXPECTS(point_indices && key && level && is_internal_node && length && offset && num_nodes,
+                 "output pointers must not be null");
+    *point_indices = nullptr;
+    *key = nullptr, *level = nullptr, *is_internal_node = nullptr, *length = nullptr,
+    *offset = nullptr, *num_nodes = 0;
+    // cpp/src/indexing/point_quadtree.cu:166
+    QPIP_EXPECTS(num_points == num_y, "x and y columns must have the same length");
+    if (num_points <= 0) return QPIP_SUCCESS;
+    QPIP_EXPECTS(x && (interleaved || y), "x and y must not be null");
+    OutputAllocator oa(mr, static_cast<hipStream_t>(stream));
+    *num_nodes = quadtree_on_points_impl(x, y, interleaved, num_points, x_min, x_max, y_min,
+                                         y_max, scale, max_depth, max_size, oa,
+                                         static_cast<hipStream_t>(stream), point_indices, key,
+                                         level, is_internal_node, length, offset);
+    return QPIP_SUCCESS;
+  } catch (logic_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_LOGIC;
+  } catch (oom_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_OOM;
+  } catch (std::exception const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_DEVICE;
+  }
+}
+
+}  // namespace
+
 extern "C" int qpip_quadtree_on_points(const double* x,
                                        const double* y,
                                        int64_t num_x,
@@ -702,31 +781,31 @@ extern "C" int qpip_quadtree_on_points(const double* x,
                                        uint32_t** offset,
                                        int64_t* num_nodes)
 {
-  using namespace qpip;
-  try {
-    QPIP_EXPECTS(point_indices && key && level && is_internal_node && length && offset && num_nodes,
-                 "output pointers must not be null");
-    *point_indices = nullptr;
-    *key = nullptr, *level = nullptr, *is_internal_node = nullptr, *length = nullptr,
-    *offset = nullptr, *num_nodes = 0;
-    // cpp/src/indexing/point_quadtree.cu:166-177
-    QPIP_EXPECTS(num_x == num_y, "x and y columns must have the same length");
-    if (num_x <= 0) return QPIP_SUCCESS;
-    QPIP_EXPECTS(x && y, "x and y must not be null");
-    OutputAllocator oa(mr, static_cast<hipStream_t>(stream));
-    *num_nodes = quadtree_on_points_impl(x, y, num_x, x_min, x_max, y_min, y_max, scale,
-                                         max_depth, max_size, oa, static_cast<hipStream_t>(stream),
-                                         point_indices, key, level, is_internal_node, length,
-                                         offset);
-    return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
-  }
+  return quadtree_on_points_entry(x, y, false, num_x, num_y, x_min, x_max, y_min, y_max, scale,
+                                  max_depth, max_size, mr, stream, point_indices, key, level,
+                                  is_internal_node, length, offset, num_nodes);
+}
+
+extern "C" int qpip_quadtree_on_points_xy(const double* points_xy,
+                                          int64_t num_points,
+                                          double x_min,
+                                          double x_max,
+                                          double y_min,
+                                          double y_max,
+                                          double scale,
+                                          int8_t max_depth,
+                                          int32_t max_size,
+                                          const qpip_resource* mr,
+                                          void* stream,
+                                          uint32_t** point_indices,
+                                          uint32_t** key,
+                                          uint8_t** level,
+                                          uint8_t** is_internal_node,
+                                          uint32_t** length,
+                                          uint32_t** offset,
+                                          int64_t* num_nodes)
+{
+  return quadtree_on_points_entry(points_xy, nullptr, true, num_points, num_points, x_min, x_max, y_min,
+                                  y_max, scale, max_depth, max_size, mr, stream, point_indices,
+                                  key, level, is_internal_node, length, offset, num_nodes);
 }

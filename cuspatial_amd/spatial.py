@@ -57,24 +57,44 @@ def _as_tensor(a, dtype, device, name) -> torch.Tensor:
 
 
 class Points:
-    """Point coordinates (the x / y columns of a point GeoSeries)."""
+    """Point coordinates. Either the x / y columns of a point GeoSeries
+    (``Points(x, y)``, the column API's layout) or its interleaved xy leaf column
+    (``Points.from_points_xy``, the GeoSeries storage and the vec_2d layout of the
+    header-only API, geoseries.py:237-250). Interleaved points are handed to the
+    library as they are; a gathered point then costs one cache line, not two."""
 
     def __init__(self, x, y, device=None):
         dev = _device(device if device is not None else
                       (x.device if isinstance(x, torch.Tensor) and x.is_cuda else None))
-        self.x = _as_tensor(x, torch.float64, dev, "x")
-        self.y = _as_tensor(y, torch.float64, dev, "y")
+        self._x = _as_tensor(x, torch.float64, dev, "x")
+        self._y = _as_tensor(y, torch.float64, dev, "y")
+        self.xy = None
         self.device = dev
 
     @classmethod
     def from_points_xy(cls, xy, device=None) -> "Points":
         """Interleaved [x0, y0, x1, y1, ...] (GeoSeries.from_points_xy)."""
         t = xy if isinstance(xy, torch.Tensor) else torch.as_tensor(np.asarray(xy, np.float64))
-        t = t.reshape(-1, 2)
-        return cls(t[:, 0].contiguous(), t[:, 1].contiguous(), device=device)
+        dev = _device(device if device is not None else (t.device if t.is_cuda else None))
+        t = _as_tensor(t.reshape(-1), torch.float64, dev, "xy")
+        if t.numel() % 2:
+            raise ValueError("interleaved xy needs an even number of values")
+        self = cls.__new__(cls)
+        self._x = self._y = None
+        self.xy = t
+        self.device = dev
+        return self
+
+    @property
+    def x(self) -> torch.Tensor:
+        return self._x if self.xy is None else self.xy[0::2]
+
+    @property
+    def y(self) -> torch.Tensor:
+        return self._y if self.xy is None else self.xy[1::2]
 
     def __len__(self):
-        return int(self.x.numel())
+        return int(self._x.numel()) if self.xy is None else int(self.xy.numel() // 2)
 
 
 class Polygons:
@@ -161,12 +181,16 @@ def quadtree_on_points(points: Points, x_min, x_max, y_min, y_max, scale, max_de
     pi, key, level, internal, length, offset = _outs(6)
     m = ctypes.c_int64()
     L = _capi.lib()
-    _capi.check(L.qpip_quadtree_on_points(
-        _p(points.x), _p(points.y), points.x.numel(), points.y.numel(), float(x_min),
-        float(x_max), float(y_min), float(y_max), float(max(scale, min_scale)),
-        ctypes.c_int8(int(max_depth)), ctypes.c_int32(int(max_size)), ctypes.byref(res.struct),
-        _capi.stream_handle(dev), ctypes.byref(pi), ctypes.byref(key), ctypes.byref(level),
-        ctypes.byref(internal), ctypes.byref(length), ctypes.byref(offset), ctypes.byref(m)))
+    common = (float(x_min), float(x_max), float(y_min), float(y_max),
+              float(max(scale, min_scale)), ctypes.c_int8(int(max_depth)),
+              ctypes.c_int32(int(max_size)), ctypes.byref(res.struct), _capi.stream_handle(dev),
+              ctypes.byref(pi), ctypes.byref(key), ctypes.byref(level), ctypes.byref(internal),
+              ctypes.byref(length), ctypes.byref(offset), ctypes.byref(m))
+    if points.xy is not None:
+        _capi.check(L.qpip_quadtree_on_points_xy(_p(points.xy), len(points), *common))
+    else:
+        _capi.check(L.qpip_quadtree_on_points(_p(points.x), _p(points.y), points.x.numel(),
+                                              points.y.numel(), *common))
     n = len(points)
     M = m.value
     point_indices = res.take(pi.value, n if M > 0 else 0, torch.uint32)
@@ -264,13 +288,18 @@ def quadtree_point_in_polygon(poly_quad_pairs, quadtree, point_indices, points: 
     res = _capi.TorchResource(dev)
     po, pt = _outs(2)
     h = ctypes.c_int64()
-    _capi.check(_capi.lib().qpip_quadtree_point_in_polygon(
-        _p(pp), _p(pq), pp.numel(), _p(key), _p(level), _p(internal), _p(length), _p(offset),
-        key.numel(), _p(pi), pi.numel(), _p(points.x), _p(points.y), points.x.numel(),
-        points.y.numel(), _p(polygons.part_offset), polygons.part_offset.numel(),
-        _p(polygons.ring_offset), polygons.ring_offset.numel(), _p(polygons.x),
-        _p(polygons.y), polygons.x.numel(), polygons.y.numel(), ctypes.byref(res.struct),
-        _capi.stream_handle(dev), ctypes.byref(po), ctypes.byref(pt), ctypes.byref(h)))
+    head = (_p(pp), _p(pq), pp.numel(), _p(key), _p(level), _p(internal), _p(length),
+            _p(offset), key.numel(), _p(pi), pi.numel())
+    tail = (_p(polygons.part_offset), polygons.part_offset.numel(), _p(polygons.ring_offset),
+            polygons.ring_offset.numel(), _p(polygons.x), _p(polygons.y), polygons.x.numel(),
+            polygons.y.numel(), ctypes.byref(res.struct), _capi.stream_handle(dev),
+            ctypes.byref(po), ctypes.byref(pt), ctypes.byref(h))
+    L = _capi.lib()
+    if points.xy is not None:
+        _capi.check(L.qpip_quadtree_point_in_polygon_xy(*head, _p(points.xy), len(points), *tail))
+    else:
+        _capi.check(L.qpip_quadtree_point_in_polygon(*head, _p(points.x), _p(points.y),
+                                                     points.x.numel(), points.y.numel(), *tail))
     out = Table()
     out["polygon_index"] = res.take(po.value, h.value, torch.uint32)
     out["point_index"] = res.take(pt.value, h.value, torch.uint32)

@@ -29,6 +29,12 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#if defined(__GNUC__)
+#define QPIP_API __attribute__((visibility("default")))
+#else
+#define QPIP_API
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -46,28 +52,28 @@ typedef struct qpip_resource {
 } qpip_resource;
 
 /* Message of the last failed call on this thread ("" if none). */
-const char* qpip_last_error_message(void);
+QPIP_API const char* qpip_last_error_message(void);
 
 /* Library version string. */
-const char* qpip_version(void);
+QPIP_API const char* qpip_version(void);
 
 /* Releases a buffer returned with mr == NULL. NULL is ignored. */
-void qpip_free(void* ptr);
+QPIP_API void qpip_free(void* ptr);
 
 /* Caps the bytes the library's pool may hold in live buffers (0: no cap).
  * Allocations beyond the cap fail with QPIP_ERROR_OOM. */
-void qpip_set_memory_limit(size_t bytes);
+QPIP_API void qpip_set_memory_limit(size_t bytes);
 
 /* Returns cached, unused pool memory to the device. */
-void qpip_trim(void);
+QPIP_API void qpip_trim(void);
 
 /* Kernel timing (no counterpart in the reference; for benchmarks). While
  * enabled, every kernel launch is bracketed by HIP events on its stream.
  * qpip_profile_report writes {"kernel name": [launches, total ms], ...} as JSON
  * into buf (truncated to len) and returns the full length. */
-void qpip_profile_enable(int on);
-void qpip_profile_reset(void);
-int64_t qpip_profile_report(char* buf, size_t len);
+QPIP_API void qpip_profile_enable(int on);
+QPIP_API void qpip_profile_reset(void);
+QPIP_API int64_t qpip_profile_report(char* buf, size_t len);
 
 /*
  * quadtree_on_points
@@ -81,7 +87,7 @@ int64_t qpip_profile_report(char* buf, size_t len);
  * offset (UINT32), each *num_nodes long.
  * Errors: "x and y columns must have the same length".
  */
-int qpip_quadtree_on_points(const double* x,
+QPIP_API int qpip_quadtree_on_points(const double* x,
                             const double* y,
                             int64_t num_x,
                             int64_t num_y,
@@ -103,6 +109,35 @@ int qpip_quadtree_on_points(const double* x,
                             int64_t* num_nodes);
 
 /*
+ * quadtree_on_points over interleaved points
+ * Replaces the header-only cuspatial::quadtree_on_points(PointIterator points_first,
+ *   PointIterator points_last, vec_2d<T> vertex_1, vec_2d<T> vertex_2, T scale,
+ *   int8_t max_depth, int32_t max_size) (cpp/include/cuspatial/point_quadtree.cuh:170-180),
+ *   whose points are vec_2d<double>: interleaved x, y pairs, the layout in which a
+ *   point GeoSeries holds its coordinates (python/cuspatial/cuspatial/core/geoseries.py:237-250).
+ * points_xy holds 2 * num_points doubles; everything else is as for
+ * qpip_quadtree_on_points.
+ */
+QPIP_API int qpip_quadtree_on_points_xy(const double* points_xy,
+                                        int64_t num_points,
+                                        double x_min,
+                                        double x_max,
+                                        double y_min,
+                                        double y_max,
+                                        double scale,
+                                        int8_t max_depth,
+                                        int32_t max_size,
+                                        const qpip_resource* mr,
+                                        void* stream,
+                                        uint32_t** point_indices,
+                                        uint32_t** key,
+                                        uint8_t** level,
+                                        uint8_t** is_internal_node,
+                                        uint32_t** length,
+                                        uint32_t** offset,
+                                        int64_t* num_nodes);
+
+/*
  * polygon_bounding_boxes
  * Replaces cuspatial::polygon_bounding_boxes
  *   (cpp/include/cuspatial/bounding_boxes.hpp:79-85,
@@ -113,7 +148,7 @@ int qpip_quadtree_on_points(const double* x,
  * num_poly_offsets - 1 rows (0 when num_poly_offsets <= 1).
  * Errors: "x and y must be the same size".
  */
-int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
+QPIP_API int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
                                 int64_t num_poly_offsets,
                                 const int32_t* ring_offsets,
                                 int64_t num_ring_offsets,
@@ -142,7 +177,7 @@ int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
  * Errors: "scale must be positive", "invalid bounding box (x_min, x_max, y_min, y_max)",
  * "maximum depth must be positive and less than 16".
  */
-int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
+QPIP_API int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
                                           const uint8_t* level,
                                           const uint8_t* is_internal_node,
                                           const uint32_t* length,
@@ -179,7 +214,7 @@ int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
  * Errors: "number of points must be the same for both x and y columns",
  * "numbers of vertices must be the same for both x and y columns".
  */
-int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
+QPIP_API int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
                                    const uint32_t* quad_index,
                                    int64_t num_pairs,
                                    const uint32_t* key,
@@ -207,6 +242,44 @@ int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
                                    uint32_t** polygon_index,
                                    uint32_t** point_index,
                                    int64_t* num_hits);
+
+/*
+ * quadtree_point_in_polygon over interleaved points
+ * Replaces the header-only cuspatial::quadtree_point_in_polygon(poly_indices_first,
+ *   poly_indices_last, quad_indices_first, quadtree, point_indices_first,
+ *   point_indices_last, PointIterator points_first, polygons)
+ *   (cpp/include/cuspatial/spatial_join.cuh:116-130), whose points are vec_2d<double>
+ *   (interleaved x, y pairs). points_xy holds 2 * num_points doubles; everything
+ *   else is as for qpip_quadtree_point_in_polygon.
+ * Errors: "number of points must be the same for both x and y columns" when
+ * num_point_indices != num_points.
+ */
+QPIP_API int qpip_quadtree_point_in_polygon_xy(const uint32_t* poly_index,
+                                               const uint32_t* quad_index,
+                                               int64_t num_pairs,
+                                               const uint32_t* key,
+                                               const uint8_t* level,
+                                               const uint8_t* is_internal_node,
+                                               const uint32_t* length,
+                                               const uint32_t* offset,
+                                               int64_t num_nodes,
+                                               const uint32_t* point_indices,
+                                               int64_t num_point_indices,
+                                               const double* points_xy,
+                                               int64_t num_points,
+                                               const uint32_t* poly_offsets,
+                                               int64_t num_poly_offsets,
+                                               const uint32_t* ring_offsets,
+                                               int64_t num_ring_offsets,
+                                               const double* poly_x,
+                                               const double* poly_y,
+                                               int64_t num_poly_x,
+                                               int64_t num_poly_y,
+                                               const qpip_resource* mr,
+                                               void* stream,
+                                               uint32_t** polygon_index,
+                                               uint32_t** point_index,
+                                               int64_t* num_hits);
 
 #ifdef __cplusplus
 }

@@ -33,11 +33,27 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 12
+    assert len(names) == 14
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)
     assert "gfx950" in capi.version()
+
+
+def test_exports_nothing_else(capi):
+    """The library is built with hidden visibility: its dynamic symbol table
+    holds exactly the declared entry points."""
+    import shutil
+    import subprocess
+
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--defined-only", capi.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    text_syms = {ln.split()[-1] for ln in out.splitlines() if len(ln.split()) == 3 and
+                 ln.split()[1] == "T"}
+    assert text_syms == set(declared_symbols())
 
 
 def _outs(n):

@@ -50,8 +50,26 @@ def clamp_scale(x_min, x_max, y_min, y_max, scale, d):
     return max(scale, max(x_max - x_min, y_max - y_min) / ((1 << d) + 2))
 
 
-def run_gpu(cs, x, y, bbox, scale, d, max_size, polys=None):
-    pts = cs.Points(x, y)
+def make_points(cs, x, y, layout):
+    """Column layout (x, y arrays) or interleaved xy (the GeoSeries / vec_2d layout);
+    "xy8" places the interleaved array 8 bytes off a 16-byte boundary."""
+    import torch
+
+    if layout == "columns":
+        return cs.Points(x, y)
+    xt = torch.as_tensor(np.asarray(x, np.float64)) if not isinstance(x, torch.Tensor) else x
+    yt = torch.as_tensor(np.asarray(y, np.float64)) if not isinstance(y, torch.Tensor) else y
+    xy = torch.stack([xt.to("cuda"), yt.to("cuda")], dim=1).reshape(-1)
+    if layout == "xy8":
+        buf = torch.empty(xy.numel() + 1, dtype=torch.float64, device="cuda")
+        buf[1:] = xy
+        xy = buf[1:]
+        assert xy.data_ptr() % 16 == 8
+    return cs.Points.from_points_xy(xy)
+
+
+def run_gpu(cs, x, y, bbox, scale, d, max_size, polys=None, layout="columns"):
+    pts = make_points(cs, x, y, layout)
     pi, tree = cs.quadtree_on_points(pts, *bbox, scale, d, max_size)
     out = {"pi": np_(pi)}
     out.update({k: np_(v) for k, v in tree.items()})
@@ -173,14 +191,15 @@ def scene(seed, n, ext=100.0, npoly=40, outside=0.05, dup=0.02):
     (6, 5000, 5, 12), (7, 20000, 7, 125), (8, 50000, 9, 30), (9, 100000, 12, 64),
     (10, 200000, 15, 256), (11, 100000, 15, 1), (12, 70000, 6, 100000), (13, 4097, 4, 16),
     (14, 65537, 8, 8), (15, 30000, 11, 2)])
-def test_random_parity(cs, oracle_mod, seed, n, d, max_size):
+@pytest.mark.parametrize("layout", ["columns", "xy"])
+def test_random_parity(cs, oracle_mod, seed, n, d, max_size, layout):
     x, y, polys = scene(seed, n)
     bbox = (0.0, 100.0, 0.0, 100.0)
     scale = 100.0 / (1 << d)
-    print(f"[parity] seed={seed} n={n} d={d} max_size={max_size}", flush=True)
+    print(f"[parity] seed={seed} n={n} d={d} max_size={max_size} {layout}", flush=True)
     # the join requires 0 < max_depth < 16 (quadtree_bbox_filtering.cu:104-105)
     p = polys if d > 0 else None
-    g = run_gpu(cs, x, y, bbox, scale, d, max_size, p)
+    g = run_gpu(cs, x, y, bbox, scale, d, max_size, p, layout=layout)
     o = run_oracle(oracle_mod, x, y, bbox, scale, d, max_size, p)
     assert_same(g, o)
     if d == 0:
@@ -190,7 +209,8 @@ def test_random_parity(cs, oracle_mod, seed, n, d, max_size):
 
 @pytest.mark.parametrize("seed,d,max_size", [(21, 2, 1), (22, 3, 2), (23, 1, 1), (24, 6, 4),
                                              (25, 15, 3), (26, 1, 100)])
-def test_clamped_scale_and_max_edge(cs, oracle_mod, seed, d, max_size):
+@pytest.mark.parametrize("layout", ["columns", "xy"])
+def test_clamped_scale_and_max_edge(cs, oracle_mod, seed, d, max_size, layout):
     """Scale below extent / 2^d is clamped to extent / (2^d + 2); points exactly on
     the bbox maximum then reach cell coordinate >= 2^d and the root level holds
     more than one node (the reference's num_parent_nodes quirk)."""
@@ -204,7 +224,7 @@ def test_clamped_scale_and_max_edge(cs, oracle_mod, seed, d, max_size):
     bbox = (1000.0, -1000.0, -1000.0, 1000.0)  # unordered on purpose
     polys = star_polygons(rng, 20, 2000.0, 50.0, 300.0, holes=0.2)
     polys = (polys[0], polys[1], polys[2] - 1000.0, polys[3] - 1000.0)
-    g = run_gpu(cs, x, y, bbox, -1.0, d, max_size, polys)
+    g = run_gpu(cs, x, y, bbox, -1.0, d, max_size, polys, layout=layout)
     o = run_oracle(oracle_mod, x, y, bbox, -1.0, d, max_size, polys)
     assert_same(g, o)
 
@@ -260,13 +280,14 @@ def test_uniform_stars_full_parity(cs, oracle_mod):
     assert len(o["polygon_index"]) > 10000
 
 
-def test_gaussian_holes_sample_parity(cs, oracle_mod):
+@pytest.mark.parametrize("layout", ["columns", "xy", "xy8"])
+def test_gaussian_holes_sample_parity(cs, oracle_mod, layout):
     from cuspatial_amd import datasets
 
     w = datasets.gaussian_holes(num_points=1_000_000)
     bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
     polys = (w.part_offset, w.ring_offset, w.px, w.py)
-    g = run_gpu(cs, w.x, w.y, bbox, w.scale, w.max_depth, w.max_size, polys)
+    g = run_gpu(cs, w.x, w.y, bbox, w.scale, w.max_depth, w.max_size, polys, layout=layout)
     o = run_oracle(oracle_mod, w.x, w.y, bbox, w.scale, w.max_depth, w.max_size, polys)
     assert_same(g, o)
     assert len(o["polygon_index"]) > 1000
@@ -348,6 +369,20 @@ def test_gaussian_holes_full_size_properties(cs, oracle_mod):
     assert np.array_equal(np_(res["point_index"]), pt)
     print(f"[full] subset hits={len(pp)}", flush=True)
     assert len(pp) > 1000
+    # interleaved points (the benchmark's layout): identical tree, pairs and hits
+    full = cs.quadtree_point_in_polygon(pairs, tree, pi, pts, P)
+    pxy = cs.Points.from_points_xy(torch.stack([w.x, w.y], dim=1).reshape(-1))
+    pi2, tree2 = cs.quadtree_on_points(pxy, *bbox, w.scale, d, ms)
+    assert torch.equal(i64(pi2), i64(pi))
+    for k in ("key", "level", "is_internal_node", "length", "offset"):
+        assert torch.equal(tree2[k].to(torch.int64) if k != "key" and k != "length" and
+                           k != "offset" else i64(tree2[k]),
+                           tree[k].to(torch.int64) if k != "key" and k != "length" and
+                           k != "offset" else i64(tree[k])), k
+    full2 = cs.quadtree_point_in_polygon(pairs, tree2, pi2, pxy, P)
+    assert torch.equal(i64(full2["polygon_index"]), i64(full["polygon_index"]))
+    assert torch.equal(i64(full2["point_index"]), i64(full["point_index"]))
+    print(f"[full] interleaved layout identical, hits={len(full2)}", flush=True)
 
 
 # ---------------------------------------------------------------------------
