@@ -65,7 +65,14 @@ struct PipMeta {
   uint32_t bad;
 };
 
-// --- 1. edge table: one thread per ring -----------------------------------------
+// --- 1. edge table: one wave per ring ----------------------------------------------
+// The reference walks a ring from its last vertex and skips a vertex whose edge to
+// the current `b` is zero within 4 ULP, without moving `b` (is_point_in_polygon.cuh:65).
+// In the common case `b` is simply the previous vertex: every vertex v > first is
+// kept when compared with v - 1, and the first vertex is kept, or skipped while
+// bitwise equal to the last one (a closed ring). Then the lanes build the edges in
+// parallel (ballot + popcount give each kept vertex its slot). Any other ring is
+// walked sequentially by lane 0, exactly as the reference does.
 __global__ void __launch_bounds__(256)
   build_edges_kernel(uint32_t const* __restrict__ ring_offsets,
                      int64_t num_rings,
@@ -76,25 +83,70 @@ __global__ void __launch_bounds__(256)
                      EdgeB* __restrict__ edges_b,
                      uint32_t* __restrict__ ring_edges)
 {
-  int64_t const r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  int64_t const r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (r >= num_rings) return;
-  int64_t vb = ring_offsets[r];
-  int64_t ve = ring_offsets[r + 1];
+  int const lane = lane_id();
+  int64_t vb     = ring_offsets[r];
+  int64_t ve     = ring_offsets[r + 1];
   if (vb > nv) vb = nv;
   if (ve > nv) ve = nv;
-  uint32_t ne = 0;
-  if (ve > vb) {
-    double bx = vx[ve - 1], by = vy[ve - 1];
-    for (int64_t v = vb; v < ve; ++v) {
+  if (ve <= vb) {
+    if (lane == 0) ring_edges[r] = 0;
+    return;
+  }
+  double const lx = vx[ve - 1], ly = vy[ve - 1];
+  // pass 1: is the pairwise rule the reference's walk?
+  bool simple = true;
+  for (int64_t v0 = vb; v0 < ve; v0 += kWave) {
+    int64_t const v = v0 + lane;
+    bool bad        = false;
+    if (v < ve) {
       double const ax = vx[v], ay = vy[v];
-      double const run = bx - ax, rise = by - ay;
-      if (float_equal(run, 0.0) && float_equal(rise, 0.0)) continue;
-      edges[vb + ne]   = Edge{ax, ay, run, rise};
-      edges_b[vb + ne] = EdgeB{bx, by};
-      ++ne;
-      bx = ax;
-      by = ay;
+      double const bx = v == vb ? lx : vx[v - 1];
+      double const by = v == vb ? ly : vy[v - 1];
+      bool const skip = float_equal(bx - ax, 0.0) && float_equal(by - ay, 0.0);
+      if (skip) bad = v != vb || !(ax == lx && ay == ly &&
+                                    __double_as_longlong(ax) == __double_as_longlong(lx) &&
+                                    __double_as_longlong(ay) == __double_as_longlong(ly));
     }
+    if (ballot(bad)) simple = false;
+  }
+  if (simple) {
+    uint32_t base = 0;
+    for (int64_t v0 = vb; v0 < ve; v0 += kWave) {
+      int64_t const v = v0 + lane;
+      bool keep       = false;
+      double ax = 0, ay = 0, bx = 0, by = 0;
+      if (v < ve) {
+        ax   = vx[v];
+        ay   = vy[v];
+        bx   = v == vb ? lx : vx[v - 1];
+        by   = v == vb ? ly : vy[v - 1];
+        keep = !(float_equal(bx - ax, 0.0) && float_equal(by - ay, 0.0));
+      }
+      uint64_t const km = ballot(keep);
+      if (keep) {
+        uint32_t const slot = base + static_cast<uint32_t>(__popcll(km & lanemask_lt()));
+        edges[vb + slot]    = Edge{ax, ay, bx - ax, by - ay};
+        edges_b[vb + slot]  = EdgeB{bx, by};
+      }
+      base += static_cast<uint32_t>(__popcll(km));
+    }
+    if (lane == 0) ring_edges[r] = base;
+    return;
+  }
+  if (lane != 0) return;
+  uint32_t ne = 0;
+  double bx = lx, by = ly;
+  for (int64_t v = vb; v < ve; ++v) {
+    double const ax = vx[v], ay = vy[v];
+    double const run = bx - ax, rise = by - ay;
+    if (float_equal(run, 0.0) && float_equal(rise, 0.0)) continue;
+    edges[vb + ne]   = Edge{ax, ay, run, rise};
+    edges_b[vb + ne] = EdgeB{bx, by};
+    ++ne;
+    bx = ax;
+    by = ay;
   }
   ring_edges[r] = ne;
 }
@@ -136,24 +188,11 @@ struct PipArgs {
   Edge const* edges;
   EdgeB const* edges_b;
   uint32_t const* chunk_start;
-  uint32_t const* chunk_pair;
   uint64_t num_chunks;
   uint64_t num_nodes;
   double const* xy;  // interleaved x, y pairs instead of x / y (null: two arrays)
   bool xy_vec;       // xy is 16-byte aligned
 };
-
-// chunk -> owning pair, so that each wave of the test finds its pair with one load
-__global__ void __launch_bounds__(256)
-  chunk_pair_kernel(uint32_t const* __restrict__ chunk_start, uint64_t K, uint64_t T,
-                    uint32_t* __restrict__ chunk_pair)
-{
-  uint64_t const k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= K) return;
-  uint32_t const c0 = chunk_start[k];
-  uint64_t const c1 = k + 1 < K ? chunk_start[k + 1] : T;
-  for (uint64_t c = c0; c < c1; ++c) chunk_pair[c] = static_cast<uint32_t>(k);
-}
 
 // --- 3. test ------------------------------------------------------------------------
 // One wave per run of pairs that share a leaf (the join orders pairs by leaf offset,
@@ -250,7 +289,7 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
 
 __global__ void __launch_bounds__(256)
   pip_test_kernel(PipArgs a, uint64_t* __restrict__ masks, uint32_t* __restrict__ hit_counts,
-                  uint32_t* __restrict__ bad)
+                  uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
 {
   uint32_t const k0 = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
   if (k0 >= a.num_pairs) return;
@@ -318,6 +357,8 @@ __global__ void __launch_bounds__(256)
         uint64_t const m = lane == 0 ? hit_m[0] : lane == 1 ? hit_m[1] : lane == 2 ? hit_m[2] : hit_m[3];
         masks[slot + lane]      = m;
         hit_counts[slot + lane] = static_cast<uint32_t>(__popcll(m));
+        // what the write kernel needs: the polygon and the chunk's first point position
+        chunk_info[slot + lane] = make_uint2(poly, first + (j0 + lane) * kWave);
       }
     }
   }
@@ -334,23 +375,20 @@ struct HitScanOp {
 
 // --- 5. write -------------------------------------------------------------------------
 __global__ void __launch_bounds__(256)
-  pip_write_kernel(PipArgs a, uint64_t const* __restrict__ masks, uint64_t const* __restrict__ hit_start,
+  pip_write_kernel(uint64_t num_chunks, uint64_t const* __restrict__ masks,
+                   uint2 const* __restrict__ chunk_info, uint64_t const* __restrict__ hit_start,
                    uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point)
 {
   uint32_t const c = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-  if (c >= a.num_chunks) return;
+  if (c >= num_chunks) return;
   uint64_t const mask = masks[c];
   if (mask == 0) return;
-  uint32_t const k     = uniform(a.chunk_pair[c]);
-  uint32_t const poly  = a.pair_poly[k];
-  uint32_t const quad  = a.pair_quad[k];
-  uint32_t const first = a.offset[quad];
-  uint32_t const j     = c - a.chunk_start[k];
-  int const l          = lane_id();
+  uint2 const info = chunk_info[c];
+  int const l      = lane_id();
   if ((mask >> l) & 1ull) {
     uint64_t const o = hit_start[c] + static_cast<uint64_t>(__popcll(mask & lanemask_lt()));
-    out_poly[o]      = poly;
-    out_point[o]     = first + j * kWave + static_cast<uint32_t>(l);
+    out_poly[o]      = info.x;
+    out_point[o]     = info.y + static_cast<uint32_t>(l);
   }
 }
 
@@ -422,7 +460,7 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     dbuf<uint32_t> ring_edges(std::max<uint64_t>(num_rings, 1));
     if (num_rings > 0) {
       KernelTimer kt("pip_edges", stream);
-      build_edges_kernel<<<grid_for(num_rings, 256), 256, 0, stream>>>(
+      build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
         ring_offsets, static_cast<int64_t>(num_rings), poly_x, poly_y, num_poly_x, edges.get(),
         edges_b.get(), ring_edges.get());
       QPIP_LAUNCH_CHECK();
@@ -444,27 +482,21 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     if (T == 0) return QPIP_SUCCESS;
     QPIP_EXPECTS(T < (uint64_t{1} << 31), "too many candidate points");
 
-    dbuf<uint32_t> chunk_pair(T);
-    {
-      KernelTimer kt("pip_chunks", stream);
-      chunk_pair_kernel<<<grid_for(K, 256), 256, 0, stream>>>(chunk_start.get(), K, T,
-                                                              chunk_pair.get());
-      QPIP_LAUNCH_CHECK();
-    }
     PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
                  static_cast<uint64_t>(num_x), x, y, poly_offsets,
                  static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
-                 ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(),
-                 chunk_pair.get(), T, static_cast<uint64_t>(num_nodes), xy,
+                 ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), T,
+                 static_cast<uint64_t>(num_nodes), xy,
                  xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0};
     // 3. test
     dbuf<uint64_t> masks(T);
     dbuf<uint32_t> hit_counts(T);
+    dbuf<uint2> chunk_info(T);
     unsigned const blocks = static_cast<unsigned>((T + 3) / 4);
     {
       KernelTimer kt("pip_test", stream);
       pip_test_kernel<<<static_cast<unsigned>((K + 3) / 4), 256, 0, stream>>>(
-        args, masks.get(), hit_counts.get(), bad.get());
+        args, masks.get(), hit_counts.get(), chunk_info.get(), bad.get());
       QPIP_LAUNCH_CHECK();
     }
     // 4. scan
@@ -487,8 +519,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     uint32_t* o_point = oa.make<uint32_t>(H);
     {
       KernelTimer kt("pip_write", stream);
-      pip_write_kernel<<<blocks, 256, 0, stream>>>(args, masks.get(), hit_start.get(), o_poly,
-                                                   o_point);
+      pip_write_kernel<<<blocks, 256, 0, stream>>>(T, masks.get(), chunk_info.get(),
+                                                   hit_start.get(), o_poly, o_point);
       QPIP_LAUNCH_CHECK();
     }
     QPIP_HIP(hipStreamSynchronize(stream));

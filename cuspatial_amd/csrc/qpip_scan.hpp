@@ -4,13 +4,14 @@
 // contributes count(i) outputs, and element i writes its outputs at the exclusive
 // prefix sum of the counts (the reference does this with thrust::exclusive_scan
 // followed by scatter / copy_if, e.g. phase_2.cuh:197-208, traversal.cuh:216-240,
-// quadtree_point_in_polygon.cuh:145-207). Here it is three launches on a fixed
+// quadtree_point_in_polygon.cuh:145-207). Here it is two launches on a fixed
 // grid, whose length may live on the device (so that no host synchronisation is
 // needed between data-dependent stages):
 //   1. chunk_count: block b reduces the counts of its contiguous chunk;
-//   2. chunk_top:   one block scans the per-chunk sums (also writes the total);
-//   3. chunk_emit:  block b rescans its chunk and calls emit(i, prefix).
-// The functor's count() is evaluated twice (passes 1 and 3), which is cheaper
+//   2. chunk_emit:  block b sums the chunk sums of the blocks before it (at most
+//                   1024 values, a few KB of L2), rescans its chunk and calls
+//                   emit(i, prefix); block 0 also stores the grand total.
+// The functor's count() is evaluated twice (passes 1 and 2), which is cheaper
 // than storing it when it is a few loads.
 #pragma once
 
@@ -61,33 +62,27 @@ __global__ void __launch_bounds__(kScanThreads) chunk_count_kernel(Op op,
   }
 }
 
-// Scans partials[0..G) in place (exclusive) and writes the grand total to
-// *total (if non-null) and partials[G].
-template <class C, class TT>
-__global__ void __launch_bounds__(1024) chunk_top_kernel(C* partials, int G, TT* total)
-{
-  __shared__ C lds[1024 / kWave + 1];
-  C v       = threadIdx.x < static_cast<unsigned>(G) ? partials[threadIdx.x] : C(0);
-  C tot     = 0;
-  C const e = block_excl_scan<1024>(v, lds, &tot);
-  if (threadIdx.x < static_cast<unsigned>(G)) partials[threadIdx.x] = e;
-  if (threadIdx.x == 0) {
-    partials[G] = tot;
-    if (total) *total = tot;  // TT converts (or splits) the total
-  }
-}
-
-template <class Op, class C>
+template <class Op, class C, class TT>
 __global__ void __launch_bounds__(kScanThreads) chunk_emit_kernel(Op op,
                                                                   uint64_t const* n_dev,
                                                                   uint64_t n_host,
-                                                                  C const* partials)
+                                                                  C* partials,
+                                                                  TT* total_dev)
 {
   __shared__ C lds[kScanThreads / kWave + 1];
   uint64_t const n = load_len(n_dev, n_host);
   uint64_t lo, hi;
   chunk_range(n, blockIdx.x, gridDim.x, lo, hi);
-  C base = partials[blockIdx.x];
+  C before = 0, all = 0;
+  for (unsigned i = threadIdx.x; i < gridDim.x; i += kScanThreads) {
+    C const p = partials[i];
+    all += p;
+    if (i < blockIdx.x) before += p;
+  }
+  C base, total;
+  block_excl_scan<kScanThreads>(before, lds, &base);
+  block_excl_scan<kScanThreads>(all, lds, &total);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && total_dev) *total_dev = total;  // TT converts
   for (uint64_t t0 = lo; t0 < hi; t0 += kScanThreads) {
     uint64_t const i = t0 + threadIdx.x;
     C const c        = i < hi ? op.count(i) : C(0);
@@ -99,8 +94,8 @@ __global__ void __launch_bounds__(kScanThreads) chunk_emit_kernel(Op op,
 }
 
 // Runs count/scan/emit over a sequence whose length is *n_dev (device) or
-// n_host (when n_dev is null). `partials` needs blocks + 1 entries; the total is
-// left at partials[blocks] and, if given, converted into *total_dev.
+// n_host (when n_dev is null). `partials` needs `blocks` entries; the total is,
+// if given, converted into *total_dev.
 template <class Op, class C, class TT = C>
 void count_scan_emit(Op const& op,
                      uint64_t const* n_dev,
@@ -115,9 +110,8 @@ void count_scan_emit(Op const& op,
   KernelTimer kt(timer, stream);
   chunk_count_kernel<Op, C><<<blocks, kScanThreads, 0, stream>>>(op, n_dev, n_host, partials);
   QPIP_LAUNCH_CHECK();
-  chunk_top_kernel<C, TT><<<1, 1024, 0, stream>>>(partials, blocks, total_dev);
-  QPIP_LAUNCH_CHECK();
-  chunk_emit_kernel<Op, C><<<blocks, kScanThreads, 0, stream>>>(op, n_dev, n_host, partials);
+  chunk_emit_kernel<Op, C, TT>
+    <<<blocks, kScanThreads, 0, stream>>>(op, n_dev, n_host, partials, total_dev);
   QPIP_LAUNCH_CHECK();
 }
 
