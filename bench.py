@@ -28,6 +28,9 @@ METRIC = ("points/sec joined on 1 MI355X, learned+batched vs plain quadtree PIP 
           "p99 ms")
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on this
+# benchmark (scaled by the same counters on a device copy of known size)
+TRAFFIC_FILE = os.path.join(HERE, "profiles", "gaussian_holes_hbm_traffic.txt")
 
 
 def log(msg):
@@ -49,6 +52,24 @@ def effective_edges(part_offset, ring_offset, px, py):
     ring_edges = np.bincount(ring_id, weights=~zero, minlength=starts.size)
     poly_of_ring = np.repeat(np.arange(part_offset.size - 1), np.diff(part_offset))
     return np.bincount(poly_of_ring, weights=ring_edges, minlength=part_offset.size - 1)
+
+
+def measured_traffic(timer: str, points: int):
+    """HBM bytes per launch of the kernel behind `timer` at this point count, from
+    the committed counter summary (rows: timer points kernel calls fetch_MiB
+    write_MiB), or None when it holds no such row."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        fields = ln.split()
+        if not fields or ln.startswith("#") or len(fields) != 6:
+            continue
+        if fields[0] == timer and fields[1] == str(points):
+            return (float(fields[4]) + float(fields[5])) * 2**20
+    return None
 
 
 def main():
@@ -170,7 +191,7 @@ def main():
             "peak": HBM_PEAK_GBS if bound == "hbm" else FP64_PEAK_TFLOPS,
             "unit": dom["unit"],
             "frac": dom["frac"],
-            "traffic": None,
+            "traffic": measured_traffic(dominant, n),
             "avg_ms": dom["avg_ms"],
             "algorithmic_per_launch": f"{units:.6g} {unit}",
         }
