@@ -81,6 +81,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="target CPU work of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="gaussian_holes",
+                    choices=["gaussian_holes", "batched_jobs"],
+                    help="gaussian_holes: the plain join at 1e8 points (the headline line); "
+                         "batched_jobs: 4096 independent jobs through the batched entry point, "
+                         "with the plain per-job calls on the same inputs timed beside it")
+    ap.add_argument("--jobs", type=int, default=4096, help="jobs of the batched workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the known "
                          "traffic that scales the HBM counters) and exit")
@@ -107,6 +113,8 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copied {args.copy_calibration} MiB three times")
         return
+    if args.workload == "batched_jobs":
+        return bench_batched(args, dev)
     log(f"[bench] generating gaussian_holes with {args.points} points on {dev}")
     w = datasets.gaussian_holes(num_points=args.points, device=dev)
     bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
@@ -261,6 +269,117 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_batched(args, dev):
+    """Batched join: every job of the batched_jobs workload in one call per step;
+    the plain path (four single-job calls per job) on the same inputs is timed
+    once beside it, and every job's hits are compared between the two."""
+    import numpy as np
+    import torch
+
+    import cuspatial_amd as cs
+    from cuspatial_amd import _capi, datasets
+
+    log(f"[bench] generating batched_jobs ({args.jobs} jobs) on {dev}")
+    w = datasets.batched_jobs(num_jobs=args.jobs, device=dev)
+    n = w.num_points
+    pts = cs.Points.from_points_xy(w.xy)
+    polys = cs.Polygons(w.part_offset, w.ring_offset, w.px, w.py)
+    torch.cuda.synchronize()
+    log(f"[bench] {n} points, {len(polys)} polygons, {len(w.px)} vertices")
+
+    def step():
+        return cs.batched_quadtree_point_in_polygon(pts, w.job_point_offsets, polys,
+                                                    w.job_polygon_offsets, w.bboxes, w.scales,
+                                                    w.max_depths, w.max_sizes)
+
+    for i in range(args.warmup):
+        log(f"[bench] warmup {i}")
+        out = step()
+    del out
+    torch.cuda.synchronize()
+    _capi.profile_reset()
+    _capi.profile_enable(True)
+    times = []
+    for i in range(args.steps):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+    _capi.profile_enable(False)
+    prof = _capi.profile_report()
+    ms = 1e3 * sum(times) / len(times)
+    value = n / (ms / 1e3)
+    log(f"[bench] batched: {ms:.2f} ms per step, {value:.4g} points/s, hits={out.num_hits}")
+
+    # the plain path on the same inputs, job by job (the same device arrays)
+    log("[bench] plain per-job calls")
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    plain_hits = []
+    xy = w.xy
+    pp = torch.as_tensor(w.part_offset, device=dev)
+    for j in range(w.num_jobs):
+        a, b = int(w.job_point_offsets[j]), int(w.job_point_offsets[j + 1])
+        pj = cs.Points.from_points_xy(xy[2 * a:2 * b])
+        q0, q1 = int(w.job_polygon_offsets[j]), int(w.job_polygon_offsets[j + 1])
+        P = cs.Polygons(pp[q0:q1 + 1], polys.ring_offset, polys.x, polys.y)
+        x0, x1, y0, y1 = w.bboxes[j]
+        d, msz, sc = int(w.max_depths[j]), int(w.max_sizes[j]), float(w.scales[j])
+        pi, tree = cs.quadtree_on_points(pj, x0, x1, y0, y1, sc, d, msz)
+        bb = _job_boxes(cs, P)
+        pairs = cs.join_quadtree_and_bounding_boxes(tree, bb, x0, x1, y0, y1, sc, d)
+        res = cs.quadtree_point_in_polygon(pairs, tree, pi, pj, P)
+        plain_hits.append(len(res))
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t
+    plain_value = n / plain_s
+    same = [int(out.job_hit_offsets[j + 1] - out.job_hit_offsets[j]) == plain_hits[j]
+            for j in range(w.num_jobs)]
+    log(f"[bench] plain: {plain_s * 1e3:.1f} ms, {plain_value:.4g} points/s; hit counts equal "
+        f"for {sum(same)}/{len(same)} jobs")
+    kernels = {k: {"launches_per_step": c / args.steps, "ms_per_step": round(v / args.steps, 6)}
+               for k, (c, v) in prof.items()}
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched_jobs: {w.num_jobs} jobs in one batched call",
+                   "points": n, "polygons": len(polys), "vertices": int(len(w.px)),
+                   "hits": out.num_hits, "seed": w.seed},
+        "plain_per_job": {"value": plain_value, "unit": "points/s", "seconds": plain_s,
+                          "jobs_with_equal_hit_counts": sum(same)},
+        "speedup_vs_plain": value / plain_value,
+        "kernels": kernels,
+        "p_ms": {"min": 1e3 * min(times), "max": 1e3 * max(times)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _job_boxes(cs, P):
+    """Bounding boxes of a job's polygons given as a slice of the global polygon
+    arrays (offsets not rebased): computed on the rebased copy, as a caller that
+    owns the job's polygons would."""
+    import torch
+
+    part = P.part_offset
+    r0, r1 = int(part[0]), int(part[-1])
+    ring = P.ring_offset[r0:r1 + 1]
+    v0, v1 = int(ring[0]), int(ring[-1])
+    Q = cs.Polygons((part - r0).to(torch.int32), (ring - v0).to(torch.int32), P.x[v0:v1],
+                    P.y[v0:v1])
+    return cs.polygon_bounding_boxes(Q)
 
 
 if __name__ == "__main__":

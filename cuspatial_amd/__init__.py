@@ -7,10 +7,15 @@ the C ABI in include/cuspatial_amd.h.
 """
 
 from .spatial import (  # noqa: F401
+    BatchResult,
     Points,
     Polygons,
     Table,
+    batched_quadtree_point_in_polygon,
     join_quadtree_and_bounding_boxes,
+    pairwise_point_in_polygon,
+    point_in_polygon,
+    point_in_polygon_bitmask,
     polygon_bounding_boxes,
     quadtree_on_points,
     quadtree_point_in_polygon,
@@ -24,4 +29,9 @@ __all__ = [
     "polygon_bounding_boxes",
     "join_quadtree_and_bounding_boxes",
     "quadtree_point_in_polygon",
+    "point_in_polygon",
+    "point_in_polygon_bitmask",
+    "pairwise_point_in_polygon",
+    "batched_quadtree_point_in_polygon",
+    "BatchResult",
 ]

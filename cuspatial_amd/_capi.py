@@ -59,6 +59,9 @@ EXPORTED = (
     "qpip_join_quadtree_and_bounding_boxes",
     "qpip_quadtree_point_in_polygon",
     "qpip_quadtree_point_in_polygon_xy",
+    "qpip_point_in_polygon",
+    "qpip_pairwise_point_in_polygon",
+    "qpip_batched_quadtree_point_in_polygon",
 )
 
 
@@ -103,7 +106,16 @@ def lib():
             _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64,
             _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, P(qpip_resource), _vp,
             P(_vp), P(_vp), P(_i64)]
+        for name in ("qpip_point_in_polygon", "qpip_pairwise_point_in_polygon"):
+            getattr(L, name).argtypes = [
+                _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
+                P(qpip_resource), _vp, P(_vp), P(_i64)]
+        L.qpip_batched_quadtree_point_in_polygon.argtypes = [
+            _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
+            _i64, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), _vp, P(_i64)]
         for name in (
+            "qpip_batched_quadtree_point_in_polygon",
+            "qpip_point_in_polygon", "qpip_pairwise_point_in_polygon",
             "qpip_quadtree_on_points", "qpip_quadtree_on_points_xy", "qpip_polygon_bounding_boxes",
             "qpip_join_quadtree_and_bounding_boxes", "qpip_quadtree_point_in_polygon",
             "qpip_quadtree_point_in_polygon_xy",

@@ -1,0 +1,966 @@
+// Batched quadtree point-in-polygon: many independent jobs (points, polygons,
+// bounding box, scale, depth, max_size each), every stage run once for all of them.
+//
+// Per job the result is exactly what the single-job sequence returns:
+// quadtree_on_points -> polygon_bounding_boxes -> join_quadtree_and_bounding_boxes ->
+// quadtree_point_in_polygon (with the Python layer's bbox ordering and scale clamp,
+// python/cuspatial/cuspatial/core/spatial/indexing.py:165-189, join.py:144-156):
+// the job's point_indices, and its (polygon_index, point_index) hits in the same
+// order, with job-local indices.
+//
+// The jobs are processed as one forest in a shared index space (a group of jobs
+// whose job number and Morton key fit one 32-bit sort key):
+//   * keys are (job << key_bits) | Morton key, so one radix sort leaves every
+//     job's points in its own input range, in its own key order;
+//   * level 0 is found per job; the level expansion is the single-tree kernel
+//     with per-node split rules (the job's depth and max_size);
+//   * the join is the single-tree breadth-first traversal, its top frontier built
+//     per job (node = i % T_j, box = i / T_j within the job) and node geometry
+//     taken from the node's job; the leaf pairs are sorted by leaf offset, which
+//     orders them by job and, within a job, as the single-job join does;
+//   * the point-in-polygon test runs on the forest's arrays as one call;
+//   * hits and point_indices are translated back to job-local indices.
+// A group's jobs never interact: every search is confined to its job's key range.
+#include <algorithm>
+#include <vector>
+
+#include "qpip_box.hpp"
+#include "qpip_join.hpp"
+#include "qpip_sort.hpp"
+#include "qpip_tree.hpp"
+
+namespace qpip {
+
+namespace {
+
+constexpr int kBatchMarked    = 16;
+constexpr int kMaxLevelsBatch = 16;
+
+// One job of a group, after the reference's clamps (detail/point_quadtree.cuh:259-268).
+struct JobDev {
+  double min_x, min_y, max_x, max_y, scale;
+  uint32_t point_begin, point_end;  // group-local point (and sorted position) range
+  uint32_t poly_begin, poly_end;    // group-local polygon range
+  uint32_t outside_key;
+  int32_t d;
+  uint32_t max_size;
+  int32_t shift0;
+};
+
+// Per job, filled on the device.
+struct JobTree {
+  uint32_t num_top;     // level-0 nodes (T)
+  uint32_t top_first;   // first level-0 node in the forest
+  uint32_t num_marked;  // E = R - 1 (root-level quirk)
+  uint32_t marked_key[kBatchMarked];
+  uint32_t marked_internal[kBatchMarked];
+  uint32_t marked_start[kBatchMarked];
+  uint32_t marked_count[kBatchMarked];
+};
+
+struct ForestMeta {
+  uint64_t sizes[kMaxLevelsBatch + 1];
+  uint32_t overflow;
+  uint64_t num_nodes;
+};
+
+__device__ __forceinline__ uint32_t find_job(JobDev const* jobs, uint32_t lo, uint32_t hi,
+                                             uint32_t pos)
+{
+  // last job in [lo, hi] whose point_begin <= pos
+  while (lo < hi) {
+    uint32_t const mid = lo + ((hi - lo + 1) >> 1);
+    if (jobs[mid].point_begin <= pos)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// --- keys ----------------------------------------------------------------------------
+// One block per radix-sort tile; the tile's jobs are found once per block, each
+// point's job by a search among them. Also counts the sort's first digit.
+__global__ void __launch_bounds__(256)
+  batch_keygen_kernel(double const* __restrict__ xy, uint64_t n, JobDev const* __restrict__ jobs,
+                      uint32_t num_jobs, int key_bits, uint32_t* __restrict__ keys,
+                      uint32_t* __restrict__ counts)
+{
+  constexpr int NW = 256 / kWave;
+  __shared__ uint32_t h[NW][kRadix];
+  __shared__ uint32_t s_jobs[2];
+  int const w = threadIdx.x / kWave;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) h[ww][threadIdx.x] = 0;
+  uint64_t const lo = static_cast<uint64_t>(blockIdx.x) * kSortTile;
+  uint64_t const hi = lo + kSortTile < n ? lo + kSortTile : n;
+  if (threadIdx.x == 0) {
+    s_jobs[0] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(lo));
+    s_jobs[1] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(hi - 1));
+  }
+  __syncthreads();
+  uint32_t const j0 = s_jobs[0], j1 = s_jobs[1];
+#pragma unroll 4
+  for (int k = 0; k < kSortItems; ++k) {
+    uint64_t const i = lo + static_cast<uint64_t>(k) * 256 + threadIdx.x;
+    uint32_t digit   = kRadix;
+    if (i < n) {
+      uint32_t const j = find_job(jobs, j0, j1, static_cast<uint32_t>(i));
+      JobDev const& J  = jobs[j];
+      double2 const p  = make_double2(xy[2 * i], xy[2 * i + 1]);
+      uint32_t const m = point_key(p.x, p.y, J.min_x, J.min_y, J.max_x, J.max_y, J.scale,
+                                   J.outside_key);
+      uint32_t const key = (key_bits < 32 ? (j << key_bits) : 0u) | m;
+      keys[i]            = key;
+      digit              = key & 0xFFu;
+    }
+    wave_count_digit(h[w], digit);
+  }
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) s += h[ww][threadIdx.x];
+  counts[static_cast<uint64_t>(blockIdx.x) * kRadix + threadIdx.x] = s;
+}
+
+// --- level 0 and the root-level quirk, one wave per job -----------------------------
+__global__ void __launch_bounds__(256)
+  batch_level0_kernel(uint32_t const* __restrict__ K, JobDev const* __restrict__ jobs,
+                      uint32_t num_jobs, int key_bits, uint4* __restrict__ staging,
+                      JobTree* __restrict__ trees)
+{
+  uint32_t const j = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (j >= num_jobs) return;
+  JobDev const J     = jobs[j];
+  int const v        = lane_id();
+  uint32_t const pb  = J.point_begin, pe = J.point_end;
+  uint64_t const pre = static_cast<uint64_t>(j) << key_bits;
+  // Prefix v covers keys [v << shift0, (v + 1) << shift0) of the job; prefixes past
+  // the job's key range (which would reach into the job number bits) hold nothing.
+  uint64_t const span = uint64_t{1} << key_bits;
+  uint64_t const v0   = static_cast<uint64_t>(v) << J.shift0;
+  bool present        = false;
+  uint32_t lo = pe, hi = pe;
+  if (v0 < span) {
+    uint64_t const v1 = static_cast<uint64_t>(v + 1) << J.shift0;
+    lo      = lower_bound_u32(K, pb, pe, pre + v0);
+    hi      = lower_bound_u32(K, lo, pe, pre + (v1 < span ? v1 : span));
+    present = hi > lo;
+  }
+  uint64_t const m = ballot(present);
+  if (present) {
+    uint32_t const slot = static_cast<uint32_t>(__popcll(m & lanemask_lt()));
+    staging[static_cast<uint64_t>(j) * 64 + slot] = make_uint4(static_cast<uint32_t>(v), lo, hi - lo, 0);
+  }
+  bool root_present = false;
+  if (J.d >= 1 && v < 16) {
+    uint64_t const r0 = static_cast<uint64_t>(v) << (2 * J.d);
+    if (r0 < span) {
+      uint64_t const r1  = static_cast<uint64_t>(v + 1) << (2 * J.d);
+      uint32_t const rlo = lower_bound_u32(K, pb, pe, pre + r0);
+      uint32_t const rhi = lower_bound_u32(K, rlo, pe, pre + (r1 < span ? r1 : span));
+      root_present       = rhi > rlo;
+    }
+  }
+  uint64_t const rm = ballot(root_present);
+  if (v == 0) {
+    JobTree& T       = trees[j];
+    T.num_top        = static_cast<uint32_t>(__popcll(m));
+    int const R      = __popcll(rm);
+    uint32_t const E = (J.d >= 1 && R > 1) ? static_cast<uint32_t>(R - 1) : 0u;
+    T.num_marked     = E;
+    uint32_t pos     = pb;
+    for (uint32_t e = 0; e < E && pos < pe; ++e) {
+      uint32_t const k     = K[pos];
+      T.marked_key[e]      = k & ((key_bits < 32) ? ((1u << key_bits) - 1u) : 0xFFFFFFFFu);
+      T.marked_internal[e] = 0;
+      T.marked_start[e]    = 0;
+      T.marked_count[e]    = 0;
+      pos                  = lower_bound_u32(K, pos, pe, static_cast<uint64_t>(k) + 1);
+    }
+  }
+}
+
+// Copies each job's level-0 nodes into the forest, job after job.
+struct Level0Op {
+  JobTree* trees;
+  uint4 const* staging;
+  uint32_t* nkey;
+  uint32_t* nstart;
+  uint32_t* ncount;
+  uint8_t* nlevel;
+  uint32_t* njob;
+  __device__ uint32_t count(uint64_t j) const { return trees[j].num_top; }
+  __device__ void emit(uint64_t j, uint32_t prefix) const
+  {
+    uint32_t const t = trees[j].num_top;
+    trees[j].top_first = prefix;
+    for (uint32_t i = 0; i < t; ++i) {
+      uint4 const s = staging[j * 64 + i];
+      nkey[prefix + i]   = s.x;
+      nstart[prefix + i] = s.y;
+      ncount[prefix + i] = s.z;
+      nlevel[prefix + i] = 0;
+      njob[prefix + i]   = static_cast<uint32_t>(j);
+    }
+  }
+};
+
+// Split rule of the forest: the node's job gives its depth, max_size and key prefix.
+struct JobLevels {
+  uint32_t* njob;
+  JobDev const* jobs;
+  int key_bits;
+  __device__ __forceinline__ bool rule(uint64_t node, int level, uint32_t& ms, int& sh,
+                                       uint64_t& prefix) const
+  {
+    uint32_t const j = njob[node];
+    int const d      = jobs[j].d;
+    ms               = jobs[j].max_size;
+    sh               = 2 * (d - 2 - level);
+    prefix           = static_cast<uint64_t>(j) << key_bits;
+    return level <= d - 2;
+  }
+  __device__ __forceinline__ void inherit(uint64_t parent, uint64_t child) const
+  {
+    njob[child] = njob[parent];
+  }
+};
+
+// First forest node in [lo, hi) whose (job, key) is not below (j, k).
+__device__ __forceinline__ uint64_t lower_bound_job_key(uint32_t const* njob, uint32_t const* nkey,
+                                                        uint64_t lo, uint64_t hi, uint32_t j,
+                                                        uint32_t k)
+{
+  while (lo < hi) {
+    uint64_t const mid = lo + ((hi - lo) >> 1);
+    bool const below   = njob[mid] < j || (njob[mid] == j && nkey[mid] < k);
+    if (below)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Per job: the kept extra "parents" of the root-level quirk (bottom level d - 1).
+__global__ void __launch_bounds__(256)
+  batch_marked_kernel(uint32_t const* __restrict__ nkey, uint32_t const* __restrict__ nstart,
+                      uint32_t const* __restrict__ ncount, uint32_t const* __restrict__ njob,
+                      JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                      ForestMeta const* __restrict__ meta, JobTree* __restrict__ trees)
+{
+  uint32_t const j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= num_jobs) return;
+  JobTree& T = trees[j];
+  if (T.num_marked == 0) return;
+  int const L = jobs[j].d - 1;  // bottom level (d >= 1 when marked)
+  uint64_t b  = 0;
+  for (int l = 0; l < L; ++l) b += meta->sizes[l];
+  uint64_t const e_ = b + meta->sizes[L];
+  for (uint32_t e = 0; e < T.num_marked; ++e) {
+    uint64_t const at = lower_bound_job_key(njob, nkey, b, e_, j, T.marked_key[e]);
+    if (at < e_ && njob[at] == j && nkey[at] == T.marked_key[e] && ncount[at] > jobs[j].max_size) {
+      T.marked_internal[e] = 1;
+      T.marked_start[e]    = nstart[at];
+      T.marked_count[e]    = ncount[at];
+    }
+  }
+}
+
+// The forest's is_internal / length / offset (detail/point_quadtree.cuh:88-133): an
+// internal node's offset is its first child's forest index, a leaf's its first
+// point position in the group's sorted order (minus the quirk's skipped points).
+__global__ void __launch_bounds__(256)
+  batch_finalize_kernel(uint32_t const* __restrict__ nkey, uint32_t const* __restrict__ nstart,
+                        uint32_t const* __restrict__ ncount, uint8_t const* __restrict__ nlevel,
+                        uint32_t const* __restrict__ njob, uint32_t const* __restrict__ cfirst,
+                        uint32_t const* __restrict__ ccount, JobDev const* __restrict__ jobs,
+                        JobTree const* __restrict__ trees, uint64_t m,
+                        uint8_t* __restrict__ internal_out, uint32_t* __restrict__ length_out,
+                        uint32_t* __restrict__ offset_out)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  uint32_t const j     = njob[i];
+  JobDev const& J      = jobs[j];
+  int const level      = nlevel[i];
+  uint32_t const count = ncount[i];
+  uint32_t const start = nstart[i];
+  bool internal        = level <= J.d - 2 && count > J.max_size;
+  uint32_t length      = internal ? ccount[i] : count;
+  uint32_t offset      = internal ? cfirst[i] : start;
+  JobTree const& T     = trees[j];
+  if (T.num_marked > 0 && !internal) {
+    bool marked      = false;
+    uint32_t skipped = 0;
+    for (uint32_t e = 0; e < T.num_marked; ++e) {
+      if (!T.marked_internal[e]) continue;
+      if (level == J.d - 1 && T.marked_key[e] == nkey[i]) marked = true;
+      if (T.marked_start[e] < start) skipped += T.marked_count[e];
+    }
+    if (marked) {
+      internal = true;  // no children: the join never descends into it
+      length   = 0;
+      offset   = 0;
+    } else {
+      offset = start - skipped;
+    }
+  }
+  internal_out[i] = internal ? 1 : 0;
+  length_out[i]   = length;
+  offset_out[i]   = offset;
+}
+
+// --- join ----------------------------------------------------------------------------
+// Node geometry of the forest: the node's job.
+struct JobGeometry {
+  uint32_t const* njob;
+  JobDev const* jobs;
+  __device__ __forceinline__ void get(uint32_t node, double& vx, double& vy, double& sc,
+                                      int& d) const
+  {
+    JobDev const& J = jobs[njob[node]];
+    vx              = J.min_x;
+    vy              = J.min_y;
+    sc              = J.scale;
+    d               = J.d;
+  }
+};
+
+// Top frontier sizes T_j * B_j.
+struct TopSizeOp {
+  JobTree const* trees;
+  JobDev const* jobs;
+  uint64_t* top_off;
+  __device__ uint64_t count(uint64_t j) const
+  {
+    return static_cast<uint64_t>(trees[j].num_top) * (jobs[j].poly_end - jobs[j].poly_begin);
+  }
+  __device__ void emit(uint64_t j, uint64_t prefix) const { top_off[j] = prefix; }
+};
+
+// Level 0 of the traversal: job j's virtual frontier i -> (i % T_j, i / T_j).
+struct BatchTopOp {
+  TreeView t;
+  BoxView bx;
+  JobGeometry g;
+  JobTree const* trees;
+  JobDev const* jobs;
+  uint64_t const* top_off;
+  uint32_t num_jobs;
+  JoinMeta* meta;
+  uint32_t* out_node;
+  uint32_t* out_bbox;
+  uint32_t* fr_node;
+  uint32_t* fr_bbox;
+  uint64_t cap_out, cap_fr;
+
+  __device__ void locate(uint64_t i, uint32_t& node, uint32_t& bbox) const
+  {
+    uint32_t lo = 0, hi = num_jobs - 1;  // last job with top_off <= i
+    while (lo < hi) {
+      uint32_t const mid = lo + ((hi - lo + 1) >> 1);
+      if (top_off[mid] <= i)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    uint64_t const local = i - top_off[lo];
+    uint32_t const T     = trees[lo].num_top;
+    node                 = trees[lo].top_first + static_cast<uint32_t>(local % T);
+    bbox                 = jobs[lo].poly_begin + static_cast<uint32_t>(local / T);
+  }
+  __device__ uint64_t count(uint64_t i) const
+  {
+    uint32_t node, bbox;
+    locate(i, node, bbox);
+    int const c = classify(t, bx, g, node, bbox);
+    return c == 0 ? 1ull : (c == 1 ? (1ull << 32) : 0ull);
+  }
+  __device__ void emit(uint64_t i, uint64_t prefix) const
+  {
+    uint32_t node, bbox;
+    locate(i, node, bbox);
+    int const c = classify(t, bx, g, node, bbox);
+    if (c == 0) {
+      uint64_t const o = prefix & 0xFFFFFFFFull;
+      if (o < cap_out) {
+        out_node[o] = node;
+        out_bbox[o] = bbox;
+      } else {
+        atomicOr(&meta->overflow, 1u);
+      }
+    } else if (c == 1) {
+      uint64_t const f = prefix >> 32;
+      if (f < cap_fr) {
+        fr_node[f] = node;
+        fr_bbox[f] = bbox;
+      } else {
+        atomicOr(&meta->overflow, 1u);
+      }
+    }
+  }
+};
+
+__global__ void __launch_bounds__(256)
+  gather_offsets_kernel(uint32_t const* __restrict__ out_node, uint32_t const* __restrict__ offset,
+                        uint64_t k, uint32_t* __restrict__ keys)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < k) keys[i] = offset[out_node[i]];
+}
+
+__global__ void __launch_bounds__(256)
+  permute_kernel(uint32_t const* __restrict__ perm, uint32_t const* __restrict__ out_node,
+                 uint32_t const* __restrict__ out_bbox, uint64_t k, uint32_t* __restrict__ bbox_o,
+                 uint32_t* __restrict__ quad_o)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < k) {
+    uint32_t const p = perm[i];
+    bbox_o[i]        = out_bbox[p];
+    quad_o[i]        = out_node[p];
+  }
+}
+
+// --- results back to job-local indices ---------------------------------------------
+__global__ void __launch_bounds__(256)
+  local_point_indices_kernel(uint32_t const* __restrict__ sorted_vals, uint64_t n,
+                             JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                             uint32_t* __restrict__ out)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t const j = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(i));
+  out[i]           = sorted_vals[i] - jobs[j].point_begin;
+}
+
+__global__ void __launch_bounds__(256)
+  local_hits_kernel(uint32_t const* __restrict__ poly, uint32_t const* __restrict__ pos,
+                    uint64_t h, JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                    uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= h) return;
+  uint32_t const p = pos[i];
+  uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
+  out_poly[i]      = poly[i] - jobs[j].poly_begin;
+  out_point[i]     = p - jobs[j].point_begin;
+}
+
+// First hit of every job (hits are in job order): first h with pos[h] >= point_begin.
+__global__ void __launch_bounds__(256)
+  job_hit_offsets_kernel(uint32_t const* __restrict__ pos, uint64_t h,
+                         JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                         uint64_t* __restrict__ first)
+{
+  uint32_t const j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= num_jobs) return;
+  uint32_t const b = jobs[j].point_begin;
+  uint64_t lo = 0, hi = h;
+  while (lo < hi) {
+    uint64_t const mid = lo + ((hi - lo) >> 1);
+    if (pos[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  first[j] = lo;
+}
+
+// --- bounding boxes ---------------------------------------------------------------------
+// polygon_bounding_boxes of each job's own polygons (detail/bounding_boxes.cuh:139-184):
+// one box per polygon with vertices, in order, then NaN rows for the job's empty
+// polygons (reduce_by_key emits one box per non-empty geometry run).
+__device__ __forceinline__ int64_t poly_first_vertex(int32_t const* part, int32_t const* ring,
+                                                     int64_t num_ring_offsets, int64_t nv,
+                                                     uint64_t p)
+{
+  int64_t const r = part[p];
+  int64_t v       = (r >= 0 && r < num_ring_offsets) ? ring[r] : nv;
+  if (v < 0) v = 0;
+  if (v > nv) v = nv;
+  return v;
+}
+
+struct NonEmptyOp {
+  int32_t const* part;  // the group's polygon offsets (npg + 1)
+  int32_t const* ring;
+  int64_t num_ring_offsets;
+  int64_t nv;
+  uint32_t* prefix;  // npg + 1
+  __device__ uint32_t count(uint64_t p) const
+  {
+    return poly_first_vertex(part, ring, num_ring_offsets, nv, p + 1) >
+           poly_first_vertex(part, ring, num_ring_offsets, nv, p);
+  }
+  __device__ void emit(uint64_t p, uint32_t pre) const { prefix[p] = pre; }
+};
+
+__global__ void __launch_bounds__(256)
+  batch_bbox_kernel(int32_t const* __restrict__ part, int32_t const* __restrict__ ring,
+                    int64_t num_ring_offsets, double const* __restrict__ vx,
+                    double const* __restrict__ vy, int64_t nv, uint32_t const* __restrict__ prefix,
+                    uint64_t npg, JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                    double* __restrict__ minx, double* __restrict__ miny, double* __restrict__ maxx,
+                    double* __restrict__ maxy)
+{
+  uint64_t const p = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (p >= npg) return;
+  int64_t const v0 = poly_first_vertex(part, ring, num_ring_offsets, nv, p);
+  int64_t const v1 = poly_first_vertex(part, ring, num_ring_offsets, nv, p + 1);
+  if (v1 <= v0) return;
+  // the job owning polygon p: last job whose poly_begin <= p (and that is not empty)
+  uint32_t lo = 0, hi = num_jobs - 1;
+  while (lo < hi) {
+    uint32_t const mid = lo + ((hi - lo + 1) >> 1);
+    if (jobs[mid].poly_begin <= p)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  uint32_t const pb   = jobs[lo].poly_begin;
+  uint64_t const dest = pb + (prefix[p] - prefix[pb]);
+  double const nan    = __longlong_as_double(0x7FF8000000000000ll);
+  double lx = nan, ly = nan, hx = nan, hy = nan;
+  for (int64_t v = v0 + lane_id(); v < v1; v += kWave) {
+    lx = box_min(lx, vx[v]);
+    ly = box_min(ly, vy[v]);
+    hx = box_max(hx, vx[v]);
+    hy = box_max(hy, vy[v]);
+  }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    lx = box_min(lx, __shfl_xor(lx, o, kWave));
+    ly = box_min(ly, __shfl_xor(ly, o, kWave));
+    hx = box_max(hx, __shfl_xor(hx, o, kWave));
+    hy = box_max(hy, __shfl_xor(hy, o, kWave));
+  }
+  if (lane_id() == 0) {
+    minx[dest] = lx;
+    miny[dest] = ly;
+    maxx[dest] = hx;
+    maxy[dest] = hy;
+  }
+}
+
+__global__ void fill_nan_kernel(double* __restrict__ p, uint64_t n)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = __longlong_as_double(0x7FF8000000000000ll);
+}
+
+// --- host -------------------------------------------------------------------------
+struct HostJob {
+  double min_x, max_x, min_y, max_y, scale;
+  int d;
+  int32_t max_size;
+  int key_bits;
+};
+
+int bits_for(uint64_t v)  // bits needed for values 0..v
+{
+  int b = 0;
+  while (b < 64 && (v >> b) != 0) ++b;
+  return b;
+}
+
+struct GroupResult {
+  dbuf<uint32_t> poly, point;  // job-local hits of the group
+  uint64_t hits = 0;
+};
+
+// Runs one group of jobs [j0, j1). Writes point_indices for the group's points into
+// out_pi (already offset) and returns the group's hits and per-job hit counts.
+void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t const* jpo,
+               int64_t const* jpoly, double const* points_xy, int32_t const* poly_offsets,
+               int32_t const* ring_offsets, int64_t num_ring_offsets, double const* poly_x,
+               double const* poly_y, int64_t num_vertices, uint32_t* out_pi, GroupResult& res,
+               std::vector<uint64_t>& job_hits, hipStream_t stream)
+{
+  uint32_t const J   = static_cast<uint32_t>(j1 - j0);
+  int64_t const P0   = jpo[j0], P1 = jpo[j1];
+  int64_t const PO0  = jpoly[j0], PO1 = jpoly[j1];
+  uint64_t const n   = static_cast<uint64_t>(P1 - P0);
+  uint64_t const npg = static_cast<uint64_t>(PO1 - PO0);
+  int key_bits = 1, maxd = 0;
+  for (int64_t j = j0; j < j1; ++j) {
+    key_bits = std::max(key_bits, hj[j].key_bits);
+    maxd     = std::max(maxd, hj[j].d);
+  }
+  int const job_bits = bits_for(J - 1);
+  int const sort_bits = std::min(32, key_bits + job_bits);
+
+  std::vector<JobDev> hd(J);
+  for (uint32_t k = 0; k < J; ++k) {
+    HostJob const& h = hj[j0 + k];
+    JobDev& d        = hd[k];
+    d.min_x = h.min_x, d.min_y = h.min_y, d.max_x = h.max_x, d.max_y = h.max_y;
+    d.scale       = h.scale;
+    d.point_begin = static_cast<uint32_t>(jpo[j0 + k] - P0);
+    d.point_end   = static_cast<uint32_t>(jpo[j0 + k + 1] - P0);
+    d.poly_begin  = static_cast<uint32_t>(jpoly[j0 + k] - PO0);
+    d.poly_end    = static_cast<uint32_t>(jpoly[j0 + k + 1] - PO0);
+    d.outside_key = static_cast<uint32_t>((1u << (2 * h.d)) - 1u);
+    d.d           = h.d;
+    d.max_size    = static_cast<uint32_t>(h.max_size);
+    d.shift0      = h.d >= 1 ? 2 * (h.d - 1) : 0;
+  }
+  dbuf<JobDev> jobs(J);
+  QPIP_HIP(hipMemcpyAsync(jobs.get(), hd.data(), J * sizeof(JobDev), hipMemcpyHostToDevice, stream));
+  QPIP_HIP(hipStreamSynchronize(stream));  // hd is pageable host memory
+  for (uint32_t k = 0; k < J; ++k) job_hits[j0 + k] = 0;
+  if (n == 0) return;
+
+  // 1-2. keys and the sort (values: group-local point index)
+  uint64_t const tiles = sort_tiles(n);
+  dbuf<uint32_t> keys_a(n), keys_b(n), vals(n), vals_tmp(n);
+  dbuf<uint32_t> counts0(static_cast<size_t>(tiles) * kRadix);
+  {
+    KernelTimer kt("batch_keygen", stream);
+    batch_keygen_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
+      points_xy + 2 * P0, n, jobs.get(), J, key_bits, keys_a.get(), counts0.get());
+    QPIP_LAUNCH_CHECK();
+  }
+  SortResult sorted = radix_sort_pairs(keys_a.get(), keys_b.get(), nullptr, vals.get(),
+                                       vals_tmp.get(), n, sort_bits, stream, counts0.get(),
+                                       "batch_sort");
+  vals_tmp.release();
+
+  // 3. forest levels
+  uint64_t cap = 64ull * J;
+  {
+    uint64_t per = 0;
+    for (int64_t j = j0; j < j1; ++j) {
+      uint64_t const nj = static_cast<uint64_t>(jpo[j + 1] - jpo[j]);
+      uint64_t const ib = nj / (static_cast<uint64_t>(hj[j].max_size) + 1);
+      for (int l = 1; l < std::max(hj[j].d, 1); ++l) {
+        uint64_t c = std::min<uint64_t>(nj, 4 * ib);
+        if (2 * l + 6 < 62) c = std::min<uint64_t>(c, uint64_t{64} << (2 * l));
+        per += c;
+      }
+    }
+    cap += per;
+  }
+  QPIP_EXPECTS(cap < (uint64_t{1} << 32), "batched forest too large");
+  dbuf<uint32_t> nkey(cap), nstart(cap), ncount(cap), cfirst(cap), ccount(cap), njob(cap);
+  dbuf<uint8_t> nlevel(cap);
+  dbuf<uint4> split(cap), staging(static_cast<size_t>(J) * 64);
+  dbuf<JobTree> trees(J);
+  dbuf<ForestMeta> fmeta(1);
+  dbuf<uint32_t> partials(std::max(kLevelBlocks, kScanBlocks) + 1);
+  QPIP_HIP(hipMemsetAsync(fmeta.get(), 0, sizeof(ForestMeta), stream));
+  {
+    KernelTimer kt("batch_levels", stream);
+    batch_level0_kernel<<<(J + 3) / 4, 256, 0, stream>>>(sorted.keys, jobs.get(), J, key_bits,
+                                                         staging.get(), trees.get());
+    QPIP_LAUNCH_CHECK();
+    Level0Op l0{trees.get(), staging.get(), nkey.get(), nstart.get(), ncount.get(), nlevel.get(),
+                njob.get()};
+    count_scan_emit<Level0Op, uint32_t, uint64_t>(l0, nullptr, J, partials.get(),
+                                                  &fmeta.get()->sizes[0], stream,
+                                                  scan_blocks_for(J), "batch_level0");
+    JobLevels rule{njob.get(), jobs.get(), key_bits};
+    for (int l = 0; l + 2 <= maxd; ++l) {
+      LevelArgs la{sorted.keys, nkey.get(), nstart.get(), ncount.get(), nlevel.get(),
+                   cfirst.get(), ccount.get(), split.get(), partials.get(), fmeta.get()->sizes,
+                   &fmeta.get()->overflow, cap, l};
+      level_split_kernel<<<kLevelBlocks, 256, 0, stream>>>(la, rule);
+      QPIP_LAUNCH_CHECK();
+      level_emit_kernel<<<kLevelBlocks, 256, 0, stream>>>(la, rule);
+      QPIP_LAUNCH_CHECK();
+    }
+  }
+  ForestMeta hm;
+  QPIP_HIP(hipMemcpyAsync(&hm, fmeta.get(), sizeof(ForestMeta), hipMemcpyDeviceToHost, stream));
+  QPIP_HIP(hipStreamSynchronize(stream));
+  if (hm.overflow) throw device_error("batched forest node capacity exceeded (internal error)");
+  uint64_t M = 0;
+  for (int l = 0; l < std::max(maxd, 1); ++l) M += hm.sizes[l];
+  batch_marked_kernel<<<(J + 255) / 256, 256, 0, stream>>>(nkey.get(), nstart.get(), ncount.get(),
+                                                          njob.get(), jobs.get(), J, fmeta.get(),
+                                                          trees.get());
+  QPIP_LAUNCH_CHECK();
+  dbuf<uint8_t> f_internal(std::max<uint64_t>(M, 1));
+  dbuf<uint32_t> f_length(std::max<uint64_t>(M, 1)), f_offset(std::max<uint64_t>(M, 1));
+  if (M > 0) {
+    batch_finalize_kernel<<<grid_for(M, 256), 256, 0, stream>>>(
+      nkey.get(), nstart.get(), ncount.get(), nlevel.get(), njob.get(), cfirst.get(), ccount.get(),
+      jobs.get(), trees.get(), M, f_internal.get(), f_length.get(), f_offset.get());
+    QPIP_LAUNCH_CHECK();
+  }
+  // point_indices (job-local)
+  local_point_indices_kernel<<<grid_for(n, 256), 256, 0, stream>>>(vals.get(), n, jobs.get(), J,
+                                                                    out_pi);
+  QPIP_LAUNCH_CHECK();
+  if (M == 0 || npg == 0) {
+    QPIP_HIP(hipStreamSynchronize(stream));
+    return;
+  }
+
+  // 4. bounding boxes of each job's polygons
+  dbuf<double> boxes(4 * npg);
+  double* bminx = boxes.get();
+  double* bminy = bminx + npg;
+  double* bmaxx = bminy + npg;
+  double* bmaxy = bmaxx + npg;
+  {
+    KernelTimer kt("batch_bbox", stream);
+    fill_nan_kernel<<<grid_for(4 * npg, 256), 256, 0, stream>>>(bminx, 4 * npg);
+    QPIP_LAUNCH_CHECK();
+    dbuf<uint32_t> prefix(npg + 1);
+    NonEmptyOp ne{poly_offsets + PO0, ring_offsets, num_ring_offsets, num_vertices, prefix.get()};
+    count_scan_emit<NonEmptyOp, uint32_t, uint32_t>(ne, nullptr, npg, partials.get(),
+                                                    prefix.get() + npg, stream,
+                                                    scan_blocks_for(npg), "batch_bbox_scan");
+    batch_bbox_kernel<<<grid_for(npg, 256 / kWave), 256, 0, stream>>>(
+      poly_offsets + PO0, ring_offsets, num_ring_offsets, poly_x, poly_y, num_vertices,
+      prefix.get(), npg, jobs.get(), J, bminx, bminy, bmaxx, bmaxy);
+    QPIP_LAUNCH_CHECK();
+  }
+  int64_t const nb = static_cast<int64_t>(npg);
+
+  // 5. join
+  TreeView tv{nkey.get(), nlevel.get(), f_internal.get(), f_length.get(), f_offset.get(), M};
+  BoxView bv{bminx, bminy, bmaxx, bmaxy, static_cast<uint64_t>(nb)};
+  JobGeometry geom{njob.get(), jobs.get()};
+  dbuf<uint64_t> top_off(J);
+  dbuf<JoinMeta> jmeta(1);
+  dbuf<uint64_t> partials64(kScanBlocks + 1);
+  uint64_t cap_out = std::max<uint64_t>(uint64_t{1} << 20, 8 * npg);
+  uint64_t cap_fr  = cap_out;
+  JoinMeta h_meta{};
+  dbuf<uint32_t> out_node, out_bbox, fa_node, fa_bbox, fb_node, fb_bbox;
+  for (int attempt = 0;; ++attempt) {
+    out_node.reset(cap_out);
+    out_bbox.reset(cap_out);
+    fa_node.reset(cap_fr);
+    fa_bbox.reset(cap_fr);
+    fb_node.reset(cap_fr);
+    fb_bbox.reset(cap_fr);
+    QPIP_HIP(hipMemsetAsync(jmeta.get(), 0, sizeof(JoinMeta), stream));
+    JoinMeta* dm = jmeta.get();
+    TopSizeOp ts{trees.get(), jobs.get(), top_off.get()};
+    count_scan_emit<TopSizeOp, uint64_t, uint64_t>(ts, nullptr, J, partials64.get(),
+                                                   &dm->top_pairs, stream, scan_blocks_for(J),
+                                                   "batch_join_setup");
+    BatchTopOp top{tv, bv, geom, trees.get(), jobs.get(), top_off.get(), J, dm, out_node.get(),
+                   out_bbox.get(), fa_node.get(), fa_bbox.get(), cap_out, cap_fr};
+    count_scan_emit<BatchTopOp, uint64_t, SplitTotal>(top, &dm->top_pairs, 0, partials64.get(),
+                                                      &dm->level[0], stream, kScanBlocks,
+                                                      "batch_join_level");
+    uint32_t* cur_node = fa_node.get();
+    uint32_t* cur_bbox = fa_bbox.get();
+    uint32_t* nxt_node = fb_node.get();
+    uint32_t* nxt_bbox = fb_bbox.get();
+    for (int lvl = 1; lvl < maxd; ++lvl) {
+      DescendOp<JobGeometry> op{tv, bv, geom, dm, cur_node, cur_bbox, out_node.get(),
+                                out_bbox.get(), nxt_node, nxt_bbox, cap_out, cap_fr, lvl};
+      count_scan_emit<DescendOp<JobGeometry>, uint64_t, SplitTotal>(
+        op, &dm->level[lvl - 1].quad, 0, partials64.get(), &dm->level[lvl], stream, kScanBlocks,
+        "batch_join_level");
+      std::swap(cur_node, nxt_node);
+      std::swap(cur_bbox, nxt_bbox);
+    }
+    QPIP_HIP(hipMemcpyAsync(&h_meta, dm, sizeof(JoinMeta), hipMemcpyDeviceToHost, stream));
+    QPIP_HIP(hipStreamSynchronize(stream));
+    if (h_meta.bad_index) throw logic_error("cuSpatial failure: forest child range out of range");
+    if (!h_meta.overflow) break;
+    uint64_t tot_out = 0, max_fr = h_meta.top_pairs;
+    for (int l = 0; l < std::max(maxd, 1); ++l) {
+      tot_out += h_meta.level[l].leaf;
+      max_fr = std::max<uint64_t>(max_fr, h_meta.level[l].quad);
+    }
+    cap_out = std::max<uint64_t>(cap_out * 2, tot_out + tot_out / 4 + 1024);
+    cap_fr  = std::max<uint64_t>(cap_fr * 2, max_fr + max_fr / 4 + 1024);
+    if (attempt > 8) throw device_error("join capacity growth did not converge");
+  }
+  uint64_t K = 0;
+  for (int l = 0; l < std::max(maxd, 1); ++l) K += h_meta.level[l].leaf;
+  if (K == 0) return;
+  QPIP_EXPECTS(K < (uint64_t{1} << 32), "join produced more than 2^32 pairs");
+  dbuf<uint32_t> ka(K), kb(K), va(K), vb(K), pb(K), pq(K);
+  gather_offsets_kernel<<<grid_for(K, 256), 256, 0, stream>>>(out_node.get(), f_offset.get(), K,
+                                                              ka.get());
+  QPIP_LAUNCH_CHECK();
+  SortResult ps = radix_sort_pairs(ka.get(), kb.get(), nullptr, va.get(), vb.get(), K,
+                                   std::max(1, bits_for(n)), stream, nullptr, "batch_join_sort");
+  permute_kernel<<<grid_for(K, 256), 256, 0, stream>>>(ps.values, out_node.get(), out_bbox.get(),
+                                                       K, pb.get(), pq.get());
+  QPIP_LAUNCH_CHECK();
+
+  // 6. point-in-polygon on the forest
+  uint32_t *hp = nullptr, *ht = nullptr;
+  int64_t H = 0;
+  int rc = qpip_quadtree_point_in_polygon_xy(
+    pb.get(), pq.get(), static_cast<int64_t>(K), nullptr, nullptr, nullptr, f_length.get(),
+    f_offset.get(), static_cast<int64_t>(M), vals.get(), static_cast<int64_t>(n),
+    points_xy + 2 * P0, static_cast<int64_t>(n), reinterpret_cast<uint32_t const*>(poly_offsets + PO0),
+    static_cast<int64_t>(npg) + 1, reinterpret_cast<uint32_t const*>(ring_offsets),
+    num_ring_offsets, poly_x, poly_y, num_vertices, num_vertices, nullptr, stream, &hp, &ht, &H);
+  if (rc != QPIP_SUCCESS) throw logic_error(qpip_last_error_message());
+  struct Hits {
+    uint32_t* p[2];
+    ~Hits()
+    {
+      qpip_free(p[0]);
+      qpip_free(p[1]);
+    }
+  } hits{{hp, ht}};
+  if (H == 0) return;
+  uint64_t const h = static_cast<uint64_t>(H);
+  res.poly.reset(h);
+  res.point.reset(h);
+  res.hits = h;
+  local_hits_kernel<<<grid_for(h, 256), 256, 0, stream>>>(hp, ht, h, jobs.get(), J, res.poly.get(),
+                                                          res.point.get());
+  QPIP_LAUNCH_CHECK();
+  dbuf<uint64_t> first(J);
+  job_hit_offsets_kernel<<<(J + 255) / 256, 256, 0, stream>>>(ht, h, jobs.get(), J, first.get());
+  QPIP_LAUNCH_CHECK();
+  std::vector<uint64_t> hf(J);
+  QPIP_HIP(hipMemcpyAsync(hf.data(), first.get(), J * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          stream));
+  QPIP_HIP(hipStreamSynchronize(stream));
+  for (uint32_t k = 0; k < J; ++k) job_hits[j0 + k] = (k + 1 < J ? hf[k + 1] : h) - hf[k];
+}
+
+}  // namespace
+
+}  // namespace qpip
+
+extern "C" int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
+                                                      const int64_t* job_point_offsets,
+                                                      const int64_t* job_polygon_offsets,
+                                                      const double* job_bbox,
+                                                      const double* job_scale,
+                                                      const int8_t* job_max_depth,
+                                                      const int32_t* job_max_size,
+                                                      const double* points_xy,
+                                                      int64_t num_points,
+                                                      const int32_t* poly_offsets,
+                                                      int64_t num_poly_offsets,
+                                                      const int32_t* ring_offsets,
+                                                      int64_t num_ring_offsets,
+                                                      const double* poly_x,
+                                                      const double* poly_y,
+                                                      int64_t num_poly_x,
+                                                      int64_t num_poly_y,
+                                                      const qpip_resource* mr,
+                                                      void* stream_,
+                                                      uint32_t** point_indices,
+                                                      uint32_t** polygon_index,
+                                                      uint32_t** point_index,
+                                                      int64_t* job_hit_offsets,
+                                                      int64_t* num_hits)
+{
+  using namespace qpip;
+  try {
+    QPIP_EXPECTS(point_indices && polygon_index && point_index && job_hit_offsets && num_hits,
+                 "output pointers must not be null");
+    *point_indices = *polygon_index = *point_index = nullptr;
+    *num_hits                                      = 0;
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    QPIP_EXPECTS(num_jobs >= 0, "number of jobs must not be negative");
+    if (num_jobs == 0) {
+      job_hit_offsets[0] = 0;
+      return QPIP_SUCCESS;
+    }
+    QPIP_EXPECTS(job_point_offsets && job_polygon_offsets && job_bbox && job_scale &&
+                   job_max_depth && job_max_size,
+                 "job descriptions must not be null");
+    QPIP_EXPECTS(num_poly_x == num_poly_y,
+                 "numbers of vertices must be the same for both x and y columns");
+    QPIP_EXPECTS(job_point_offsets[0] == 0 && job_point_offsets[num_jobs] == num_points,
+                 "job point offsets must start at 0 and end at the number of points");
+    int64_t const num_polys = num_poly_offsets > 0 ? num_poly_offsets - 1 : 0;
+    QPIP_EXPECTS(job_polygon_offsets[0] == 0 && job_polygon_offsets[num_jobs] == num_polys,
+                 "job polygon offsets must start at 0 and end at the number of polygons");
+    std::vector<HostJob> hj(num_jobs);
+    for (int64_t j = 0; j < num_jobs; ++j) {
+      QPIP_EXPECTS(job_point_offsets[j + 1] >= job_point_offsets[j] &&
+                     job_polygon_offsets[j + 1] >= job_polygon_offsets[j],
+                   "job offsets must not decrease");
+      HostJob& h  = hj[j];
+      double const x0 = job_bbox[4 * j], x1 = job_bbox[4 * j + 1];
+      double const y0 = job_bbox[4 * j + 2], y1 = job_bbox[4 * j + 3];
+      // indexing.py:165-189 / detail/point_quadtree.cuh:259-268
+      h.min_x    = std::min(x0, x1);
+      h.max_x    = std::max(x0, x1);
+      h.min_y    = std::min(y0, y1);
+      h.max_y    = std::max(y0, y1);
+      h.max_size = std::max(1, job_max_size[j]);
+      h.d        = std::max(0, std::min(15, static_cast<int>(job_max_depth[j])));
+      h.scale    = std::max(job_scale[j],
+                         std::max(h.max_x - h.min_x, h.max_y - h.min_y) / ((1 << h.d) + 2));
+      // quadtree_bbox_filtering.cu:102-106, for jobs that reach the join
+      if (job_point_offsets[j + 1] > job_point_offsets[j] &&
+          job_polygon_offsets[j + 1] > job_polygon_offsets[j]) {
+        QPIP_EXPECTS(h.scale > 0, "scale must be positive");
+        QPIP_EXPECTS(h.min_x < h.max_x && h.min_y < h.max_y,
+                     "invalid bounding box (x_min, x_max, y_min, y_max)");
+        QPIP_EXPECTS(h.d > 0 && h.d < 16, "maximum depth must be positive and less than 16");
+      }
+      int cbits = 0;
+      while ((uint64_t{1} << cbits) <= (uint64_t{1} << h.d) + 2) ++cbits;
+      h.key_bits = std::min(32, 2 * cbits);
+    }
+    if (num_points > 0) QPIP_EXPECTS(points_xy, "points must not be null");
+
+    OutputAllocator oa(mr, stream);
+    uint32_t* o_pi = num_points > 0 ? oa.make<uint32_t>(num_points) : nullptr;
+    std::vector<uint64_t> job_hits(num_jobs, 0);
+    std::vector<GroupResult> groups;
+    // groups of consecutive jobs whose job number and key fit 32 bits, < 2^31 points
+    int64_t j0 = 0;
+    while (j0 < num_jobs) {
+      int64_t j1  = j0 + 1;
+      int kb      = hj[j0].key_bits;
+      while (j1 < num_jobs) {
+        int const kb2 = std::max(kb, hj[j1].key_bits);
+        if (kb2 + bits_for(static_cast<uint64_t>(j1 - j0)) > 32) break;
+        if (job_point_offsets[j1 + 1] - job_point_offsets[j0] >= (int64_t{1} << 31)) break;
+        kb = kb2;
+        ++j1;
+      }
+      QPIP_EXPECTS(job_point_offsets[j1] - job_point_offsets[j0] < (int64_t{1} << 31),
+                   "a job must have fewer than 2^31 points");
+      groups.emplace_back();
+      run_group(hj, j0, j1, job_point_offsets, job_polygon_offsets, points_xy, poly_offsets,
+                ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
+                o_pi ? o_pi + job_point_offsets[j0] : nullptr, groups.back(), job_hits, stream);
+      j0 = j1;
+    }
+    uint64_t H = 0;
+    job_hit_offsets[0] = 0;
+    for (int64_t j = 0; j < num_jobs; ++j) {
+      H += job_hits[j];
+      job_hit_offsets[j + 1] = static_cast<int64_t>(H);
+    }
+    uint32_t* o_poly  = H > 0 ? oa.make<uint32_t>(H) : nullptr;
+    uint32_t* o_point = H > 0 ? oa.make<uint32_t>(H) : nullptr;
+    uint64_t at = 0;
+    for (auto& g : groups) {
+      if (g.hits == 0) continue;
+      QPIP_HIP(hipMemcpyAsync(o_poly + at, g.poly.get(), g.hits * 4, hipMemcpyDeviceToDevice, stream));
+      QPIP_HIP(hipMemcpyAsync(o_point + at, g.point.get(), g.hits * 4, hipMemcpyDeviceToDevice, stream));
+      at += g.hits;
+    }
+    QPIP_HIP(hipStreamSynchronize(stream));
+    *point_indices = o_pi;
+    *polygon_index = o_poly;
+    *point_index   = o_point;
+    *num_hits      = static_cast<int64_t>(H);
+    return QPIP_SUCCESS;
+  } catch (logic_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_LOGIC;
+  } catch (oom_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_OOM;
+  } catch (std::exception const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_DEVICE;
+  }
+}

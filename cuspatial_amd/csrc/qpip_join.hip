@@ -18,96 +18,11 @@
 // rerun with the sizes the device recorded.
 #include <algorithm>
 
-#include "qpip_scan.hpp"
+#include "qpip_join.hpp"
 
 namespace qpip {
 
 namespace {
-
-constexpr int kMaxJoinLevels = 16;
-
-struct SplitTotal {
-  uint64_t leaf;
-  uint64_t quad;
-  __device__ SplitTotal& operator=(uint64_t packed)
-  {
-    leaf = packed & 0xFFFFFFFFull;
-    quad = packed >> 32;
-    return *this;
-  }
-};
-
-struct JoinMeta {
-  SplitTotal level[kMaxJoinLevels];  // per level: leaf pairs emitted, next frontier size
-  uint64_t top_pairs;                // T * B
-  uint32_t num_top;                  // T = count(level == 0)
-  uint32_t max_leaf_offset;
-  uint32_t overflow;
-  uint32_t bad_index;
-  uint64_t need_frontier;
-  uint64_t need_output;
-};
-
-// z_order.cuh:80-94
-__device__ __forceinline__ uint32_t undilate_2(uint32_t x)
-{
-  x &= 0x55555555u;
-  x = (x | (x >> 1)) & 0x33333333u;
-  x = (x | (x >> 2)) & 0x0F0F0F0Fu;
-  x = (x | (x >> 4)) & 0x00FF00FFu;
-  x = (x | (x >> 8)) & 0x0000FFFFu;
-  return x;
-}
-
-struct TreeView {
-  uint32_t const* key;
-  uint8_t const* level;
-  uint8_t const* internal;
-  uint32_t const* length;
-  uint32_t const* offset;
-  uint64_t m;
-};
-
-struct BoxView {
-  double const* minx;
-  double const* miny;
-  double const* maxx;
-  double const* maxy;
-  uint64_t b;
-};
-
-struct Params {
-  double vmin_x, vmin_y, scale;
-  int max_depth;
-};
-
-// intersection.cuh:99-128: 0 leaf, 1 quad, 2 none. The node extent is
-// v_min + key * level_scale, which nvcc contracts to an FMA (default --fmad=true).
-__device__ __forceinline__ int classify(TreeView const& t, BoxView const& bx, Params const& p,
-                                        uint32_t node, uint32_t bbox)
-{
-  uint32_t const key = t.key[node];
-  int const level    = t.level[node];
-  double const key_x = static_cast<double>(undilate_2(key));
-  double const key_y = static_cast<double>(undilate_2(key >> 1));
-  int const shift    = p.max_depth - 1 - level;
-  double const ls    = p.scale * static_cast<double>(1 << (shift < 0 ? 0 : shift));
-  double const nx0   = fma(key_x + 0, ls, p.vmin_x);
-  double const ny0   = fma(key_y + 0, ls, p.vmin_y);
-  double const nx1   = fma(key_x + 1, ls, p.vmin_x);
-  double const ny1   = fma(key_y + 1, ls, p.vmin_y);
-  if ((nx0 > bx.maxx[bbox]) || (nx1 < bx.minx[bbox]) || (ny0 > bx.maxy[bbox]) ||
-      (ny1 < bx.miny[bbox]))
-    return 2;
-  return t.internal[node] ? 1 : 0;
-}
-
-__device__ __forceinline__ uint64_t level_out_base(JoinMeta const* meta, int lvl)
-{
-  uint64_t b = 0;
-  for (int l = 0; l < lvl; ++l) b += meta->level[l].leaf;
-  return b;
-}
 
 // Level 0: the virtual frontier i -> (i % T, i / T), each pair classified once.
 struct TopOp {
@@ -148,79 +63,6 @@ struct TopOp {
         fr_bbox[f] = bbox;
       } else {
         atomicOr(&meta->overflow, 1u);
-      }
-    }
-  }
-};
-
-// Level lvl >= 1: expand each quad pair of the previous level to its children
-// (traversal.cuh:63-145), classify, and split into leaf output / next frontier.
-struct DescendOp {
-  TreeView t;
-  BoxView bx;
-  Params p;
-  JoinMeta* meta;
-  uint32_t const* in_node;
-  uint32_t const* in_bbox;
-  uint32_t* out_node;
-  uint32_t* out_bbox;
-  uint32_t* fr_node;
-  uint32_t* fr_bbox;
-  uint64_t cap_out, cap_fr;
-  int lvl;
-
-  __device__ bool child_range(uint64_t i, uint32_t& first, uint32_t& n) const
-  {
-    // After a frontier overflow the recorded length exceeds what was stored;
-    // the level is rerun with larger buffers, so the excess is skipped here.
-    if (i >= cap_fr) return false;
-    uint32_t const parent = in_node[i];
-    first                 = t.offset[parent];
-    n                     = t.length[parent];
-    if (static_cast<uint64_t>(first) + n > t.m) {
-      atomicOr(&meta->bad_index, 1u);
-      return false;
-    }
-    return true;
-  }
-  __device__ uint64_t count(uint64_t i) const
-  {
-    uint32_t first, n;
-    if (!child_range(i, first, n)) return 0;
-    uint32_t const bbox = in_bbox[i];
-    uint64_t acc        = 0;
-    for (uint32_t j = 0; j < n; ++j) {
-      int const c = classify(t, bx, p, first + j, bbox);
-      acc += c == 0 ? 1ull : (c == 1 ? (1ull << 32) : 0ull);
-    }
-    return acc;
-  }
-  __device__ void emit(uint64_t i, uint64_t prefix) const
-  {
-    uint32_t first, n;
-    if (!child_range(i, first, n)) return;
-    uint32_t const bbox = in_bbox[i];
-    uint64_t o          = level_out_base(meta, lvl) + (prefix & 0xFFFFFFFFull);
-    uint64_t f          = prefix >> 32;
-    for (uint32_t j = 0; j < n; ++j) {
-      uint32_t const node = first + j;
-      int const c         = classify(t, bx, p, node, bbox);
-      if (c == 0) {
-        if (o < cap_out) {
-          out_node[o] = node;
-          out_bbox[o] = bbox;
-        } else {
-          atomicOr(&meta->overflow, 1u);
-        }
-        ++o;
-      } else if (c == 1) {
-        if (f < cap_fr) {
-          fr_node[f] = node;
-          fr_bbox[f] = bbox;
-        } else {
-          atomicOr(&meta->overflow, 1u);
-        }
-        ++f;
       }
     }
   }
@@ -358,9 +200,9 @@ extern "C" int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
       uint32_t* nxt_bbox = fb_bbox.get();
       // quadtree_bbox_filtering.cuh:116-163: levels 1 .. max_depth-1
       for (int lvl = 1; lvl < max_depth; ++lvl) {
-        DescendOp op{tv, bv, prm, dm, cur_node, cur_bbox, out_node.get(), out_bbox.get(),
-                     nxt_node, nxt_bbox, cap_out, cap_fr, lvl};
-        count_scan_emit<DescendOp, uint64_t, SplitTotal>(op, &dm->level[lvl - 1].quad, 0,
+        DescendOp<Params> op{tv, bv, prm, dm, cur_node, cur_bbox, out_node.get(), out_bbox.get(),
+                             nxt_node, nxt_bbox, cap_out, cap_fr, lvl};
+        count_scan_emit<DescendOp<Params>, uint64_t, SplitTotal>(op, &dm->level[lvl - 1].quad, 0,
                                                          partials64.get(), &dm->level[lvl], stream,
                                                          kScanBlocks, "join_level");
         std::swap(cur_node, nxt_node);

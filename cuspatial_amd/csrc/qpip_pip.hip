@@ -392,6 +392,101 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// --- brute force: every point against every polygon ------------------------------
+// point_in_polygon (detail/point_in_polygon.cuh:42-102): bit j of the result of point
+// i is point i inside polygon j, for at most 31 polygons. One wave per 256 points
+// (4 per lane); each polygon's edges are wave-uniform, as in the quadtree test.
+__device__ __forceinline__ void load_point(PipArgs const& a, uint64_t i, double& px, double& py)
+{
+  if (a.xy) {
+    if (a.xy_vec) {
+      double2 const p = reinterpret_cast<double2 const*>(a.xy)[i];
+      px              = p.x;
+      py              = p.y;
+    } else {
+      px = a.xy[2 * i];
+      py = a.xy[2 * i + 1];
+    }
+  } else {
+    px = a.x[i];
+    py = a.y[i];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+  pip_bitmask_kernel(PipArgs a, uint64_t n, int32_t* __restrict__ out)
+{
+  uint64_t const w  = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  uint64_t const p0 = w * 4 * kWave;
+  if (p0 >= n) return;
+  int const lane    = lane_id();
+  uint64_t const left = (n - p0 + kWave - 1) / kWave;
+  uint32_t const nu   = left < 4 ? static_cast<uint32_t>(left) : 4u;
+  double px[4] = {0.0, 0.0, 0.0, 0.0}, py[4] = {0.0, 0.0, 0.0, 0.0};
+  uint64_t valid_m[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    uint64_t const i = p0 + static_cast<uint64_t>(u) * kWave + lane;
+    bool const valid = i < n;
+    if (valid) load_point(a, i, px[u], py[u]);
+    valid_m[u] = ballot(valid);
+  }
+  uint32_t bits[4] = {0, 0, 0, 0};
+  bool err         = false;
+  for (uint32_t j = 0; j < a.num_polys; ++j) {
+    uint64_t hit_m[4] = {0, 0, 0, 0};
+    switch (nu) {
+      case 1: test_polygon<1>(a, j, px, py, valid_m, hit_m, err); break;
+      case 2: test_polygon<2>(a, j, px, py, valid_m, hit_m, err); break;
+      case 3: test_polygon<3>(a, j, px, py, valid_m, hit_m, err); break;
+      default: test_polygon<4>(a, j, px, py, valid_m, hit_m, err); break;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) bits[u] |= static_cast<uint32_t>((hit_m[u] >> lane) & 1ull) << j;
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    uint64_t const i = p0 + static_cast<uint64_t>(u) * kWave + lane;
+    if (i < n) out[i] = static_cast<int32_t>(bits[u]);
+  }
+}
+
+// pairwise_point_in_polygon (detail/point_in_polygon.cuh:104-160): point i against
+// polygon i. Each lane has its own polygon, so the edges are read per lane.
+__global__ void __launch_bounds__(256)
+  pip_pairwise_kernel(PipArgs a, uint64_t n, uint8_t* __restrict__ out)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double px, py;
+  load_point(a, i, px, py);
+  uint32_t r0 = a.poly_offsets[i], r1 = a.poly_offsets[i + 1];
+  if (r1 > a.num_rings) r1 = static_cast<uint32_t>(a.num_rings);
+  bool inside = false, on_edge = false;
+  for (uint32_t r = r0; r < r1; ++r) {
+    uint32_t const e0 = a.ring_offsets[r];
+    uint32_t const ne = a.ring_edges[r];
+    if (ne == 0) continue;
+    bool y0 = a.edges_b[e0].by > py;
+    for (uint32_t e = 0; e < ne; ++e) {
+      Edge const E      = a.edges[e0 + e];
+      double const lhs  = (px - E.ax) * E.rise;  // run_to_point * rise
+      double const rhs  = E.run * (py - E.ay);   // run * rise_to_point
+      if (float_equal(rhs, lhs)) {
+        // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
+        double const bx   = a.edges_b[e0 + e].bx;
+        double const minx = E.ax < bx ? E.ax : bx;
+        double const maxx = E.ax < bx ? bx : E.ax;
+        if (minx <= px && px <= maxx) on_edge = true;
+      }
+      bool const y1 = E.ay > py;
+      if (y1 != y0 && ((lhs < rhs) != y1)) inside = !inside;
+      y0 = y1;
+    }
+  }
+  out[i] = (inside && !on_edge) ? 1 : 0;
+}
+
 }  // namespace
 
 }  // namespace qpip
@@ -610,4 +705,149 @@ extern "C" int qpip_quadtree_point_in_polygon_xy(const uint32_t* poly_index,
     point_indices, num_point_indices, nullptr, nullptr, points_xy, num_points, num_points,
     poly_offsets, num_poly_offsets, ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
     num_poly_y, mr, stream, polygon_index, point_index, num_hits);
+}
+
+namespace {
+
+// cpp/src/point_in_polygon/point_in_polygon.cu:105-148 (column API checks), shared by
+// the bitmask and pairwise entry points.
+int point_in_polygon_entry(const double* x,
+                           const double* y,
+                           const double* xy,
+                           int64_t num_x,
+                           int64_t num_y,
+                           const int32_t* poly_offsets,
+                           int64_t num_poly_offsets,
+                           const int32_t* ring_offsets,
+                           int64_t num_ring_offsets,
+                           const double* poly_x,
+                           const double* poly_y,
+                           int64_t num_poly_x,
+                           int64_t num_poly_y,
+                           const qpip_resource* mr,
+                           void* stream_,
+                           bool pairwise,
+                           void** result,
+                           int64_t* num_results)
+{
+  using namespace qpip;
+  try {
+    QPIP_EXPECTS(result && num_results, "output pointers must not be null");
+    *result            = nullptr;
+    *num_results       = 0;
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    QPIP_EXPECTS(num_x == num_y && num_poly_x == num_poly_y,
+                 "All points must have both x and y values");
+    int64_t const num_polys = num_poly_offsets > 0 ? num_poly_offsets - 1 : 0;
+    if (pairwise) {
+      QPIP_EXPECTS(num_x == num_polys, "Must pass in the same number of points as polygons.");
+    } else {
+      // detail/point_in_polygon.cuh:94-95
+      QPIP_EXPECTS(num_polys <= 31, "Number of polygons cannot exceed 31");
+    }
+    if (num_x <= 0) return QPIP_SUCCESS;
+    QPIP_EXPECTS(num_x < (int64_t{1} << 32), "inputs must have fewer than 2^32 rows");
+    uint64_t const n         = static_cast<uint64_t>(num_x);
+    uint64_t const num_rings = num_ring_offsets > 0 ? static_cast<uint64_t>(num_ring_offsets - 1) : 0;
+    if (num_polys > 0) QPIP_EXPECTS(poly_offsets && ring_offsets, "offsets must not be null");
+
+    dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
+    dbuf<EdgeB> edges_b(std::max<int64_t>(num_poly_x, 1));
+    dbuf<uint32_t> ring_edges(std::max<uint64_t>(num_rings, 1));
+    if (num_rings > 0) {
+      KernelTimer kt("pip_edges", stream);
+      build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
+        reinterpret_cast<uint32_t const*>(ring_offsets), static_cast<int64_t>(num_rings), poly_x,
+        poly_y, num_poly_x, edges.get(), edges_b.get(), ring_edges.get());
+      QPIP_LAUNCH_CHECK();
+    }
+    PipArgs a{};
+    a.num_points   = n;
+    a.x            = x;
+    a.y            = y;
+    a.xy           = xy;
+    a.xy_vec       = xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0;
+    a.poly_offsets = reinterpret_cast<uint32_t const*>(poly_offsets);
+    a.num_polys    = static_cast<uint64_t>(num_polys);
+    a.num_rings    = num_rings;
+    a.ring_offsets = reinterpret_cast<uint32_t const*>(ring_offsets);
+    a.ring_edges   = ring_edges.get();
+    a.edges        = edges.get();
+    a.edges_b      = edges_b.get();
+    OutputAllocator oa(mr, stream);
+    if (pairwise) {
+      uint8_t* out = oa.make<uint8_t>(n);
+      KernelTimer kt("pip_pairwise", stream);
+      pip_pairwise_kernel<<<grid_for(n, 256), 256, 0, stream>>>(a, n, out);
+      QPIP_LAUNCH_CHECK();
+      *result = out;
+    } else {
+      int32_t* out = oa.make<int32_t>(n);
+      KernelTimer kt("pip_bitmask", stream);
+      uint64_t const waves = (n + 4 * kWave - 1) / (4 * kWave);
+      pip_bitmask_kernel<<<static_cast<unsigned>((waves + 3) / 4), 256, 0, stream>>>(a, n, out);
+      QPIP_LAUNCH_CHECK();
+      *result = out;
+    }
+    QPIP_HIP(hipStreamSynchronize(stream));
+    *num_results = num_x;
+    return QPIP_SUCCESS;
+  } catch (logic_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_LOGIC;
+  } catch (oom_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_OOM;
+  } catch (std::exception const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_DEVICE;
+  }
+}
+
+}  // namespace
+
+extern "C" int qpip_point_in_polygon(const double* x,
+                                     const double* y,
+                                     int64_t num_x,
+                                     int64_t num_y,
+                                     const int32_t* poly_offsets,
+                                     int64_t num_poly_offsets,
+                                     const int32_t* ring_offsets,
+                                     int64_t num_ring_offsets,
+                                     const double* poly_x,
+                                     const double* poly_y,
+                                     int64_t num_poly_x,
+                                     int64_t num_poly_y,
+                                     const qpip_resource* mr,
+                                     void* stream,
+                                     int32_t** result,
+                                     int64_t* num_results)
+{
+  return point_in_polygon_entry(x, y, nullptr, num_x, num_y, poly_offsets, num_poly_offsets,
+                                ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
+                                num_poly_y, mr, stream, false, reinterpret_cast<void**>(result),
+                                num_results);
+}
+
+extern "C" int qpip_pairwise_point_in_polygon(const double* x,
+                                              const double* y,
+                                              int64_t num_x,
+                                              int64_t num_y,
+                                              const int32_t* poly_offsets,
+                                              int64_t num_poly_offsets,
+                                              const int32_t* ring_offsets,
+                                              int64_t num_ring_offsets,
+                                              const double* poly_x,
+                                              const double* poly_y,
+                                              int64_t num_poly_x,
+                                              int64_t num_poly_y,
+                                              const qpip_resource* mr,
+                                              void* stream,
+                                              uint8_t** result,
+                                              int64_t* num_results)
+{
+  return point_in_polygon_entry(x, y, nullptr, num_x, num_y, poly_offsets, num_poly_offsets,
+                                ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
+                                num_poly_y, mr, stream, true, reinterpret_cast<void**>(result),
+                                num_results);
 }

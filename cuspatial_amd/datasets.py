@@ -90,6 +90,42 @@ def _star_rings(rng, cx, cy, radius, counts, jitter=0.3):
     return ox, oy, out_counts
 
 
+def _star_rings_sorted_spacings(rng, cx, cy, radius, counts, jitter=0.3):
+    """Closed star-shaped rings whose sorted angles are drawn directly: the
+    normalised partial sums of k + 1 exponential spacings are k sorted uniforms
+    (no sort, so it scales to ~1e8 vertices)."""
+    n = counts.size
+    k1 = counts + 1
+    total1 = int(k1.sum())
+    ring_id1 = np.repeat(np.arange(n), k1)
+    e = rng.exponential(1.0, total1)
+    cs = np.cumsum(e)
+    ends = np.cumsum(k1) - 1
+    starts1 = ends - counts
+    base = np.concatenate([[0.0], cs[ends[:-1]]])
+    tot = cs[ends] - base
+    frac = (cs - base[ring_id1]) / tot[ring_id1]
+    keep = np.ones(total1, dtype=bool)
+    keep[ends] = False  # the last partial sum is 1
+    ang = 2.0 * math.pi * frac[keep]
+    r = np.repeat(radius, counts) * rng.uniform(1.0 - jitter, 1.0 + jitter, ang.size)
+    vx = np.repeat(cx, counts) + r * np.cos(ang)
+    vy = np.repeat(cy, counts) + r * np.sin(ang)
+    # close each ring: first vertex repeated at the end
+    out_counts = counts + 1
+    out_starts = np.concatenate([[0], np.cumsum(out_counts)[:-1]])
+    starts = out_starts - np.arange(n)
+    ox = np.empty(vx.size + n)
+    oy = np.empty(vy.size + n)
+    dst = np.arange(vx.size) + np.repeat(np.arange(n), counts)
+    ox[dst] = vx
+    oy[dst] = vy
+    ox[out_starts + counts] = vx[starts]
+    oy[out_starts + counts] = vy[starts]
+    del starts1
+    return ox, oy, out_counts
+
+
 def uniform_stars(seed: int = 1001, num_points: int = 1_000_000, num_polygons: int = 1000) -> Workload:
     pts = np.random.Generator(np.random.PCG64([seed, 0]))
     rng = np.random.Generator(np.random.PCG64([seed, 1]))
@@ -237,4 +273,97 @@ def subsample(w: Workload, n: int) -> Workload:
                     w.max_depth, w.max_size, w.seed)
 
 
-WORKLOADS = {"uniform_stars": uniform_stars, "gaussian_holes": gaussian_holes}
+@dataclass
+class BatchWorkload:
+    """Many independent jobs: job j owns points [job_point_offsets[j], [j+1]) of the
+    interleaved xy array and polygons [job_polygon_offsets[j], [j+1]) of the
+    GeoArrow polygon arrays, with its own bbox, scale, depth and max_size."""
+
+    name: str
+    xy: object  # interleaved x, y (numpy array or torch tensor)
+    job_point_offsets: np.ndarray
+    part_offset: np.ndarray
+    ring_offset: np.ndarray
+    px: object
+    py: object
+    job_polygon_offsets: np.ndarray
+    bboxes: np.ndarray  # (J, 4): x_min, x_max, y_min, y_max
+    scales: np.ndarray
+    max_depths: np.ndarray
+    max_sizes: np.ndarray
+    seed: int
+
+    @property
+    def num_jobs(self) -> int:
+        return len(self.job_point_offsets) - 1
+
+    @property
+    def num_points(self) -> int:
+        return int(self.job_point_offsets[-1])
+
+    def job_points(self, j):
+        """(x, y) of job j as host arrays."""
+        a, b = int(self.job_point_offsets[j]), int(self.job_point_offsets[j + 1])
+        xy = self.xy[2 * a:2 * b]
+        xy = xy.cpu().numpy() if hasattr(xy, "cpu") else np.asarray(xy)
+        return xy[0::2].copy(), xy[1::2].copy()
+
+    def job_polygons(self, j):
+        """Job j's polygons as standalone GeoArrow arrays (offsets rebased)."""
+        a, b = int(self.job_polygon_offsets[j]), int(self.job_polygon_offsets[j + 1])
+        part = np.asarray(self.part_offset[a:b + 1], dtype=np.int64)
+        r0, r1 = int(part[0]), int(part[-1])
+        ring = np.asarray(self.ring_offset[r0:r1 + 1], dtype=np.int64)
+        v0, v1 = int(ring[0]), int(ring[-1])
+        px = self.px.cpu().numpy() if hasattr(self.px, "cpu") else np.asarray(self.px)
+        py = self.py.cpu().numpy() if hasattr(self.py, "cpu") else np.asarray(self.py)
+        return ((part - r0).astype(np.int32), (ring - v0).astype(np.int32),
+                px[v0:v1].copy(), py[v0:v1].copy())
+
+
+def batched_jobs(seed: int = 1003, num_jobs: int = 4096, min_points: int = 1_000,
+                 max_points: int = 1_000_000, min_polygons: int = 1, max_polygons: int = 2000,
+                 min_vertices: int = 4, max_vertices: int = 256, ext: float = 1e4,
+                 device=None) -> BatchWorkload:
+    """The batched-join workload: per job, log-uniform point and polygon counts,
+    points uniform in a job-local ext-wide square, star polygons (closed single
+    rings, radius U[2,10] * ext/1000, vertex count uniform), max_depth =
+    ceil(log4(points / 128)) (at least 1), max_size 128, scale = ext / 2^depth.
+    With `device` the points are drawn there (torch, seeded) and returned as a
+    tensor; the polygons are host arrays."""
+    rng = np.random.Generator(np.random.PCG64([seed, 2]))
+    npts = np.round(np.exp(rng.uniform(math.log(min_points), math.log(max_points), num_jobs)))
+    npts = npts.astype(np.int64)
+    npoly = np.round(np.exp(rng.uniform(math.log(min_polygons), math.log(max_polygons),
+                                        num_jobs))).astype(np.int64)
+    depth = np.maximum(1, np.ceil(np.log(np.maximum(npts, 1) / 128.0) / math.log(4.0)))
+    depth = np.minimum(depth, 15).astype(np.int8)
+    jpo = np.concatenate([[0], np.cumsum(npts)]).astype(np.int64)
+    jpy = np.concatenate([[0], np.cumsum(npoly)]).astype(np.int64)
+    N, P = int(jpo[-1]), int(jpy[-1])
+    if device is not None:
+        import torch
+
+        gen = torch.Generator(device=device)
+        gen.manual_seed(seed)
+        xy = torch.rand(2 * N, generator=gen, device=device, dtype=torch.float64) * ext
+        xy = torch.where(xy >= ext, torch.zeros_like(xy), xy)
+    else:
+        prng = np.random.Generator(np.random.PCG64([seed, 0]))
+        xy = prng.uniform(0.0, ext, 2 * N)
+    grng = np.random.Generator(np.random.PCG64([seed, 1]))
+    cx = grng.uniform(0.0, ext, P)
+    cy = grng.uniform(0.0, ext, P)
+    radius = grng.uniform(2.0, 10.0, P) * (ext / 1000.0)
+    counts = grng.integers(min_vertices, max_vertices + 1, P)
+    vx, vy, rc = _star_rings_sorted_spacings(grng, cx, cy, radius, counts)
+    ring_offset = np.concatenate([[0], np.cumsum(rc)]).astype(np.int32)
+    part_offset = np.arange(P + 1, dtype=np.int32)
+    bboxes = np.tile(np.array([0.0, ext, 0.0, ext]), (num_jobs, 1))
+    scales = ext / np.power(2.0, depth.astype(np.float64))
+    return BatchWorkload("batched_jobs", xy, jpo, part_offset, ring_offset, vx, vy, jpy, bboxes,
+                         scales, depth, np.full(num_jobs, 128, np.int32), seed)
+
+
+WORKLOADS = {"uniform_stars": uniform_stars, "gaussian_holes": gaussian_holes,
+             "batched_jobs": batched_jobs}

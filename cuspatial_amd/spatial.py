@@ -6,6 +6,8 @@ warnings and column names):
   join_quadtree_and_bounding_boxes  python/cuspatial/cuspatial/core/spatial/join.py:105-175
   quadtree_point_in_polygon         python/cuspatial/cuspatial/core/spatial/join.py:178-262
   polygon_bounding_boxes            python/cuspatial/cuspatial/core/spatial/bounding.py:19-80
+  point_in_polygon                  python/cuspatial/cuspatial/core/spatial/join.py:23-101
+  pairwise_point_in_polygon         cpp/include/cuspatial/point_in_polygon.hpp:124-131
 
 cuDF is not used: points and polygons are given as :class:`Points` /
 :class:`Polygons` (GeoArrow offset + coordinate arrays, as the reference's
@@ -304,3 +306,127 @@ def quadtree_point_in_polygon(poly_quad_pairs, quadtree, point_indices, points: 
     out["polygon_index"] = res.take(po.value, h.value, torch.uint32)
     out["point_index"] = res.take(pt.value, h.value, torch.uint32)
     return out
+
+
+def _brute_force(points: Points, polygons: Polygons, pairwise: bool):
+    if not isinstance(points, Points):
+        raise ValueError("`points` must contain only points")
+    if not isinstance(polygons, Polygons):
+        raise ValueError("`polygons` must contain only polygons")
+    if polygons.geometry_offset is not None and \
+            polygons.geometry_offset.numel() != polygons.part_offset.numel():
+        raise ValueError("GeoSeries cannot contain multipolygon.")
+    dev = points.device
+    res = _capi.TorchResource(dev)
+    out = _vp()
+    n = ctypes.c_int64()
+    x = points.x.contiguous()
+    y = points.y.contiguous()
+    fn = (_capi.lib().qpip_pairwise_point_in_polygon if pairwise
+          else _capi.lib().qpip_point_in_polygon)
+    _capi.check(fn(_p(x), _p(y), x.numel(), y.numel(), _p(polygons.part_offset),
+                   polygons.part_offset.numel(), _p(polygons.ring_offset),
+                   polygons.ring_offset.numel(), _p(polygons.x), _p(polygons.y),
+                   polygons.x.numel(), polygons.y.numel(), ctypes.byref(res.struct),
+                   _capi.stream_handle(dev), ctypes.byref(out), ctypes.byref(n)))
+    return res.take(out.value, n.value, torch.uint8 if pairwise else torch.int32)
+
+
+def point_in_polygon(points: Points, polygons: Polygons):
+    """Which points fall within which polygons, every point against every
+    polygon (join.py:23-101); at most 31 polygons. Returns a Table with one bool
+    column per polygon (keyed by polygon position), one row per point."""
+    if len(polygons) == 0:
+        return Table()
+    bits = _brute_force(points, polygons, pairwise=False)
+    out = Table()
+    for j in range(len(polygons)):
+        out[j] = ((bits >> j) & 1).to(torch.bool)
+    return out
+
+
+def point_in_polygon_bitmask(points: Points, polygons: Polygons) -> torch.Tensor:
+    """The INT32 bitmask column of cuspatial::point_in_polygon (bit j: polygon j)."""
+    return _brute_force(points, polygons, pairwise=False)
+
+
+def pairwise_point_in_polygon(points: Points, polygons: Polygons) -> torch.Tensor:
+    """Point i against polygon i (cuspatial::pairwise_point_in_polygon); bool per pair."""
+    return _brute_force(points, polygons, pairwise=True).to(torch.bool)
+
+
+# ---------------------------------------------------------------------------
+# Batched jobs
+# ---------------------------------------------------------------------------
+class BatchResult:
+    """Results of batched_quadtree_point_in_polygon. Job j's point_indices are
+    point_indices[job_point_offsets[j]:job_point_offsets[j+1]] and its hits
+    polygon_index / point_index[job_hit_offsets[j]:job_hit_offsets[j+1]], all with
+    job-local values, exactly as the single-job calls return them."""
+
+    def __init__(self, point_indices, polygon_index, point_index, job_point_offsets,
+                 job_hit_offsets):
+        self.point_indices = point_indices
+        self.polygon_index = polygon_index
+        self.point_index = point_index
+        self.job_point_offsets = job_point_offsets
+        self.job_hit_offsets = job_hit_offsets
+
+    def __len__(self):
+        return len(self.job_hit_offsets) - 1
+
+    @property
+    def num_hits(self):
+        return int(self.job_hit_offsets[-1])
+
+    def job(self, j):
+        """(point_indices, Table(polygon_index, point_index)) of job j."""
+        p0, p1 = int(self.job_point_offsets[j]), int(self.job_point_offsets[j + 1])
+        h0, h1 = int(self.job_hit_offsets[j]), int(self.job_hit_offsets[j + 1])
+        out = Table()
+        out["polygon_index"] = self.polygon_index[h0:h1]
+        out["point_index"] = self.point_index[h0:h1]
+        return self.point_indices[p0:p1], out
+
+
+def batched_quadtree_point_in_polygon(points: Points, job_point_offsets, polygons: Polygons,
+                                      job_polygon_offsets, bboxes, scales, max_depths,
+                                      max_sizes) -> BatchResult:
+    """quadtree_on_points -> polygon_bounding_boxes -> join_quadtree_and_bounding_boxes
+    -> quadtree_point_in_polygon for many independent jobs in one pass of each
+    stage. Job j owns points [job_point_offsets[j], job_point_offsets[j+1]) and
+    polygons [job_polygon_offsets[j], job_polygon_offsets[j+1]); bboxes holds
+    (x_min, x_max, y_min, y_max) per job. Per job the result equals the four
+    single-job calls with the same parameters (bbox ordering and scale clamp
+    included; the clamp warning is not repeated per job)."""
+    if not isinstance(points, Points):
+        raise ValueError("GeoSeries must contain only points.")
+    if not isinstance(polygons, Polygons):
+        raise ValueError("`polygons` Geoseries must contains only polygons geometries.")
+    jpo = np.ascontiguousarray(np.asarray(job_point_offsets, dtype=np.int64))
+    jpy = np.ascontiguousarray(np.asarray(job_polygon_offsets, dtype=np.int64))
+    J = len(jpo) - 1
+    if len(jpy) != J + 1:
+        raise ValueError("job_point_offsets and job_polygon_offsets must have the same length")
+    bb = np.ascontiguousarray(np.asarray(bboxes, dtype=np.float64).reshape(J, 4))
+    sc = np.ascontiguousarray(np.asarray(scales, dtype=np.float64).reshape(J))
+    md = np.ascontiguousarray(np.asarray(max_depths, dtype=np.int8).reshape(J))
+    ms = np.ascontiguousarray(np.asarray(max_sizes, dtype=np.int32).reshape(J))
+    dev = points.device
+    xy = points.xy if points.xy is not None else \
+        torch.stack([points.x, points.y], dim=1).reshape(-1).contiguous()
+    res = _capi.TorchResource(dev)
+    pi, po, pt = _outs(3)
+    hits = np.zeros(J + 1, dtype=np.int64)
+    h = ctypes.c_int64()
+    ptr = lambda a: a.ctypes.data_as(_vp)  # noqa: E731
+    _capi.check(_capi.lib().qpip_batched_quadtree_point_in_polygon(
+        J, ptr(jpo), ptr(jpy), ptr(bb), ptr(sc), ptr(md), ptr(ms), _p(xy), len(points),
+        _p(polygons.part_offset), polygons.part_offset.numel(), _p(polygons.ring_offset),
+        polygons.ring_offset.numel(), _p(polygons.x), _p(polygons.y), polygons.x.numel(),
+        polygons.y.numel(), ctypes.byref(res.struct), _capi.stream_handle(dev), ctypes.byref(pi),
+        ctypes.byref(po), ctypes.byref(pt), ptr(hits), ctypes.byref(h)))
+    n = len(points)
+    return BatchResult(res.take(pi.value, n if J > 0 else 0, torch.uint32),
+                       res.take(po.value, h.value, torch.uint32),
+                       res.take(pt.value, h.value, torch.uint32), jpo, hits)

@@ -281,6 +281,108 @@ QPIP_API int qpip_quadtree_point_in_polygon_xy(const uint32_t* poly_index,
                                                uint32_t** point_index,
                                                int64_t* num_hits);
 
+/*
+ * point_in_polygon (brute force, every point against every polygon)
+ * Replaces cuspatial::point_in_polygon
+ *   (cpp/include/cuspatial/point_in_polygon.hpp:75-82,
+ *    cpp/src/point_in_polygon/point_in_polygon.cu:105-148;
+ *    algorithm detail/point_in_polygon.cuh:42-102).
+ * Outputs one INT32 per point (*num_results = num_x): bit j is set when the point
+ * is inside polygon j (same test as the quadtree path). At most 31 polygons.
+ * Errors: "All points must have both x and y values",
+ * "Number of polygons cannot exceed 31".
+ */
+QPIP_API int qpip_point_in_polygon(const double* x,
+                                   const double* y,
+                                   int64_t num_x,
+                                   int64_t num_y,
+                                   const int32_t* poly_offsets,
+                                   int64_t num_poly_offsets,
+                                   const int32_t* ring_offsets,
+                                   int64_t num_ring_offsets,
+                                   const double* poly_x,
+                                   const double* poly_y,
+                                   int64_t num_poly_x,
+                                   int64_t num_poly_y,
+                                   const qpip_resource* mr,
+                                   void* stream,
+                                   int32_t** result,
+                                   int64_t* num_results);
+
+/*
+ * pairwise_point_in_polygon (point i against polygon i)
+ * Replaces cuspatial::pairwise_point_in_polygon
+ *   (cpp/include/cuspatial/point_in_polygon.hpp:124-131,
+ *    cpp/src/point_in_polygon/point_in_polygon.cu:105-148;
+ *    algorithm detail/point_in_polygon.cuh:104-160).
+ * Outputs one UINT8 (0/1) per pair.
+ * Errors: "All points must have both x and y values",
+ * "Must pass in the same number of points as polygons.".
+ */
+QPIP_API int qpip_pairwise_point_in_polygon(const double* x,
+                                            const double* y,
+                                            int64_t num_x,
+                                            int64_t num_y,
+                                            const int32_t* poly_offsets,
+                                            int64_t num_poly_offsets,
+                                            const int32_t* ring_offsets,
+                                            int64_t num_ring_offsets,
+                                            const double* poly_x,
+                                            const double* poly_y,
+                                            int64_t num_poly_x,
+                                            int64_t num_poly_y,
+                                            const qpip_resource* mr,
+                                            void* stream,
+                                            uint8_t** result,
+                                            int64_t* num_results);
+
+/*
+ * Batched quadtree point-in-polygon (no counterpart in the reference's API: it
+ * runs the reference's four-call sequence for many independent jobs at once).
+ * For every job j the results are exactly those of
+ *   qpip_quadtree_on_points_xy -> qpip_polygon_bounding_boxes ->
+ *   qpip_join_quadtree_and_bounding_boxes -> qpip_quadtree_point_in_polygon_xy
+ * on the job's points and polygons with the job's bbox, scale, max_depth and
+ * max_size, after the Python layer's bbox ordering and scale clamp
+ * (python/cuspatial/cuspatial/core/spatial/indexing.py:165-189, join.py:144-156).
+ * Job descriptions are HOST arrays: job_point_offsets and job_polygon_offsets
+ * (num_jobs + 1 each; job j owns points [job_point_offsets[j], [j+1]) of points_xy
+ * and polygons [job_polygon_offsets[j], [j+1]) of the GeoArrow polygon arrays),
+ * job_bbox (4 per job: x_min, x_max, y_min, y_max), job_scale, job_max_depth,
+ * job_max_size. Device inputs: points_xy (interleaved, 2 * num_points doubles)
+ * and the polygons.
+ * Outputs (device): point_indices (num_points; job j's point_indices at
+ * [job_point_offsets[j], ...), values job-local), polygon_index and point_index
+ * (*num_hits; job j's hits at [job_hit_offsets[j], job_hit_offsets[j+1]),
+ * job-local, in the single-job order). job_hit_offsets is a HOST array of
+ * num_jobs + 1 entries filled here.
+ * Errors: the preconditions of the four single-job calls, per job.
+ */
+QPIP_API int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
+                                                    const int64_t* job_point_offsets,
+                                                    const int64_t* job_polygon_offsets,
+                                                    const double* job_bbox,
+                                                    const double* job_scale,
+                                                    const int8_t* job_max_depth,
+                                                    const int32_t* job_max_size,
+                                                    const double* points_xy,
+                                                    int64_t num_points,
+                                                    const int32_t* poly_offsets,
+                                                    int64_t num_poly_offsets,
+                                                    const int32_t* ring_offsets,
+                                                    int64_t num_ring_offsets,
+                                                    const double* poly_x,
+                                                    const double* poly_y,
+                                                    int64_t num_poly_x,
+                                                    int64_t num_poly_y,
+                                                    const qpip_resource* mr,
+                                                    void* stream,
+                                                    uint32_t** point_indices,
+                                                    uint32_t** polygon_index,
+                                                    uint32_t** point_index,
+                                                    int64_t* job_hit_offsets,
+                                                    int64_t* num_hits);
+
 #ifdef __cplusplus
 }
 #endif

@@ -66,6 +66,8 @@ def lib():
         L.oq_is_point_in_polygon.restype = ctypes.c_int
         L.oq_point_in_polygon_bitmask.argtypes = [
             _c_double_p, _c_double_p, i64, _c_u32_p, i64, _c_u32_p, _c_double_p, _c_double_p, _c_i32_p]
+        L.oq_pairwise_point_in_polygon.argtypes = [
+            _c_double_p, _c_double_p, i64, _c_u32_p, _c_u32_p, _c_double_p, _c_double_p, _c_u8_p]
         L.oq_all_pairs_bruteforce.argtypes = [
             _c_double_p, _c_double_p, i64, _c_u32_p, i64, _c_u32_p, _c_double_p, _c_double_p]
         L.oq_all_pairs_bruteforce.restype = vp
@@ -165,6 +167,23 @@ def point_in_polygon_bitmask(x, y, poly_offsets, ring_offsets, vx, vy):
         _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double), len(x), _ptr(po, ctypes.c_uint32),
         len(po), _ptr(ro, ctypes.c_uint32), _ptr(vx, ctypes.c_double), _ptr(vy, ctypes.c_double),
         _ptr(out, ctypes.c_int32))
+    return out
+
+
+def pairwise_point_in_polygon(x, y, poly_offsets, ring_offsets, vx, vy):
+    """Point i against polygon i (0/1 per pair)."""
+    x = _arr(x, np.float64)
+    y = _arr(y, np.float64)
+    po = _arr(poly_offsets, np.uint32)
+    ro = _arr(ring_offsets, np.uint32)
+    vx = _arr(vx, np.float64)
+    vy = _arr(vy, np.float64)
+    assert len(po) == len(x) + 1
+    out = np.zeros(len(x), dtype=np.uint8)
+    lib().oq_pairwise_point_in_polygon(
+        _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double), len(x), _ptr(po, ctypes.c_uint32),
+        _ptr(ro, ctypes.c_uint32), _ptr(vx, ctypes.c_double), _ptr(vy, ctypes.c_double),
+        _ptr(out, ctypes.c_uint8))
     return out
 
 

@@ -589,6 +589,25 @@ void oq_point_in_polygon_bitmask(const double* x,
   }
 }
 
+// pairwise_point_in_polygon (detail/point_in_polygon.cuh:104-160 with
+// pairwise_point_in_polygon_functor): point i against polygon i, 0 or 1.
+void oq_pairwise_point_in_polygon(const double* x,
+                                  const double* y,
+                                  int64_t n,
+                                  const uint32_t* poly_offsets,
+                                  const uint32_t* ring_offsets,
+                                  const double* vx,
+                                  const double* vy,
+                                  uint8_t* out)
+{
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i)
+    out[i] = is_point_in_polygon(x[i], y[i], ring_offsets, poly_offsets[i], poly_offsets[i + 1],
+                                 vx, vy)
+               ? 1
+               : 0;
+}
+
 // All (polygon, point) pairs with the point strictly inside, by brute force over
 // every polygon and every point, ordered by point then polygon. Used as the
 // set-equality check of the reference's large test (quadtree_point_in_polygon_test_large.cu:170-224).

@@ -173,6 +173,70 @@ def pip_kats():
     return cases
 
 
+def pairwise_pip_kats():
+    """cpp/tests/point_in_polygon/pairwise_point_in_polygon_test.cu: point i is
+    tested against polygon i. Cases that the reference runs as several calls
+    against one polygon range are written as one call with that polygon repeated.
+
+    Each case: points, part_offsets, ring_offsets, vertices, expected (0/1 per pair)."""
+    src = "cpp/tests/point_in_polygon/pairwise_point_in_polygon_test.cu"
+    sq = [[-1.0, -1.0], [1.0, -1.0], [1.0, 1.0], [-1.0, 1.0], [-1.0, -1.0]]
+    sq8 = [[-2.0, 0.0], [2.0, 0.0], [0.0, -2.0], [0.0, 2.0],
+           [-0.5, 0.0], [0.5, 0.0], [0.0, -0.5], [0.0, 0.5]]
+
+    def repeat(vertex_lists, counts):
+        """polygons given as lists of rings (lists of vertices), in pair order"""
+        parts, rings, verts = [0], [0], []
+        for poly in vertex_lists:
+            for ring in poly:
+                verts += ring
+                rings.append(len(verts))
+            parts.append(len(rings) - 1)
+        return parts, rings, verts
+
+    cases = []
+    p, r, v = repeat([[sq]] * 8, None)
+    cases.append(dict(name="OnePolygonOneRing", source=src + ":75-114", points=sq8,
+                      part_offsets=p, ring_offsets=r, vertices=v,
+                      expected=[0, 0, 0, 0, 1, 1, 1, 1]))
+    a = [[-1.0, -1.0], [-1.0, 1.0], [1.0, 1.0], [1.0, -1.0], [-1.0, -1.0]]
+    b = [[0.0, 1.0], [1.0, 0.0], [0.0, -1.0], [-1.0, 0.0], [0.0, 1.0]]
+    p, r, v = repeat([[a], [b], [a], [b]], None)
+    cases.append(dict(name="TwoPolygonsOneRingEach", source=src + ":116-164",
+                      points=sq8[:4], part_offsets=p, ring_offsets=r, vertices=v,
+                      expected=[0, 0, 0, 0]))
+    hole = [[-0.5, -0.5], [-0.5, 0.5], [0.5, 0.5], [0.5, -0.5], [-0.5, -0.5]]
+    p, r, v = repeat([[sq, hole]] * 5, None)
+    cases.append(dict(name="OnePolygonTwoRings", source=src + ":166-209",
+                      points=[[0.0, 0.0], [-0.4, 0.0], [-0.6, 0.0], [0.0, 0.4], [0.0, -0.6]],
+                      part_offsets=p, ring_offsets=r, vertices=v, expected=[0, 0, 1, 0, 1]))
+    cases.append(dict(name="EdgesOfSquare", source=src + ":211-226", points=[[0.0, 0.0]] * 4,
+                      part_offsets=[0, 1, 2, 3, 4], ring_offsets=[0, 5, 10, 15, 20],
+                      vertices=[[-1.0, -1.0], [0.0, -1.0], [0.0, 1.0], [-1.0, 1.0],
+                                [-1.0, -1.0], [0.0, -1.0], [1.0, -1.0], [1.0, 1.0], [0.0, 1.0],
+                                [0.0, -1.0], [-1.0, -1.0], [-1.0, 0.0], [1.0, 0.0], [1.0, -1.0],
+                                [-1.0, 1.0], [-1.0, 0.0], [-1.0, 1.0], [1.0, 1.0], [1.0, 0.0],
+                                [-1.0, 0.0]],
+                      expected=[0, 0, 0, 0]))
+    cases.append(dict(name="CornersOfSquare", source=src + ":228-244", points=[[0.0, 0.0]] * 4,
+                      part_offsets=[0, 1, 2, 3, 4], ring_offsets=[0, 5, 10, 15, 20],
+                      vertices=[[-1.0, -1.0], [-1.0, 0.0], [0.0, 0.0], [0.0, -1.0],
+                                [-1.0, -1.0], [-1.0, 0.0], [-1.0, 1.0], [0.0, 1.0], [-1.0, 0.0],
+                                [-1.0, 0.0], [0.0, -1.0], [0.0, 0.0], [1.0, 0.0], [1.0, -1.0],
+                                [0.0, -1.0], [0.0, 0.0], [0.0, 1.0], [1.0, 1.0], [1.0, 0.0],
+                                [0.0, 0.0]],
+                      expected=[0, 0, 0, 0]))
+    # 32PolygonSupport (:246-323): 32 index-generated squares, points alternating
+    # (0,0) (inside) and (2,0) (outside)
+    xs = [(-1.0, -1.0, 1.0, 1.0, -1.0)[i % 5] for i in range(32 * 5)]
+    ys = [(-1.0, 1.0, 1.0, -1.0, -1.0)[i % 5] for i in range(32 * 5)]
+    cases.append(dict(name="32PolygonSupport", source=src + ":246-323",
+                      points=[[0.0, 0.0], [2.0, 0.0]] * 16, part_offsets=list(range(33)),
+                      ring_offsets=[5 * i for i in range(33)],
+                      vertices=[[a, b] for a, b in zip(xs, ys)], expected=[1, 0] * 16))
+    return cases
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     with open(os.path.join(HERE, "quadtree_cases.json"), "w") as f:
@@ -181,6 +245,8 @@ def main():
         json.dump(join_pip_small(ref), f, indent=1)
     with open(os.path.join(HERE, "pip_kats.json"), "w") as f:
         json.dump(pip_kats(), f, indent=1)
+    with open(os.path.join(HERE, "pairwise_pip_kats.json"), "w") as f:
+        json.dump(pairwise_pip_kats(), f, indent=1)
     print("fixtures written to", HERE)
 
 

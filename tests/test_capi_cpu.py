@@ -33,7 +33,7 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 14
+    assert len(names) == 17
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)
@@ -186,3 +186,66 @@ def test_python_scale_clamp_warning():
             pass
     assert any(str(m.message) == "scale {} is less than required minimum scale {}. Clamping to "
                "minimum scale".format(scale, min_scale) for m in w)
+
+
+def _brute(capi, pairwise, nx=0, ny=0, npo=1, nvx=0, nvy=0):
+    L = capi.lib()
+    out = ctypes.c_void_p()
+    n = ctypes.c_int64(-1)
+    fn = L.qpip_pairwise_point_in_polygon if pairwise else L.qpip_point_in_polygon
+    rc = fn(None, None, nx, ny, None, npo, None, 1, None, None, nvx, nvy, None, None,
+            ctypes.byref(out), ctypes.byref(n))
+    return rc, n.value, out.value
+
+
+@pytest.mark.parametrize("pairwise", [False, True])
+def test_brute_force_pip_preconditions(capi, pairwise):
+    # point_in_polygon.cu:118-136 and detail/point_in_polygon.cuh:94-95, in order
+    rc, _, _ = _brute(capi, pairwise, nx=3, ny=4)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert capi.lib().qpip_last_error_message().decode().endswith(
+        "All points must have both x and y values")
+    rc, _, _ = _brute(capi, pairwise, nvx=5, nvy=4)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    if pairwise:
+        rc, _, _ = _brute(capi, True, nx=2, ny=2, npo=2)
+        assert rc == capi.QPIP_ERROR_LOGIC
+        assert capi.lib().qpip_last_error_message().decode().endswith(
+            "Must pass in the same number of points as polygons.")
+    else:
+        rc, _, _ = _brute(capi, False, npo=33)
+        assert rc == capi.QPIP_ERROR_LOGIC
+        assert capi.lib().qpip_last_error_message().decode().endswith(
+            "Number of polygons cannot exceed 31")
+        assert _brute(capi, False, npo=32)[0] == capi.QPIP_SUCCESS  # 31 polygons, no points
+
+
+def test_brute_force_pip_empty(capi):
+    assert _brute(capi, False) == (capi.QPIP_SUCCESS, 0, None)
+    assert _brute(capi, True, npo=1) == (capi.QPIP_SUCCESS, 0, None)
+
+
+def test_batched_preconditions(capi):
+    import numpy as np
+
+    L = capi.lib()
+    outs = _outs(3)
+    hits = np.zeros(2, np.int64)
+    h = ctypes.c_int64(-1)
+    jpo = np.array([0, 5], np.int64)  # 5 points claimed, 3 given
+    jpy = np.array([0, 0], np.int64)
+    bb = np.array([0.0, 1.0, 0.0, 1.0])
+    sc = np.array([1.0])
+    md = np.array([3], np.int8)
+    ms = np.array([4], np.int32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    rc = L.qpip_batched_quadtree_point_in_polygon(
+        1, p(jpo), p(jpy), p(bb), p(sc), p(md), p(ms), None, 3, None, 1, None, 1, None, None,
+        0, 0, None, None, *[ctypes.byref(o) for o in outs], p(hits), ctypes.byref(h))
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert "job point offsets must start at 0 and end at the number of points" in \
+        L.qpip_last_error_message().decode()
+    rc = L.qpip_batched_quadtree_point_in_polygon(
+        0, None, None, None, None, None, None, None, 0, None, 0, None, 0, None, None, 0, 0,
+        None, None, *[ctypes.byref(o) for o in outs], p(hits), ctypes.byref(h))
+    assert rc == capi.QPIP_SUCCESS and h.value == 0 and hits[0] == 0

@@ -469,3 +469,73 @@ def test_float32_rejected(cs):
 
     with pytest.raises(TypeError, match="float64"):
         cs.Points(torch.zeros(3, dtype=torch.float32), torch.zeros(3, dtype=torch.float32))
+
+
+# ---------------------------------------------------------------------------
+# brute-force point-in-polygon (every point x every polygon; pairwise)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", load_golden("pip_kats.json"), ids=lambda c: c["name"])
+def test_point_in_polygon_bitmask_kats(cs, case):
+    pts = np.array(case["points"], dtype=np.float64).reshape(-1, 2)
+    v = np.array(case["vertices"], dtype=np.float64).reshape(-1, 2)
+    P = cs.Polygons(case["part_offsets"], case["ring_offsets"], v[:, 0], v[:, 1], device="cuda")
+    bm = cs.point_in_polygon_bitmask(cs.Points(pts[:, 0], pts[:, 1], device="cuda"), P)
+    assert np_(bm).astype(np.int64).tolist() == case["expected"]
+
+
+@pytest.mark.parametrize("case", load_golden("pairwise_pip_kats.json"), ids=lambda c: c["name"])
+def test_pairwise_point_in_polygon_kats(cs, case):
+    pts = np.array(case["points"], dtype=np.float64).reshape(-1, 2)
+    v = np.array(case["vertices"], dtype=np.float64).reshape(-1, 2)
+    P = cs.Polygons(case["part_offsets"], case["ring_offsets"], v[:, 0], v[:, 1], device="cuda")
+    got = cs.pairwise_point_in_polygon(cs.Points(pts[:, 0], pts[:, 1], device="cuda"), P)
+    assert np_(got).astype(np.int64).tolist() == case["expected"]
+
+
+@pytest.mark.parametrize("seed,n,npoly", [(51, 1, 1), (52, 1000, 7), (53, 100003, 31),
+                                          (54, 65, 31), (55, 300000, 12)])
+@pytest.mark.parametrize("layout", ["columns", "xy"])
+def test_point_in_polygon_bitmask_random(cs, oracle_mod, seed, n, npoly, layout):
+    rng = np.random.default_rng(seed)
+    part, ring, vx, vy = star_polygons(rng, npoly, 100.0, 5.0, 40.0, holes=0.3)
+    x = rng.uniform(-5, 105, n)
+    y = rng.uniform(-5, 105, n)
+    # points on vertices and on edges' x ranges exercise the on-edge rule
+    k = min(n, 50)
+    x[:k] = vx[:k]
+    y[:k] = vy[:k]
+    pts = make_points(cs, x, y, layout)
+    bm = cs.point_in_polygon_bitmask(pts, cs.Polygons(part, ring, vx, vy, device="cuda"))
+    ref = oracle_mod.point_in_polygon_bitmask(x, y, part, ring, vx, vy)
+    assert np.array_equal(np_(bm).astype(np.int64), ref.astype(np.int64))
+    tbl = cs.point_in_polygon(pts, cs.Polygons(part, ring, vx, vy, device="cuda"))
+    assert tbl.columns == list(range(npoly))
+    for j in range(npoly):
+        assert np.array_equal(np_(tbl[j]), ((ref >> j) & 1).astype(bool))
+
+
+@pytest.mark.parametrize("seed,n", [(61, 1), (62, 5000), (63, 200001)])
+def test_pairwise_point_in_polygon_random(cs, oracle_mod, seed, n):
+    rng = np.random.default_rng(seed)
+    part, ring, vx, vy = star_polygons(rng, n, 1000.0, 5.0, 40.0, holes=0.3)
+    cx = np.array([vx[ring[part[i]]:ring[part[i] + 1]].mean() for i in range(min(n, 2000))])
+    x = rng.uniform(0, 1000, n)
+    y = rng.uniform(0, 1000, n)
+    m = len(cx)
+    x[:m] = cx + rng.normal(0, 5, m)  # many points near their own polygon
+    y[:m] = np.array([vy[ring[part[i]]:ring[part[i] + 1]].mean() for i in range(m)])
+    got = cs.pairwise_point_in_polygon(cs.Points(x, y, device="cuda"),
+                                       cs.Polygons(part, ring, vx, vy, device="cuda"))
+    ref = oracle_mod.pairwise_point_in_polygon(x, y, part, ring, vx, vy)
+    assert np.array_equal(np_(got).astype(np.uint8), ref)
+    assert int(ref.sum()) > 0 or n < 10
+
+
+def test_point_in_polygon_errors(cs):
+    rng = np.random.default_rng(71)
+    part, ring, vx, vy = star_polygons(rng, 32, 100.0, 5.0, 20.0)
+    pts = cs.Points(np.zeros(3), np.zeros(3), device="cuda")
+    with pytest.raises(RuntimeError, match="Number of polygons cannot exceed 31"):
+        cs.point_in_polygon_bitmask(pts, cs.Polygons(part, ring, vx, vy, device="cuda"))
+    with pytest.raises(RuntimeError, match="Must pass in the same number of points as polygons."):
+        cs.pairwise_point_in_polygon(pts, cs.Polygons(part, ring, vx, vy, device="cuda"))

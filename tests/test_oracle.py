@@ -53,6 +53,30 @@ def test_join_and_pip_golden(oracle_mod):
     assert pt.tolist() == g["expected_point_index"]
 
 
+@pytest.mark.parametrize("case", load_golden("pairwise_pip_kats.json"), ids=lambda c: c["name"])
+def test_pairwise_pip_kats(oracle_mod, case):
+    """The reference's pairwise vectors (pairwise_point_in_polygon_test.cu)."""
+    pts = np.array(case["points"], dtype=np.float64).reshape(-1, 2)
+    v = np.array(case["vertices"], dtype=np.float64).reshape(-1, 2)
+    got = oracle_mod.pairwise_point_in_polygon(pts[:, 0], pts[:, 1], case["part_offsets"],
+                                               case["ring_offsets"], v[:, 0], v[:, 1])
+    assert got.tolist() == case["expected"]
+
+
+def test_pairwise_matches_bitmask_diagonal(oracle_mod):
+    """Pairwise result i equals bit i of the brute-force bitmask of point i."""
+    rng = np.random.default_rng(5)
+    from conftest import star_polygons
+
+    part, ring, vx, vy = star_polygons(rng, 20, 100.0, 5.0, 30.0, holes=0.3)
+    n = len(part) - 1
+    x = rng.uniform(0, 100, n)
+    y = rng.uniform(0, 100, n)
+    bm = oracle_mod.point_in_polygon_bitmask(x, y, part, ring, vx, vy)
+    pw = oracle_mod.pairwise_point_in_polygon(x, y, part, ring, vx, vy)
+    assert pw.tolist() == [(int(b) >> i) & 1 for i, b in enumerate(bm)]
+
+
 @pytest.mark.parametrize("case", load_golden("pip_kats.json"), ids=lambda c: c["name"])
 def test_pip_kats(oracle_mod, case):
     pts = np.array(case["points"]).reshape(-1, 2)
