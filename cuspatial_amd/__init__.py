@@ -20,6 +20,8 @@ from .spatial import (  # noqa: F401
     quadtree_on_points,
     quadtree_point_in_polygon,
 )
+from .binpreds import contains_properly, polygon_contains_properly  # noqa: F401
+from .ranker import CandidateGrid, LearnedLocator, Ranker  # noqa: F401
 
 __all__ = [
     "Points",
@@ -34,4 +36,9 @@ __all__ = [
     "pairwise_point_in_polygon",
     "batched_quadtree_point_in_polygon",
     "BatchResult",
+    "contains_properly",
+    "polygon_contains_properly",
+    "CandidateGrid",
+    "Ranker",
+    "LearnedLocator",
 ]

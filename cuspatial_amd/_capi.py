@@ -62,6 +62,12 @@ EXPORTED = (
     "qpip_point_in_polygon",
     "qpip_pairwise_point_in_polygon",
     "qpip_batched_quadtree_point_in_polygon",
+    "qpip_ranker_num_params",
+    "qpip_ranker_features",
+    "qpip_ranker_logits",
+    "qpip_ranker_gradients",
+    "qpip_ranker_adam",
+    "qpip_ranker_locate",
 )
 
 
@@ -113,7 +119,18 @@ def lib():
         L.qpip_batched_quadtree_point_in_polygon.argtypes = [
             _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
             _i64, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), _vp, P(_i64)]
+        L.qpip_ranker_num_params.argtypes = [_i64]
+        L.qpip_ranker_num_params.restype = _i64
+        L.qpip_ranker_features.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp]
+        L.qpip_ranker_logits.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp]
+        L.qpip_ranker_gradients.argtypes = [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]
+        _f = ctypes.c_float
+        L.qpip_ranker_adam.argtypes = [_vp, _vp, _vp, _vp, _vp, _i64, _f, _f, _f, _f, _i64, _vp]
+        L.qpip_ranker_locate.argtypes = [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64,
+                                         ctypes.c_int32, _vp, _vp, _vp, _vp]
         for name in (
+            "qpip_ranker_features", "qpip_ranker_logits", "qpip_ranker_gradients",
+            "qpip_ranker_adam", "qpip_ranker_locate",
             "qpip_batched_quadtree_point_in_polygon",
             "qpip_point_in_polygon", "qpip_pairwise_point_in_polygon",
             "qpip_quadtree_on_points", "qpip_quadtree_on_points_xy", "qpip_polygon_bounding_boxes",

@@ -367,3 +367,55 @@ def batched_jobs(seed: int = 1003, num_jobs: int = 4096, min_points: int = 1_000
 
 WORKLOADS = {"uniform_stars": uniform_stars, "gaussian_holes": gaussian_holes,
              "batched_jobs": batched_jobs}
+
+
+# ---------------------------------------------------------------------------
+# Voronoi partition (non-overlapping polygons covering the domain)
+# ---------------------------------------------------------------------------
+def voronoi_cells(seed: int, num_seeds: int, ext: float):
+    """Clipped Voronoi tessellation of `num_seeds` uniform seeds in [0, ext)^2 as
+    (part_offset, ring_offset, px, py): one closed counter-clockwise ring per
+    cell. The seeds are mirrored across the four sides of the square so that
+    every original cell is bounded and ends on the square's sides. Neighbouring
+    cells share their common vertices bit for bit (one Qhull vertex each), so the
+    cells do not overlap."""
+    from scipy.spatial import Voronoi
+
+    rng = np.random.Generator(np.random.PCG64([seed, 2]))
+    s = rng.uniform(0.0, ext, (num_seeds, 2))
+    mirrored = [s,
+                np.stack([-s[:, 0], s[:, 1]], 1), np.stack([2 * ext - s[:, 0], s[:, 1]], 1),
+                np.stack([s[:, 0], -s[:, 1]], 1), np.stack([s[:, 0], 2 * ext - s[:, 1]], 1)]
+    vor = Voronoi(np.concatenate(mirrored))
+    verts = vor.vertices
+    xs, ys, sizes = [], [], []
+    for i in range(num_seeds):
+        reg = [v for v in vor.regions[vor.point_region[i]]]
+        if -1 in reg or len(reg) < 3:
+            raise RuntimeError("unbounded Voronoi cell after mirroring")
+        reg = np.asarray(reg)
+        ang = np.arctan2(verts[reg, 1] - s[i, 1], verts[reg, 0] - s[i, 0])
+        reg = reg[np.argsort(ang, kind="stable")]
+        reg = np.concatenate([reg, reg[:1]])
+        xs.append(verts[reg, 0])
+        ys.append(verts[reg, 1])
+        sizes.append(reg.size)
+    ring_offset = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+    part_offset = np.arange(num_seeds + 1, dtype=np.int32)
+    return part_offset, ring_offset, np.concatenate(xs), np.concatenate(ys)
+
+
+def voronoi_partition(seed: int = 1004, num_points: int = 200_000_000, num_seeds: int = 20_000,
+                      device=None) -> Workload:
+    """Points from the 64-Gaussian mixture of gaussian_holes over [0, 1e5)^2 and
+    polygons from the clipped Voronoi tessellation of `num_seeds` uniform seeds
+    (convex, non-overlapping, covering the domain). The quadtree parameters are
+    those of gaussian_holes (max_depth 15, max_size 256, scale 1e5/2^15)."""
+    ext = 1e5
+    if device is not None:
+        x, y = _gaussian_mixture_device(seed, num_points, ext, device)
+    else:
+        x, y = _gaussian_mixture(np.random.Generator(np.random.PCG64([seed, 0])), num_points, ext)
+    part, ring, px, py = voronoi_cells(seed, num_seeds, ext)
+    return Workload("voronoi_partition", x, y, part, ring, px, py, 0.0, ext, 0.0, ext,
+                    ext / 2**15, 15, 256, seed)

@@ -226,13 +226,29 @@ def polygon_bounding_boxes(polygons: Polygons):
     return Table((k, res.take(o.value, nb.value, torch.float64)) for k, o in zip(names, outs))
 
 
+def _u32(t: torch.Tensor, name) -> torch.Tensor:
+    """A UINT32 column as 4-byte words (other integer widths keep their low 32
+    bits, which is the value for every index that fits the column type)."""
+    if t.dtype in (torch.uint32, torch.int32):
+        return t.contiguous()
+    if t.dtype.is_floating_point or t.dtype == torch.bool:
+        raise TypeError(f"{name}: expected an integer column (got {t.dtype})")
+    return t.to(torch.int32).contiguous()
+
+
+def _u8(t: torch.Tensor, name) -> torch.Tensor:
+    if t.dtype.is_floating_point:
+        raise TypeError(f"{name}: expected an integer column (got {t.dtype})")
+    return t.to(torch.uint8).contiguous()
+
+
 def _tree_columns(quadtree):
     cols = [quadtree[k] for k in ("key", "level", "is_internal_node", "length", "offset")]
     if len(cols) != 5:
         raise RuntimeError("cuSpatial failure: quadtree table must have 5 columns")
     key, level, internal, length, offset = cols
-    return (key.contiguous(), level.contiguous(), internal.to(torch.uint8).contiguous(),
-            length.contiguous(), offset.contiguous())
+    return (_u32(key, "key"), _u8(level, "level"), _u8(internal, "is_internal_node"),
+            _u32(length, "length"), _u32(offset, "offset"))
 
 
 def join_quadtree_and_bounding_boxes(quadtree, bounding_boxes, x_min, x_max, y_min, y_max,
@@ -284,9 +300,9 @@ def quadtree_point_in_polygon(poly_quad_pairs, quadtree, point_indices, points: 
         raise RuntimeError("cuSpatial failure: a quadrant-polygon table must have 2 columns")
     key, level, internal, length, offset = _tree_columns(quadtree)
     dev = points.device
-    pp = pcols[0].contiguous()
-    pq = pcols[1].contiguous()
-    pi = point_indices.contiguous()
+    pp = _u32(pcols[0], "poly_quad_pairs")
+    pq = _u32(pcols[1], "poly_quad_pairs")
+    pi = _u32(point_indices, "point_indices")
     res = _capi.TorchResource(dev)
     po, pt = _outs(2)
     h = ctypes.c_int64()

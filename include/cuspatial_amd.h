@@ -383,6 +383,117 @@ QPIP_API int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
                                                     int64_t* job_hit_offsets,
                                                     int64_t* num_hits);
 
+/*
+ * Learned candidate ranking over a coarse grid (no counterpart in the
+ * reference; an addition for point location over non-overlapping polygons).
+ *
+ * qpip_grid: a dim x dim grid of square cells of side `cell` from (x0, y0).
+ * A point's cell is (floor((x - x0) / cell), floor((y - y0) / cell)), each
+ * clamped to [0, dim - 1] (NaN -> 0), row-major (iy * dim + ix). candidates
+ * holds 16 polygon ids per cell (-1 padded) and counts the number of valid
+ * slots; a count above 16 marks a cell whose points always take the exact
+ * quadtree path. The candidate list of a cell must contain every polygon whose
+ * bounding box reaches the cell (the locator's results are exact only then).
+ *
+ * qpip_ranker: the model, as one flat fp32 parameter array (params) and its bf16
+ * copy (params_bf16, same element offsets): E[num_cells][64], W1[128][96],
+ * b1[128], W2[128][128], b2[128], W3[16][128], b3[16] (qpip_ranker_num_params
+ * floats). Features of a point: E[cell] and sin, cos of pi k u and pi k v for
+ * k = 1..8, where (u, v) is the point's position inside its cell in cell units.
+ * logits = W3 relu(W2 relu(W1 x + b1) + b2) + b3 with bf16 operands and fp32
+ * accumulation.
+ */
+typedef struct qpip_grid {
+  double x0;
+  double y0;
+  double cell;
+  int32_t dim;
+  const int32_t* candidates; /* [dim * dim][16] */
+  const uint8_t* counts;     /* [dim * dim] */
+} qpip_grid;
+
+typedef struct qpip_ranker {
+  const float* params;
+  const uint16_t* params_bf16;
+  int64_t num_cells; /* must equal dim * dim of the grid it is used with */
+} qpip_ranker;
+
+/* Number of floats in the flat parameter array for num_cells embeddings. */
+QPIP_API int64_t qpip_ranker_num_params(int64_t num_cells);
+
+/* Feature rows (bf16 bits, [num_points][96]) and grid cells of points_xy. */
+QPIP_API int qpip_ranker_features(const qpip_ranker* model,
+                                  const qpip_grid* grid,
+                                  const double* points_xy,
+                                  int64_t num_points,
+                                  uint16_t* features,
+                                  uint32_t* cells,
+                                  void* stream);
+
+/* logits[num_points][16] of the model for points_xy (device arrays). */
+QPIP_API int qpip_ranker_logits(const qpip_ranker* model,
+                                const qpip_grid* grid,
+                                const double* points_xy,
+                                int64_t num_points,
+                                float* logits,
+                                void* stream);
+
+/* Gradient of the mean masked softmax cross-entropy over a batch:
+ * labels[i] is the slot (0..15) of point i's polygon in its cell's candidate
+ * list, or -1 to leave the point out; the loss is summed over labelled points
+ * and divided by num_points. grad (qpip_ranker_num_params floats) is
+ * overwritten. stats (3 floats, device) accumulates: loss sum, top-1 correct,
+ * labelled points. */
+QPIP_API int qpip_ranker_gradients(const qpip_ranker* model,
+                                   const qpip_grid* grid,
+                                   const double* points_xy,
+                                   const int32_t* labels,
+                                   int64_t num_points,
+                                   float* grad,
+                                   float* stats,
+                                   void* stream);
+
+/* One Adam step (step counts from 1) on params; refreshes params_bf16. */
+QPIP_API int qpip_ranker_adam(float* params,
+                              uint16_t* params_bf16,
+                              const float* grad,
+                              float* m,
+                              float* v,
+                              int64_t num_params,
+                              float lr,
+                              float beta1,
+                              float beta2,
+                              float eps,
+                              int64_t step,
+                              void* stream);
+
+/* Locate points: for each point, test the candidates of its cell in the order
+ * of the model's logits (highest first) with the quadtree path's exact FP64
+ * point-in-polygon test, at most max_tests of them, stopping at the first
+ * polygon that contains the point.
+ * out_polygon[i]: the polygon (>= 0), -1 when no candidate contains the point
+ * (all were tested), or -2 when the point must take the exact quadtree path:
+ * its cell overflowed, its hit lies within the collinearity filter of an edge
+ * (another polygon's test could also hold), or max_tests ran out.
+ * out_tests[i]: tests run. out_flags[i]: bit 0 quadtree path, bit 1 the first
+ * prediction was not the answer. All arrays are device arrays. */
+QPIP_API int qpip_ranker_locate(const qpip_ranker* model,
+                                const qpip_grid* grid,
+                                const int32_t* poly_offsets,
+                                int64_t num_poly_offsets,
+                                const int32_t* ring_offsets,
+                                int64_t num_ring_offsets,
+                                const double* poly_x,
+                                const double* poly_y,
+                                int64_t num_vertices,
+                                const double* points_xy,
+                                int64_t num_points,
+                                int32_t max_tests,
+                                int32_t* out_polygon,
+                                uint8_t* out_tests,
+                                uint8_t* out_flags,
+                                void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -237,6 +237,57 @@ def pairwise_pip_kats():
     return cases
 
 
+def contains_properly_kats(ref):
+    """python/cuspatial/cuspatial/tests/binpreds/test_contains_properly.py: one
+    point against one polygon (a closed single ring) and the value
+    GeoSeries.contains_properly returns for it. The points and polygons are read
+    from the cited parametrize lists; `expected` is what each test asserts of
+    the result (test_float_precision_limits_failures asserts that none holds).
+    Multi-row cases list their rows and the expected per-point values."""
+    src = "python/cuspatial/cuspatial/tests/binpreds/test_contains_properly.py"
+    pt_re = r"Point\(\[\s*(%s),\s*(%s)\]\)" % (NUM, NUM)
+    poly_re = r"Polygon\(\[((?:\[\s*%s,\s*%s\],?\s*)+)\]\)" % (NUM, NUM)
+
+    def ring(text):
+        return [[float(a), float(b)] for a, b in re.findall(r"\[\s*(%s),\s*(%s)\]" % (NUM, NUM),
+                                                             text)]
+
+    cases = []
+
+    def single(a, b, forced=None):
+        text = lines(ref, src, a, b)
+        named = {"clockwiseTriangle": [[0, 0], [0, 1], [1, 1], [0, 0]],
+                 "clockwiseSquare": [[-0.5, -0.5], [-0.5, 0.5], [0.5, 0.5], [0.5, -0.5],
+                                     [-0.5, -0.5]]}
+        for m in re.finditer(r"\[\s*%s,\s*(%s|clockwiseTriangle|clockwiseSquare),\s*(True|False),?\s*\]"
+                             % (pt_re, poly_re), text):
+            x, y, poly_text, verts, expect = m.group(1), m.group(2), m.group(3), m.group(4), m.group(5)
+            poly = named[poly_text] if poly_text in named else ring(verts)
+            e = (expect == "True") if forced is None else forced
+            cases.append(dict(source="%s:%d-%d" % (src, a, b), points=[[float(x), float(y)]],
+                              polygons=[[[float(u), float(v)] for u, v in poly]],
+                              expected=[e]))
+
+    single(85, 100, forced=False)  # test_float_precision_limits_failures: `not np.any(got)`
+    single(113, 127)  # test_float_precision_limits
+    single(152, 184)  # test_point_in_polygon
+    tri_ccw = [[0.0, 0.0], [1.0, 0.0], [1.0, 1.0], [0.0, 0.0]]
+    tri_cw = [[0.0, 0.0], [0.0, 1.0], [1.0, 1.0], [0.0, 0.0]]
+    cases.append(dict(source=src + ":196-203", points=[[0.0, 0.0], [0.0, 0.0]],
+                      polygons=[tri_ccw], expected=[False, False]))
+    cases.append(dict(source=src + ":206-218", points=[[0.0, 0.0]],
+                      polygons=[tri_ccw, [[-2.0, -2.0], [-2.0, 2.0], [2.0, 2.0], [-2.0, -2.0]]],
+                      expected=[False]))
+    # test_ten_fixed_points (:221-254): rows pair up; (0,0) is a vertex of every
+    # triangle, (0.5, 0.25) lies inside the counter-clockwise one only
+    cases.append(dict(source=src + ":221-254",
+                      points=[[0.0, 0.0]] * 2 + [[0.5, 0.25]] * 6 + [[0.0, 0.0]] * 2,
+                      polygons=[tri_ccw] * 5 + [tri_cw] * 5,
+                      expected=[False, False, True, True, True, False, False, False, False,
+                                False]))
+    return cases
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     with open(os.path.join(HERE, "quadtree_cases.json"), "w") as f:
@@ -247,6 +298,8 @@ def main():
         json.dump(pip_kats(), f, indent=1)
     with open(os.path.join(HERE, "pairwise_pip_kats.json"), "w") as f:
         json.dump(pairwise_pip_kats(), f, indent=1)
+    with open(os.path.join(HERE, "contains_properly_kats.json"), "w") as f:
+        json.dump(contains_properly_kats(ref), f, indent=1)
     print("fixtures written to", HERE)
 
 

@@ -33,7 +33,7 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 17
+    assert len(names) == 23
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)
@@ -249,3 +249,37 @@ def test_batched_preconditions(capi):
         0, None, None, None, None, None, None, None, 0, None, 0, None, 0, None, None, 0, 0,
         None, None, *[ctypes.byref(o) for o in outs], p(hits), ctypes.byref(h))
     assert rc == capi.QPIP_SUCCESS and h.value == 0 and hits[0] == 0
+
+
+def test_ranker_preconditions(capi):
+    """The ranker entry points check their arguments before touching the device."""
+    from cuspatial_amd.ranker import qpip_grid, qpip_ranker
+
+    L = capi.lib()
+    assert L.qpip_ranker_num_params(0) == 96 * 128 + 128 + 128 * 128 + 128 + 16 * 128 + 16
+    assert L.qpip_ranker_num_params(10) == L.qpip_ranker_num_params(0) + 640
+    assert L.qpip_ranker_num_params(-1) == -1
+    g = qpip_grid(0.0, 0.0, 1.0, 4, None, None)
+    bad_dim = qpip_grid(0.0, 0.0, 1.0, 0, None, None)
+    bad_cell = qpip_grid(0.0, 0.0, -1.0, 4, None, None)
+    m = qpip_ranker(8, 8, 16)  # placeholder addresses: never dereferenced by the checks
+    m_wrong = qpip_ranker(8, 8, 15)
+    a = ctypes.addressof
+    for grid, model, msg in ((bad_dim, m, "grid dimension must be in [1, 65536]"),
+                             (bad_cell, m, "grid cell size must be positive"),
+                             (g, m_wrong, "embedding count must equal the grid's cell count")):
+        rc = L.qpip_ranker_logits(a(model), a(grid), None, 10, None, None)
+        assert rc == capi.QPIP_ERROR_LOGIC and msg in L.qpip_last_error_message().decode()
+    rc = L.qpip_ranker_locate(a(m), a(g), None, 1, None, 1, None, None, 0, None, 10, 0, None,
+                              None, None, None)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert "max_tests must be at least 1" in L.qpip_last_error_message().decode()
+    rc = L.qpip_ranker_gradients(a(m), a(g), None, None, 0, None, None, None)
+    assert rc == capi.QPIP_ERROR_LOGIC
+    assert "at least one point" in L.qpip_last_error_message().decode()
+    rc = L.qpip_ranker_adam(None, None, None, None, None, 4, 1e-3, 0.9, 0.999, 1e-8, 0, None)
+    assert rc == capi.QPIP_ERROR_LOGIC and "Adam step" in L.qpip_last_error_message().decode()
+    # empty inputs return without work
+    assert L.qpip_ranker_logits(a(m), a(g), None, 0, None, None) == capi.QPIP_SUCCESS
+    assert L.qpip_ranker_locate(a(m), a(g), None, 1, None, 1, None, None, 0, None, 0, 16, None,
+                                None, None, None) == capi.QPIP_SUCCESS

@@ -1,0 +1,182 @@
+"""The learned ranker on the GPU against its oracle (oracle/ranker.py) and the
+learned locator against the plain quadtree join.
+
+Bars (stated per test): features within one bf16 step of the oracle's; logits
+and gradients within a relative norm error of 2e-2 of the oracle's bf16
+emulation (fp32 MFMA accumulation against float64, activations rounded to bf16
+at the same points); Adam within 1e-6 of its rule; the locator's (polygon,
+point) pairs bit-identical to the plain join's.
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a ROCm GPU"
+    import cuspatial_amd as cs
+    from cuspatial_amd import datasets, ranker
+    from oracle import ranker as rk
+
+    return torch, cs, ranker, rk, datasets
+
+
+def voronoi(env, n_points, n_seeds, seed, dim):
+    torch, cs, ranker, rk, datasets = env
+    part, ring, px, py = datasets.voronoi_cells(seed, n_seeds, 1e5)
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1e5, n_points)
+    y = rng.uniform(0, 1e5, n_points)
+    xy = torch.as_tensor(np.stack([x, y], 1).reshape(-1), device="cuda")
+    P = cs.Polygons(part, ring, px, py, device="cuda")
+    grid = ranker.CandidateGrid.build(P, 0.0, 0.0, 1e5, dim)
+    return (part, ring, px, py), P, grid, xy, x, y
+
+
+def relerr(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+def test_features_match_oracle(env):
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 5000, 200, 3, 16)
+    model = ranker.Ranker(grid.num_cells, seed=1)
+    X, cells = model.features(grid, xy)
+    p = model.params.cpu().numpy().astype(np.float64)
+    Xo, co = rk.features(p, grid.num_cells, x, y, 0.0, 0.0, grid.cell, grid.dim)
+    assert np.array_equal(cells.cpu().numpy().astype(np.int64), co)
+    Xg = rk.from_bf16_bits(X.cpu().numpy().view(np.uint16))
+    assert np.array_equal(Xg[:, :64], Xo[:, :64].astype(np.float32))  # embedding rows, exact
+    # Fourier features: GPU sinf/cosf against numpy, then bf16: one bf16 step at most
+    assert np.max(np.abs(Xg[:, 64:] - Xo[:, 64:])) <= 2 ** -8 * 1.01
+
+
+def test_logits_match_oracle(env):
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 3001, 200, 4, 16)  # not a multiple of 16: padding
+    model = ranker.Ranker(grid.num_cells, seed=2)
+    z = model.logits(grid, xy).cpu().numpy()
+    X, _ = model.features(grid, xy)
+    Xg = rk.from_bf16_bits(X.cpu().numpy().view(np.uint16)).astype(np.float64)
+    zo = rk.forward(model.params.cpu().numpy().astype(np.float64), grid.num_cells, Xg)["Z"]
+    assert relerr(z, zo) < 2e-2, relerr(z, zo)
+    # the MFMA lane layout: a wrong operand order or transpose would be O(1) off
+    assert np.corrcoef(z.reshape(-1), zo.reshape(-1))[0, 1] > 0.999
+
+
+def test_gradients_match_oracle(env):
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 2000, 150, 5, 16)
+    model = ranker.Ranker(grid.num_cells, seed=3)
+    rng = np.random.default_rng(6)
+    labels = np.where(rng.uniform(size=2000) < 0.9, rng.integers(0, 6, 2000), -1).astype(np.int32)
+    grad, stats = model.gradients(grid, xy, torch.as_tensor(labels, device="cuda"))
+    g = grad.cpu().numpy()
+    st = stats.cpu().numpy()
+    X, cells = model.features(grid, xy)
+    Xg = rk.from_bf16_bits(X.cpu().numpy().view(np.uint16)).astype(np.float64)
+    loss, go, so = rk.loss_and_grad(model.params.cpu().numpy().astype(np.float64),
+                                    grid.num_cells, Xg, cells.cpu().numpy().astype(np.int64),
+                                    grid.counts.cpu().numpy(), labels)
+    assert so["labelled"] > 1000 and st[2] == so["labelled"]
+    assert abs(st[0] - so["loss_sum"]) <= 1e-3 * so["loss_sum"]
+    o = rk.layout(grid.num_cells)
+    for a, b in (("e", "w1"), ("w1", "b1"), ("b1", "w2"), ("w2", "b2"), ("b2", "w3"),
+                 ("w3", "b3"), ("b3", "total")):
+        assert relerr(g[o[a]:o[b]], go[o[a]:o[b]]) < 2e-2, (a, relerr(g[o[a]:o[b]], go[o[a]:o[b]]))
+
+
+def test_adam_step_matches_rule(env):
+    torch, cs, ranker, rk, _ = env
+    model = ranker.Ranker(16, seed=4)
+    p0 = model.params.cpu().numpy().astype(np.float64)
+    g = torch.randn(model.params.numel(), generator=torch.Generator().manual_seed(1)) * 0.01
+    model.grad.copy_(g.to("cuda"))
+    model.adam(1e-2)
+    model.grad.copy_((g * 2).to("cuda"))
+    model.adam(1e-2)
+    q, m, v = rk.adam(p0, g.numpy().astype(np.float64), 0.0, 0.0, 1, 1e-2)
+    q, m, v = rk.adam(q, 2 * g.numpy().astype(np.float64), m, v, 2, 1e-2)
+    got = model.params.cpu().numpy()
+    assert np.max(np.abs(got - q)) < 1e-6
+    assert torch.equal(model.params_bf16.cpu(), model.params.to(torch.bfloat16).cpu())
+
+
+def exact_pairs(env, P, xy, bbox=(0.0, 1e5, 0.0, 1e5), d=10, ms=64):
+    torch, cs, ranker, rk, _ = env
+    pts = cs.Points.from_points_xy(xy)
+    return ranker.plain_pairs(pts, P, bbox, (bbox[1] - bbox[0]) / (1 << d), d, ms)
+
+
+def test_locator_equals_plain_join_untrained_and_trained(env):
+    """Exact whatever the model: an untrained model, a zero model (list order),
+    and a trained one all give the plain join's pairs; training lowers the tests
+    per point below list order's."""
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 200000, 400, 7, 32)
+    want_poly, want_pt = exact_pairs(env, P, xy)
+    per_point = ranker.per_point_polygon(want_poly, want_pt, len(x))
+    tests = {}
+    for name, model in (("random", ranker.Ranker(grid.num_cells, seed=5)),
+                        ("zero", ranker.Ranker(grid.num_cells, zero=True))):
+        loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+        res = loc.locate(cs.Points.from_points_xy(xy))
+        poly, pt = res.pairs()
+        assert torch.equal(poly, want_poly) and torch.equal(pt, want_pt), name
+        tests[name] = res.tests.to(torch.float64).mean().item()
+    labels = grid.slots_of(xy, per_point)
+    model = ranker.Ranker(grid.num_cells, seed=6)
+    hist = model.train(grid, xy, labels, epochs=6, batch=8192, lr=5e-3)
+    assert hist[-1][0] < hist[0][0] and hist[-1][1] > hist[0][1], hist
+    loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+    res = loc.locate(cs.Points.from_points_xy(xy))
+    poly, pt = res.pairs()
+    assert torch.equal(poly, want_poly) and torch.equal(pt, want_pt)
+    trained = res.tests.to(torch.float64).mean().item()
+    assert trained < tests["zero"], (trained, tests)
+
+
+def test_locator_fallback_paths(env):
+    """max_tests = 1 sends every first-prediction miss to the quadtree path; an
+    overflowed cell (more than 16 candidates) sends all its points there; the
+    answer stays exact."""
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 50000, 3000, 8, 16)  # coarse grid: some cells overflow
+    assert int((grid.counts == ranker.OVERFLOW).sum()) > 0
+    want_poly, want_pt = exact_pairs(env, P, xy)
+    model = ranker.Ranker(grid.num_cells, seed=9)
+    loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+    for mt in (1, 16):
+        res = loc.locate(cs.Points.from_points_xy(xy), max_tests=mt)
+        poly, pt = res.pairs()
+        assert torch.equal(poly, want_poly) and torch.equal(pt, want_pt), mt
+        assert res.fallback > 0
+    # points outside the domain and on a shared vertex take the exact answer too
+    part, ring, px, py = (P.part_offset.cpu().numpy(), P.ring_offset.cpu().numpy(),
+                          P.x.cpu().numpy(), P.y.cpu().numpy())
+    extra = np.array([[-10.0, 5.0], [1e5 + 1, 3.0], [px[3], py[3]], [np.nan, 1.0]])
+    xy2 = torch.as_tensor(extra.reshape(-1), device="cuda")
+    res = loc.locate(cs.Points.from_points_xy(xy2))
+    wp, wt = exact_pairs(env, P, xy2)
+    poly, pt = res.pairs()
+    assert torch.equal(poly, wp) and torch.equal(pt, wt)
+
+
+def test_locator_errors(env):
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 100, 50, 10, 8)
+    model = ranker.Ranker(16 * 16)  # embeddings for a 16 x 16 grid, grid is 8 x 8
+    loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+    with pytest.raises(RuntimeError, match="embedding count must equal"):
+        loc.locate(cs.Points.from_points_xy(xy))
+    model = ranker.Ranker(grid.num_cells)
+    loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+    with pytest.raises(RuntimeError, match="max_tests must be at least 1"):
+        loc.locate(cs.Points.from_points_xy(xy), max_tests=0)

@@ -77,6 +77,29 @@ def test_pairwise_matches_bitmask_diagonal(oracle_mod):
     assert pw.tolist() == [(int(b) >> i) & 1 for i, b in enumerate(bm)]
 
 
+def _polys(polygons):
+    part, ring, verts = [0], [0], []
+    for r in polygons:
+        verts += r
+        ring.append(len(verts))
+        part.append(len(ring) - 1)
+    v = np.array(verts, dtype=np.float64).reshape(-1, 2)
+    return part, ring, v
+
+
+@pytest.mark.parametrize("case", [c for c in load_golden("contains_properly_kats.json")
+                                  if len(c["points"]) == 1 and len(c["polygons"]) == 1],
+                         ids=lambda c: "%s_%s" % (c["source"].split(":")[-1], c["points"][0]))
+def test_contains_properly_kats(oracle_mod, case):
+    """The reference's contains_properly expectations for one point against one
+    polygon (tests/binpreds/test_contains_properly.py): the brute-force test the
+    caller uses for up to 31 polygons is the predicate itself."""
+    part, ring, v = _polys(case["polygons"])
+    pts = np.array(case["points"], dtype=np.float64)
+    got = oracle_mod.point_in_polygon_bitmask(pts[:, 0], pts[:, 1], part, ring, v[:, 0], v[:, 1])
+    assert [bool(b & 1) for b in got] == case["expected"]
+
+
 @pytest.mark.parametrize("case", load_golden("pip_kats.json"), ids=lambda c: c["name"])
 def test_pip_kats(oracle_mod, case):
     pts = np.array(case["points"]).reshape(-1, 2)
