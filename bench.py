@@ -82,10 +82,16 @@ def main():
                     help="target CPU work of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="gaussian_holes",
-                    choices=["gaussian_holes", "batched_jobs"],
+                    choices=["gaussian_holes", "batched_jobs", "voronoi_learned"],
                     help="gaussian_holes: the plain join at 1e8 points (the headline line); "
                          "batched_jobs: 4096 independent jobs through the batched entry point, "
-                         "with the plain per-job calls on the same inputs timed beside it")
+                         "with the plain per-job calls on the same inputs timed beside it; "
+                         "voronoi_learned: 2e8 points located in a 20,000-cell Voronoi "
+                         "partition by the learned ranker + exact test, against the plain "
+                         "join on the same inputs")
+    ap.add_argument("--train-points", type=int, default=10_000_000,
+                    help="labelled points the ranker is trained on (voronoi_learned)")
+    ap.add_argument("--epochs", type=int, default=3, help="ranker training epochs")
     ap.add_argument("--jobs", type=int, default=4096, help="jobs of the batched workload")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the known "
@@ -115,6 +121,8 @@ def main():
         return
     if args.workload == "batched_jobs":
         return bench_batched(args, dev)
+    if args.workload == "voronoi_learned":
+        return bench_learned(args, dev)
     log(f"[bench] generating gaussian_holes with {args.points} points on {dev}")
     w = datasets.gaussian_holes(num_points=args.points, device=dev)
     bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
@@ -361,6 +369,148 @@ def bench_batched(args, dev):
         "plain_per_job": {"value": plain_value, "unit": "points/s", "seconds": plain_s,
                           "jobs_with_equal_hit_counts": sum(same)},
         "speedup_vs_plain": value / plain_value,
+        "kernels": kernels,
+        "p_ms": {"min": 1e3 * min(times), "max": 1e3 * max(times)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_learned(args, dev):
+    """Learned candidate ranking (voronoi_partition: 2e8 points, 20,000 Voronoi
+    cells, 256 x 256 grid): the plain quadtree join labels the first
+    --train-points points, the ranker trains on them on the GPU, and then both
+    the plain join and the learned locator run over all points; their pairs are
+    compared exactly, and the miss rate and exact tests per point are reported
+    with and without the model (list order)."""
+    import torch
+
+    import cuspatial_amd as cs
+    from cuspatial_amd import _capi, datasets, ranker
+
+    n = args.points if args.points != 100_000_000 else 200_000_000
+    log(f"[bench] generating voronoi_partition with {n} points on {dev}")
+    w = datasets.voronoi_partition(num_points=n, device=dev)
+    bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
+    xy = torch.stack([w.x, w.y], dim=1).reshape(-1)
+    del w.x, w.y
+    pts = cs.Points.from_points_xy(xy)
+    polys = cs.Polygons(w.part_offset, w.ring_offset, w.px, w.py)
+    t = time.perf_counter()
+    grid = ranker.CandidateGrid.build(polys, 0.0, 0.0, 1e5, 256)
+    grid_s = time.perf_counter() - t
+    over = int((grid.counts == ranker.OVERFLOW).sum())
+    log(f"[bench] {n} points, {len(polys)} polygons, grid 256x256 built in {grid_s:.2f} s "
+        f"({over} cells overflow, mean candidates "
+        f"{grid.counts[grid.counts <= 16].float().mean().item():.2f})")
+
+    def plain_step():
+        pi, tree = cs.quadtree_on_points(pts, *bbox, w.scale, w.max_depth, w.max_size)
+        bb = cs.polygon_bounding_boxes(polys)
+        pairs = cs.join_quadtree_and_bounding_boxes(tree, bb, *bbox, w.scale, w.max_depth)
+        return pi, cs.quadtree_point_in_polygon(pairs, tree, pi, pts, polys)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            out = fn()
+        del out
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            out = fn()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        return out, 1e3 * sum(times) / len(times), times
+
+    (pi, hits), plain_ms, _ = timed(plain_step, args.steps, args.warmup)
+    plain_value = n / (plain_ms / 1e3)
+    log(f"[bench] plain join: {plain_ms:.2f} ms per step, {plain_value:.4g} points/s, "
+        f"{len(hits)} hits")
+    pos = hits["point_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    want_pt = pi.view(torch.int32).to(torch.int64)[pos] & 0xFFFFFFFF
+    want_poly = hits["polygon_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    del pi, hits, pos
+    order = torch.argsort(want_pt * (1 << 32) + want_poly)
+    want_pt, want_poly = want_pt[order], want_poly[order]
+    del order
+    per_point = ranker.per_point_polygon(want_poly, want_pt, n)
+
+    # training on train_points points drawn at random, labelled by the plain join
+    m = min(args.train_points, n)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(w.seed)
+    sample = torch.randperm(n, generator=gen, device=dev)[:m]
+    train_xy = xy.view(-1, 2)[sample].reshape(-1)
+    labels = grid.slots_of(train_xy, per_point[sample])
+    model = ranker.Ranker(grid.num_cells, seed=1)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    hist = model.train(grid, train_xy, labels, epochs=args.epochs, batch=16384, lr=3e-3,
+                       log=lambda s: log("[bench] train " + s))
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t
+    del labels, train_xy, sample
+
+    loc = ranker.LearnedLocator(polys, grid, model, bbox, w.scale, w.max_depth, w.max_size)
+    _capi.profile_reset()
+    _capi.profile_enable(True)
+    res, learned_ms, times = timed(lambda: loc.locate(pts), args.steps, args.warmup)
+    _capi.profile_enable(False)
+    prof = _capi.profile_report()
+    value = n / (learned_ms / 1e3)
+    poly, pt = res.pairs()
+    exact = bool(torch.equal(poly, want_poly) and torch.equal(pt, want_pt))
+    has = per_point >= 0
+    miss = ((res.flags & 2) > 0) & has
+    tests_learned = res.tests.to(torch.float64).mean().item()
+    log(f"[bench] learned: {learned_ms:.2f} ms per step, {value:.4g} points/s; pairs identical "
+        f"to the plain join: {exact}; miss rate {miss.sum().item() / max(has.sum().item(), 1):.4f}"
+        f"; tests/point {tests_learned:.3f}; quadtree fallback {res.fallback}")
+    zero = ranker.LearnedLocator(polys, grid, ranker.Ranker(grid.num_cells, zero=True), bbox,
+                                 w.scale, w.max_depth, w.max_size)
+    zres, zero_ms, _ = timed(lambda: zero.locate(pts), 1, 1)
+    zpoly, zpt = zres.pairs()
+    zexact = bool(torch.equal(zpoly, want_poly) and torch.equal(zpt, want_pt))
+    zmiss = ((zres.flags & 2) > 0) & has
+    tests_zero = zres.tests.to(torch.float64).mean().item()
+    log(f"[bench] list order (no model): tests/point {tests_zero:.3f}, miss rate "
+        f"{zmiss.sum().item() / max(has.sum().item(), 1):.4f}, pairs identical: {zexact}")
+    launches = max(prof.get("ranker_rank", (1, 0.0))[0], 1)
+    loc_ms = prof.get("ranker_rank", (0, 0.0))[1] / launches
+    flops = 2.0 * (96 * 128 + 128 * 128 + 128 * 16) * n
+    kernels = {k: {"launches": c, "ms_per_launch": round(v / max(c, 1), 6)}
+               for k, (c, v) in prof.items()}
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": learned_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 exact test; bf16 MLP with fp32 accumulation",
+        "data": "synthetic",
+        "config": {"workload": "voronoi_learned: point location in a Voronoi partition",
+                   "points": n, "polygons": len(polys), "grid": "256x256",
+                   "overflowed_cells": over, "train_points": m, "epochs": args.epochs,
+                   "seed": w.seed},
+        "plain_join": {"value": plain_value, "unit": "points/s", "ms_per_step": plain_ms},
+        "speedup_vs_plain": value / plain_value,
+        "pairs_identical_to_plain": exact,
+        "miss_rate": miss.sum().item() / max(has.sum().item(), 1),
+        "tests_per_point": {"learned": tests_learned, "list_order": tests_zero},
+        "list_order_miss_rate": zmiss.sum().item() / max(has.sum().item(), 1),
+        "quadtree_fallback_points": res.fallback,
+        "training": {"seconds": train_s, "history": hist},
+        "roofline": {"kernel": "rank_kernel", "bound": "mfma", "engine": "bf16 MFMA",
+                     "achieved": flops / (loc_ms / 1e3) / 1e12 if loc_ms else None,
+                     "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": (flops / (loc_ms / 1e3) / 1e12 / 2500.0) if loc_ms else None,
+                     "traffic": None},
         "kernels": kernels,
         "p_ms": {"min": 1e3 * min(times), "max": 1e3 * max(times)},
     }

@@ -41,13 +41,10 @@ constexpr int kFreqs  = 8;   // frequencies per axis (kFour = 2 axes x 8 x sin/c
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-// Round to nearest even; NaN stays NaN.
+// Round to nearest even (v_cvt_pk_bf16_f32); NaN stays NaN.
 __device__ __forceinline__ uint16_t to_bf16(float f)
 {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7fffffffu) > 0x7f800000u) return static_cast<uint16_t>((u >> 16) | 0x40u);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<uint16_t>(u >> 16);
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
 }
 __device__ __forceinline__ float from_bf16(uint16_t h)
 {
@@ -79,19 +76,19 @@ struct Layout {
 };
 
 struct GridView {
-  double x0, y0, cell;
+  double x0, y0, cell, inv_cell;
   uint32_t dim;
   int32_t const* cand;
   uint8_t const* count;
 };
 
-// Cell coordinate floor((v - v0) / cell), clamped to [0, dim - 1] (NaN -> 0). The
-// function is monotone in v, so a polygon whose box spans [lo, hi] is listed in
-// every cell from coord(lo) to coord(hi), which includes the cell of each point it
-// contains.
-__device__ __forceinline__ uint32_t grid_coord(double v, double v0, double cell, uint32_t dim)
+// Cell coordinate floor((v - v0) * inv_cell) with inv_cell = 1 / cell, clamped to
+// [0, dim - 1] (NaN -> 0). The function is monotone in v, so a polygon whose box
+// spans [lo, hi] is listed in every cell from coord(lo) to coord(hi), which
+// includes the cell of each point it contains.
+__device__ __forceinline__ uint32_t grid_coord(double v, double v0, double inv_cell, uint32_t dim)
 {
-  double const t = floor((v - v0) / cell);
+  double const t = floor((v - v0) * inv_cell);
   if (!(t >= 0.0)) return 0u;
   if (t >= static_cast<double>(dim - 1)) return dim - 1;
   return static_cast<uint32_t>(t);
@@ -104,21 +101,25 @@ struct Loc {
 
 __device__ __forceinline__ Loc locate_cell(GridView const& g, double x, double y)
 {
-  uint32_t const ix = grid_coord(x, g.x0, g.cell, g.dim);
-  uint32_t const iy = grid_coord(y, g.y0, g.cell, g.dim);
-  double const u    = (x - (g.x0 + static_cast<double>(ix) * g.cell)) / g.cell;
-  double const v    = (y - (g.y0 + static_cast<double>(iy) * g.cell)) / g.cell;
+  uint32_t const ix = grid_coord(x, g.x0, g.inv_cell, g.dim);
+  uint32_t const iy = grid_coord(y, g.y0, g.inv_cell, g.dim);
+  double const u    = (x - (g.x0 + static_cast<double>(ix) * g.cell)) * g.inv_cell;
+  double const v    = (y - (g.y0 + static_cast<double>(iy) * g.cell)) * g.inv_cell;
   return Loc{iy * g.dim + ix, static_cast<float>(u), static_cast<float>(v)};
 }
 
-// Fourier features of frequency index k (0..7): sin, cos of pi (k+1) u, then of v.
+// Fourier features of frequency index k (0..7): sin, cos of pi (k+1) u, then of v,
+// from the hardware sine and cosine (v_sin_f32 / v_cos_f32 take revolutions:
+// sin(pi (k+1) u) = sin(2 pi r) with r = (k+1) u / 2). Training and location use
+// this same function, so the model sees identical features in both.
 __device__ __forceinline__ void fourier4(float u, float v, int k, float (&f)[4])
 {
-  float const w = 3.14159265358979f * static_cast<float>(k + 1);
-  f[0]          = sinf(w * u);
-  f[1]          = cosf(w * u);
-  f[2]          = sinf(w * v);
-  f[3]          = cosf(w * v);
+  float const h  = 0.5f * static_cast<float>(k + 1);
+  float const ru = h * u, rv = h * v;
+  f[0]           = __builtin_amdgcn_sinf(ru);
+  f[1]           = __builtin_amdgcn_cosf(ru);
+  f[2]           = __builtin_amdgcn_sinf(rv);
+  f[3]           = __builtin_amdgcn_cosf(rv);
 }
 
 // ---------------------------------------------------------------------------------
@@ -313,7 +314,35 @@ struct PolyView {
   Edge const* edges;
   EdgeB const* edges_b;
   uint32_t const* ring_edges;
+  struct Record const* records;
 };
+
+// Per polygon with exactly one ring: its effective edges [e0, e0 + ne) and the y
+// the ring's walk starts from (edges_b[e0].by), so a test needs one load before
+// the edges. ne == kMultiRing marks a polygon tested through the ring tables.
+constexpr uint32_t kMultiRing = 0xffffffffu;
+struct alignas(16) Record {
+  uint32_t e0, ne;
+  double y0;
+};
+
+__global__ void __launch_bounds__(256) records_kernel(PolyView P, Record* __restrict__ out)
+{
+  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= P.num_polys) return;
+  uint32_t const r0 = P.poly_offsets[i];
+  uint32_t r1       = P.poly_offsets[i + 1];
+  if (r1 > P.num_rings) r1 = static_cast<uint32_t>(P.num_rings);
+  Record rec{0u, kMultiRing, 0.0};
+  if (r1 == r0 + 1) {
+    rec.e0 = P.ring_offsets[r0];
+    rec.ne = P.ring_edges[r0];
+    rec.y0 = rec.ne ? P.edges_b[rec.e0].by : 0.0;
+  } else if (r1 <= r0) {
+    rec.ne = 0;  // no rings: never contains a point
+  }
+  out[i] = rec;
+}
 
 struct LocateArgs {
   GridView g;
@@ -327,14 +356,45 @@ struct LocateArgs {
   int32_t* out_poly;
   uint8_t* out_tests;
   uint8_t* out_flags;
+  uint64_t* order;  // per point: candidate slots by rank, 4 bits each
 };
 
 // The exact test of the quadtree path for one point and one polygon, plus whether
 // the point lies within the collinearity filter of any edge (then a neighbouring
 // polygon's test could also hold, and the point goes to the quadtree path).
+__device__ __forceinline__ void edge_step(Edge const& E, EdgeB const* eb, double px, double py,
+                                          bool& y0, bool& inside, bool& on_edge, bool& near)
+{
+  double const lhs = (px - E.ax) * E.rise;  // run_to_point * rise
+  double const rhs = E.run * (py - E.ay);   // run * rise_to_point
+  double const gap = fma(fabs(rhs), -0x1p-48, fabs(rhs - lhs));
+  if (!(gap > 0x1p-1020)) {
+    near = true;
+    if (float_equal(rhs, lhs)) {  // is_point_in_polygon.cuh:56-65
+      double const bx   = eb->bx;
+      double const minx = E.ax < bx ? E.ax : bx;
+      double const maxx = E.ax < bx ? bx : E.ax;
+      if (minx <= px && px <= maxx) on_edge = true;
+    }
+  }
+  bool const y1 = E.ay > py;
+  if (y1 != y0 && ((lhs < rhs) != y1)) inside = !inside;
+  y0 = y1;
+}
+
 __device__ __forceinline__ void exact_test(PolyView const& P, uint32_t poly, double px, double py,
                                            bool& inside_out, bool& near_out)
 {
+  Record const rec = P.records[poly];
+  if (rec.ne != kMultiRing) {
+    bool inside = false, on_edge = false, near = false;
+    bool y0 = rec.y0 > py;
+    for (uint32_t e = 0; e < rec.ne; ++e)
+      edge_step(P.edges[rec.e0 + e], P.edges_b + rec.e0 + e, px, py, y0, inside, on_edge, near);
+    inside_out = inside && !on_edge;
+    near_out   = near;
+    return;
+  }
   uint32_t const r0 = P.poly_offsets[poly];
   uint32_t r1       = P.poly_offsets[poly + 1];
   if (r1 > P.num_rings) r1 = static_cast<uint32_t>(P.num_rings);
@@ -344,24 +404,8 @@ __device__ __forceinline__ void exact_test(PolyView const& P, uint32_t poly, dou
     uint32_t const ne = P.ring_edges[r];
     if (ne == 0) continue;
     bool y0 = P.edges_b[e0].by > py;
-    for (uint32_t e = 0; e < ne; ++e) {
-      Edge const E     = P.edges[e0 + e];
-      double const lhs = (px - E.ax) * E.rise;  // run_to_point * rise
-      double const rhs = E.run * (py - E.ay);   // run * rise_to_point
-      double const gap = fma(fabs(rhs), -0x1p-48, fabs(rhs - lhs));
-      if (!(gap > 0x1p-1020)) {
-        near = true;
-        if (float_equal(rhs, lhs)) {  // is_point_in_polygon.cuh:56-65
-          double const bx   = P.edges_b[e0 + e].bx;
-          double const minx = E.ax < bx ? E.ax : bx;
-          double const maxx = E.ax < bx ? bx : E.ax;
-          if (minx <= px && px <= maxx) on_edge = true;
-        }
-      }
-      bool const y1 = E.ay > py;
-      if (y1 != y0 && ((lhs < rhs) != y1)) inside = !inside;
-      y0 = y1;
-    }
+    for (uint32_t e = 0; e < ne; ++e)
+      edge_step(P.edges[e0 + e], P.edges_b + e0 + e, px, py, y0, inside, on_edge, near);
   }
   inside_out = inside && !on_edge;
   near_out   = near;
@@ -372,7 +416,7 @@ __device__ __forceinline__ bf16x4 load_frag(uint16_t const* lds, int frag, int l
   return *reinterpret_cast<bf16x4 const*>(lds + (static_cast<int64_t>(frag) * kWave + lane) * 4);
 }
 
-__global__ void __launch_bounds__(kLocWaves * kWave) locate_kernel(LocateArgs a)
+__global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
 {
   __shared__ __attribute__((aligned(16))) uint16_t w_lds[kFrags * kWave * 4];
   __shared__ float b_lds[kHid + kHid + kCand];
@@ -412,16 +456,13 @@ __global__ void __launch_bounds__(kLocWaves * kWave) locate_kernel(LocateArgs a)
 
   for (int64_t base = (static_cast<int64_t>(blockIdx.x) * kLocWaves + wave) * kWave; base < a.n;
        base += waves * kWave) {
-    double px[kLocTiles], py[kLocTiles];
     uint32_t cell[kLocTiles];
     bf16x4 X[kLocTiles][kIn / 16];
 #pragma unroll
     for (int t = 0; t < kLocTiles; ++t) {
       int64_t const p = base + 16 * t + r;
       bool const in   = p < a.n;
-      px[t]           = in ? a.xy[2 * p] : 0.0;
-      py[t]           = in ? a.xy[2 * p + 1] : 0.0;
-      Loc const L     = locate_cell(a.g, px[t], py[t]);
+      Loc const L     = locate_cell(a.g, in ? a.xy[2 * p] : 0.0, in ? a.xy[2 * p + 1] : 0.0);
       cell[t]         = L.cell;
       uint16_t const* e = a.pb + a.lay.e + static_cast<int64_t>(L.cell) * kEmb;
 #pragma unroll
@@ -505,57 +546,71 @@ __global__ void __launch_bounds__(kLocWaves * kWave) locate_kernel(LocateArgs a)
     // this lane's point is tile g, row r: base + 16 g + r = base + l
     int64_t const p = base + l;
     if (p >= a.n) continue;
-    double x = px[0], y = py[0];
     uint32_t c = cell[0];
 #pragma unroll
     for (int t = 1; t < kLocTiles; ++t)
-      if (g == t) {
-        x = px[t];
-        y = py[t];
-        c = cell[t];
-      }
-    int const cnt = a.g.count[c];
-    int32_t result = -1;
-    uint8_t flags  = 0;
-    int tests      = 0;
-    if (cnt > kCand) {
-      flags = 1;  // overflowed cell: quadtree path
-    } else {
-      uint32_t tested = 0;
-      int const limit = cnt < a.max_tests ? cnt : a.max_tests;
-      while (tests < limit) {
-        int s   = -1;
-        float b = 0.f;
-        for (int q = 0; q < cnt; ++q) {
-          if ((tested >> q) & 1u) continue;
-          float const zq = z[q] == z[q] ? z[q] : -INFINITY;
-          if (s < 0 || zq > b) {
-            s = q;
-            b = zq;
-          }
-        }
-        tested |= 1u << s;
-        ++tests;
-        int32_t const poly = a.g.cand[static_cast<int64_t>(c) * kCand + s];
-        if (poly < 0 || poly >= a.P.num_polys) {
-          flags |= 1;
-          break;
-        }
-        bool inside, near;
-        exact_test(a.P, static_cast<uint32_t>(poly), x, y, inside, near);
-        if (inside) {
-          if (near) flags |= 1;
-          else result = poly;
-          break;
+      if (g == t) c = cell[t];
+    int const cnt   = a.g.count[c];
+    int const limit = cnt > kCand ? 0 : (cnt < a.max_tests ? cnt : a.max_tests);
+    uint64_t order  = 0;
+    uint32_t used   = 0;
+    for (int k = 0; k < limit; ++k) {  // slots by decreasing logit, ties to the lower slot
+      int s   = -1;
+      float b = 0.f;
+      for (int q = 0; q < cnt; ++q) {
+        if ((used >> q) & 1u) continue;
+        float const zq = z[q] == z[q] ? z[q] : -INFINITY;
+        if (s < 0 || zq > b) {
+          s = q;
+          b = zq;
         }
       }
-      if (result < 0 && !(flags & 1) && tests < cnt) flags |= 1;  // stopped at max_tests
-      if (!(result >= 0 && tests == 1)) flags |= 2;  // first prediction was not the answer
+      used |= 1u << s;
+      order |= static_cast<uint64_t>(s) << (4 * k);
     }
-    a.out_poly[p]  = (flags & 1) ? -2 : result;
-    a.out_tests[p] = static_cast<uint8_t>(tests);
-    a.out_flags[p] = flags;
+    a.order[p] = order;
   }
+}
+
+// Exact tests in predicted order, one lane per point.
+__global__ void __launch_bounds__(256) verify_kernel(LocateArgs a)
+{
+  int64_t const p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  double const x  = a.xy[2 * p];
+  double const y  = a.xy[2 * p + 1];
+  uint32_t const c = locate_cell(a.g, x, y).cell;
+  int const cnt    = a.g.count[c];
+  int32_t result   = -1;
+  uint8_t flags    = 0;
+  int tests        = 0;
+  if (cnt > kCand) {
+    flags = 1;  // overflowed cell: quadtree path
+  } else {
+    uint64_t const order = a.order[p];
+    int const limit      = cnt < a.max_tests ? cnt : a.max_tests;
+    while (tests < limit) {
+      int const s = static_cast<int>((order >> (4 * tests)) & 15u);
+      ++tests;
+      int32_t const poly = a.g.cand[static_cast<int64_t>(c) * kCand + s];
+      if (poly < 0 || poly >= a.P.num_polys) {
+        flags |= 1;
+        break;
+      }
+      bool inside, near;
+      exact_test(a.P, static_cast<uint32_t>(poly), x, y, inside, near);
+      if (inside) {
+        if (near) flags |= 1;
+        else result = poly;
+        break;
+      }
+    }
+    if (result < 0 && !(flags & 1) && tests < cnt) flags |= 1;  // stopped at max_tests
+    if (!(result >= 0 && tests == 1)) flags |= 2;  // first prediction was not the answer
+  }
+  a.out_poly[p]  = (flags & 1) ? -2 : result;
+  a.out_tests[p] = static_cast<uint8_t>(tests);
+  a.out_flags[p] = flags;
 }
 
 // ---------------------------------------------------------------------------------
@@ -566,7 +621,8 @@ GridView grid_view(qpip_grid const* g)
   QPIP_EXPECTS(g != nullptr, "grid must not be null");
   QPIP_EXPECTS(g->dim > 0 && g->dim <= 65536, "grid dimension must be in [1, 65536]");
   QPIP_EXPECTS(g->cell > 0.0, "grid cell size must be positive");
-  return GridView{g->x0, g->y0, g->cell, static_cast<uint32_t>(g->dim), g->candidates, g->counts};
+  return GridView{g->x0, g->y0, g->cell, 1.0 / g->cell, static_cast<uint32_t>(g->dim),
+                  g->candidates, g->counts};
 }
 
 void check_model(qpip_ranker const* m, qpip_grid const* g)
@@ -856,15 +912,31 @@ extern "C" int qpip_ranker_locate(const qpip_ranker* model,
     a.out_poly  = out_polygon;
     a.out_tests = out_tests;
     a.out_flags = out_flags;
+    dbuf<Record> records(std::max<int64_t>(polys, 1));
+    a.P.records = records.get();
+    if (polys > 0) {
+      records_kernel<<<grid_for(static_cast<uint64_t>(polys), 256), 256, 0, stream>>>(
+        a.P, records.get());
+      QPIP_LAUNCH_CHECK();
+    }
+    dbuf<uint64_t> order(num_points);
+    a.order = order.get();
     int dev = 0, cus = 0;
     QPIP_HIP(hipGetDevice(&dev));
     QPIP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     uint64_t const groups = (static_cast<uint64_t>(num_points) + kWave - 1) / kWave;
     uint64_t const want   = (groups + kLocWaves - 1) / kLocWaves;
     unsigned const blocks = static_cast<unsigned>(std::min<uint64_t>(want, uint64_t(2) * cus));
-    KernelTimer kt("ranker_locate", stream);
-    locate_kernel<<<blocks, kLocWaves * kWave, 0, stream>>>(a);
-    QPIP_LAUNCH_CHECK();
+    {
+      KernelTimer kt("ranker_rank", stream);
+      rank_kernel<<<blocks, kLocWaves * kWave, 0, stream>>>(a);
+      QPIP_LAUNCH_CHECK();
+    }
+    {
+      KernelTimer kt("ranker_verify", stream);
+      verify_kernel<<<grid_for(static_cast<uint64_t>(num_points), 256, 1u << 30), 256, 0, stream>>>(a);
+      QPIP_LAUNCH_CHECK();
+    }
     QPIP_HIP(hipStreamSynchronize(stream));
   });
 }

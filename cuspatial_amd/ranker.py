@@ -55,13 +55,13 @@ class qpip_ranker(ctypes.Structure):
 
 
 def _grid_coord_np(v, v0, cell, dim):
-    t = np.floor((np.asarray(v, np.float64) - v0) / cell)
+    t = np.floor((np.asarray(v, np.float64) - v0) * (1.0 / cell))
     t = np.where(t >= 0.0, t, 0.0)
     return np.minimum(t, dim - 1).astype(np.int64)
 
 
 def _grid_coord_t(v: torch.Tensor, v0, cell, dim) -> torch.Tensor:
-    t = torch.floor((v - v0) / cell)
+    t = torch.floor((v - v0) * (1.0 / cell))
     t = torch.where(t >= 0.0, t, torch.zeros_like(t))
     return torch.clamp(t, max=dim - 1).to(torch.int64)
 
@@ -79,8 +79,8 @@ class CandidateGrid:
     def build(cls, polygons: Polygons, x0: float, y0: float, extent: float, dim: int = 256,
               device=None) -> "CandidateGrid":
         """Lists every polygon whose bounding box reaches each cell. The cell of a
-        coordinate is floor((v - v0) / cell) clamped to [0, dim - 1], the same
-        function the kernels use, so the list of a point's cell contains every
+        coordinate is floor((v - v0) * (1 / cell)) clamped to [0, dim - 1], the
+        same function the kernels use, so the list of a point's cell contains every
         polygon that contains the point."""
         dev = device if device is not None else polygons.device
         part = polygons.part_offset.cpu().numpy().astype(np.int64)

@@ -388,8 +388,9 @@ QPIP_API int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
  * reference; an addition for point location over non-overlapping polygons).
  *
  * qpip_grid: a dim x dim grid of square cells of side `cell` from (x0, y0).
- * A point's cell is (floor((x - x0) / cell), floor((y - y0) / cell)), each
- * clamped to [0, dim - 1] (NaN -> 0), row-major (iy * dim + ix). candidates
+ * A point's cell is (floor((x - x0) * r), floor((y - y0) * r)) with r = 1 / cell
+ * (one double division), each clamped to [0, dim - 1] (NaN -> 0), row-major
+ * (iy * dim + ix). candidates
  * holds 16 polygon ids per cell (-1 padded) and counts the number of valid
  * slots; a count above 16 marks a cell whose points always take the exact
  * quadtree path. The candidate list of a cell must contain every polygon whose
@@ -399,7 +400,9 @@ QPIP_API int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
  * copy (params_bf16, same element offsets): E[num_cells][64], W1[128][96],
  * b1[128], W2[128][128], b2[128], W3[16][128], b3[16] (qpip_ranker_num_params
  * floats). Features of a point: E[cell] and sin, cos of pi k u and pi k v for
- * k = 1..8, where (u, v) is the point's position inside its cell in cell units.
+ * k = 1..8, where (u, v) = ((x - (x0 + ix cell)) r, (y - (y0 + iy cell)) r) is
+ * the point's position inside its cell in cell units (sine and cosine from the
+ * hardware's v_sin_f32 / v_cos_f32).
  * logits = W3 relu(W2 relu(W1 x + b1) + b2) + b3 with bf16 operands and fp32
  * accumulation.
  */

@@ -77,8 +77,8 @@ def from_bf16_bits(h) -> np.ndarray:
 # Grid
 # ---------------------------------------------------------------------------
 def grid_coord(v, v0: float, cell: float, dim: int) -> np.ndarray:
-    """floor((v - v0) / cell) clamped to [0, dim - 1], NaN -> 0."""
-    t = np.floor((np.asarray(v, np.float64) - v0) / cell)
+    """floor((v - v0) * (1 / cell)) clamped to [0, dim - 1], NaN -> 0."""
+    t = np.floor((np.asarray(v, np.float64) - v0) * (1.0 / cell))
     t = np.where(t >= 0.0, t, 0.0)
     return np.minimum(t, dim - 1).astype(np.int64)
 
@@ -90,8 +90,8 @@ def point_cells(x, y, x0, y0, cell, dim) -> np.ndarray:
 def local_uv(x, y, x0, y0, cell, dim):
     ix = grid_coord(x, x0, cell, dim)
     iy = grid_coord(y, y0, cell, dim)
-    u = (np.asarray(x, np.float64) - (x0 + ix.astype(np.float64) * cell)) / cell
-    v = (np.asarray(y, np.float64) - (y0 + iy.astype(np.float64) * cell)) / cell
+    u = (np.asarray(x, np.float64) - (x0 + ix.astype(np.float64) * cell)) * (1.0 / cell)
+    v = (np.asarray(y, np.float64) - (y0 + iy.astype(np.float64) * cell)) * (1.0 / cell)
     return u.astype(np.float32), v.astype(np.float32)
 
 

@@ -180,3 +180,48 @@ def test_locator_errors(env):
     loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
     with pytest.raises(RuntimeError, match="max_tests must be at least 1"):
         loc.locate(cs.Points.from_points_xy(xy), max_tests=0)
+
+
+def test_locator_partition_with_holes(env):
+    """A partition whose first 40 cells have a hole (the cell shrunk towards its
+    centroid) filled by an island polygon: the holed cells take the ring-table
+    path of the exact test, the islands the single-ring path; pairs equal the
+    plain join's."""
+    torch, cs, ranker, rk, datasets = env
+    part, ring, px, py = datasets.voronoi_cells(12, 300, 1e5)
+    parts, rings, xs, ys = [0], [0], [], []
+    islands = []
+    for i in range(len(part) - 1):
+        a, b = ring[part[i]], ring[part[i + 1]]
+        vx, vy = px[a:b], py[a:b]
+        xs.append(vx)
+        ys.append(vy)
+        rings.append(rings[-1] + len(vx))
+        nr = 1
+        if i < 40:
+            cx, cy = vx[:-1].mean(), vy[:-1].mean()
+            hx, hy = cx + 0.4 * (vx - cx), cy + 0.4 * (vy - cy)
+            xs.append(hx)
+            ys.append(hy)
+            rings.append(rings[-1] + len(hx))
+            nr = 2
+            islands.append((hx, hy))
+        parts.append(parts[-1] + nr)
+    for hx, hy in islands:
+        xs.append(hx)
+        ys.append(hy)
+        rings.append(rings[-1] + len(hx))
+        parts.append(parts[-1] + 1)
+    P = cs.Polygons(np.array(parts, np.int32), np.array(rings, np.int32), np.concatenate(xs),
+                    np.concatenate(ys), device="cuda")
+    grid = ranker.CandidateGrid.build(P, 0.0, 0.0, 1e5, 32)
+    rng = np.random.default_rng(13)
+    x, y = rng.uniform(0, 1e5, 100000), rng.uniform(0, 1e5, 100000)
+    xy = torch.as_tensor(np.stack([x, y], 1).reshape(-1), device="cuda")
+    want_poly, want_pt = exact_pairs(env, P, xy)
+    assert int((want_poly >= 300).sum()) > 100  # points inside islands
+    loc = ranker.LearnedLocator(P, grid, ranker.Ranker(grid.num_cells, seed=14),
+                                (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+    res = loc.locate(cs.Points.from_points_xy(xy))
+    poly, pt = res.pairs()
+    assert torch.equal(poly, want_poly) and torch.equal(pt, want_pt)
