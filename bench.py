@@ -82,13 +82,18 @@ def main():
                     help="target CPU work of the oracle baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="gaussian_holes",
-                    choices=["gaussian_holes", "batched_jobs", "voronoi_learned"],
+                    choices=["gaussian_holes", "batched_jobs", "voronoi_learned", "serving"],
                     help="gaussian_holes: the plain join at 1e8 points (the headline line); "
                          "batched_jobs: 4096 independent jobs through the batched entry point, "
                          "with the plain per-job calls on the same inputs timed beside it; "
                          "voronoi_learned: 2e8 points located in a 20,000-cell Voronoi "
                          "partition by the learned ranker + exact test, against the plain "
-                         "join on the same inputs")
+                         "join on the same inputs; serving: the request loop over both "
+                         "resident indexes at a sweep of Poisson request rates")
+    ap.add_argument("--qps", default="1000,2000,5000,10000,20000,50000",
+                    help="request rates of the serving sweep")
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="arrival window per rate of the serving sweep")
     ap.add_argument("--train-points", type=int, default=10_000_000,
                     help="labelled points the ranker is trained on (voronoi_learned)")
     ap.add_argument("--epochs", type=int, default=3, help="ranker training epochs")
@@ -123,6 +128,8 @@ def main():
         return bench_batched(args, dev)
     if args.workload == "voronoi_learned":
         return bench_learned(args, dev)
+    if args.workload == "serving":
+        return bench_serving(args, dev)
     log(f"[bench] generating gaussian_holes with {args.points} points on {dev}")
     w = datasets.gaussian_holes(num_points=args.points, device=dev)
     bbox = (w.x_min, w.x_max, w.y_min, w.y_max)
@@ -513,6 +520,84 @@ def bench_learned(args, dev):
                      "traffic": None},
         "kernels": kernels,
         "p_ms": {"min": 1e3 * min(times), "max": 1e3 * max(times)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serving(args, dev):
+    """Reverse geocoding under load: Poisson arrivals at each rate of --qps, request
+    sizes log-uniform in [1, 1e6] points drawn from the Gaussian mixture; two
+    resident indexes: the 50,000 star polygons of gaussian_holes (plain quadtree
+    join per batch) and the 20,000-cell Voronoi partition (learned locator with
+    the exact fallback; ranker trained at start-up on 1e7 points labelled by the
+    plain join). Reports p50/p95/p99 latency and points/s per rate."""
+    import torch
+
+    import cuspatial_amd as cs
+    from cuspatial_amd import datasets, ranker, serving
+
+    log("[bench] serving: building the resident indexes")
+    gh = datasets.gaussian_holes(num_points=1, device=dev)
+    stars = cs.Polygons(gh.part_offset, gh.ring_offset, gh.px, gh.py)
+    bbox = (gh.x_min, gh.x_max, gh.y_min, gh.y_max)
+    plain_index = serving.ResidentIndex(stars, bbox, gh.scale, gh.max_depth, gh.max_size)
+    vw = datasets.voronoi_partition(num_points=10_000_000, device=dev)
+    vor = cs.Polygons(vw.part_offset, vw.ring_offset, vw.px, vw.py)
+    grid = ranker.CandidateGrid.build(vor, 0.0, 0.0, 1e5, 256)
+    txy = torch.stack([vw.x, vw.y], 1).reshape(-1)
+    poly, pt = ranker.plain_pairs(cs.Points.from_points_xy(txy), vor, bbox, vw.scale,
+                                  vw.max_depth, vw.max_size)
+    labels = grid.slots_of(txy, ranker.per_point_polygon(poly, pt, txy.numel() // 2))
+    model = ranker.Ranker(grid.num_cells, seed=1)
+    model.train(grid, txy, labels, epochs=3, batch=16384, lr=3e-3)
+    del txy, labels, poly, pt
+    loc = ranker.LearnedLocator(vor, grid, model, bbox, vw.scale, vw.max_depth, vw.max_size)
+    learned_index = serving.ResidentIndex(vor, bbox, vw.scale, vw.max_depth, vw.max_size,
+                                          locator=loc)
+    # request points: a pool of 2e8 mixture points resident on the device
+    px, py = datasets._gaussian_mixture_device(2005, 200_000_000, 1e5, dev)
+    pool_xy = torch.stack([px, py], 1).reshape(-1)
+    del px, py
+    torch.cuda.synchronize()
+    rates = [float(q) for q in args.qps.split(",") if q]
+    sweep = {}
+    for name, index in (("plain_stars", plain_index), ("learned_voronoi", learned_index)):
+        # warm the allocator and the kernels once
+        serving.Server(index).serve(serving.poisson_requests(pool_xy, 1000.0, 20, seed=7))
+        rows = []
+        for qps in rates:
+            count = max(200, int(qps * args.serve_seconds))
+            reqs = serving.poisson_requests(pool_xy, qps, count, seed=int(qps))
+            st = serving.Server(index, workers=3, memory_budget=32 << 30,
+                                bytes_per_point=256).serve(reqs)
+            row = {"qps": qps, "requests": st.requests, "points": st.points,
+                   "points_per_s": st.points_per_s, "achieved_qps": st.requests / st.seconds,
+                   "latency_ms": st.latency_ms, "batches": st.batches,
+                   "mean_batch_requests": st.mean_batch_requests,
+                   "max_in_flight": st.max_in_flight}
+            rows.append(row)
+            log(f"[bench] {name} qps {qps:g}: p50 {st.latency_ms['p50']:.2f} ms, p95 "
+                f"{st.latency_ms['p95']:.2f} ms, p99 {st.latency_ms['p99']:.2f} ms, "
+                f"{st.points_per_s:.4g} points/s, {st.batches} batches")
+        sweep[name] = rows
+    best = max(r["points_per_s"] for rows in sweep.values() for r in rows)
+    line = {
+        "metric": METRIC,
+        "value": best,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": len(rates),
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "serving: Poisson reverse-geocoding requests, sizes "
+                               "log-uniform [1, 1e6] points",
+                   "workers": 3, "memory_budget_gib": 32, "serve_seconds": args.serve_seconds},
+        "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
 

@@ -89,12 +89,13 @@ struct PipArgs {
 };
 
 // --- 3. test ------------------------------------------------------------------------
-// One wave per run of pairs that share a leaf (the join orders pairs by leaf offset,
-// so such pairs are adjacent): the wave of the run's first pair gathers the leaf's
-// points up to 4 chunks (256 points, U per lane) at a time and tests them against
-// every polygon of the run; the waves of the run's other pairs return at once.
-// Masks are stored per (pair, chunk) in pair order: chunk j of the run's i-th pair
-// is chunk_start[first pair] + i * chunks + j.
+// One wave per segment of at most kPairsPerWave pairs that share a leaf (the join
+// orders pairs by leaf offset, so such pairs are adjacent; segments start at a run's
+// first pair and at every multiple of kPairsPerWave inside a run): the segment's
+// first wave gathers the leaf's points up to 4 chunks (256 points, U per lane) at a
+// time and tests them against every polygon of the segment; the other waves return
+// at once. Masks are stored per (pair, chunk) in pair order: chunk j of pair k is
+// chunk_start[k] + j.
 //
 // Collinearity filter. If float_equal(rhs, lhs) (4 ULP), then for normal values
 // |rhs - lhs| is exact (Sterbenz) and below 2^-50 * max(|rhs|, |lhs|) <= 2^-48 *
@@ -181,6 +182,8 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
   }
 }
 
+constexpr uint32_t kPairsPerWave = 16;
+
 __global__ void __launch_bounds__(256)
   pip_test_kernel(PipArgs a, uint64_t* __restrict__ masks, uint32_t* __restrict__ hit_counts,
                   uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
@@ -188,7 +191,10 @@ __global__ void __launch_bounds__(256)
   uint32_t const k0 = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
   if (k0 >= a.num_pairs) return;
   uint32_t const quad = uniform(a.pair_quad[k0]);
-  if (k0 > 0 && uniform(a.pair_quad[k0 - 1]) == quad) return;  // not the run's first pair
+  // A wave takes the pairs of one leaf from k0 up to the run's end or the next
+  // multiple of kPairsPerWave, whichever comes first: a leaf that overlaps many
+  // polygons is spread over several waves instead of being walked by one.
+  if (k0 > 0 && k0 % kPairsPerWave != 0 && uniform(a.pair_quad[k0 - 1]) == quad) return;
   // Errors are recorded at the end: a global store before the edge loads would
   // keep them off the scalar unit (the compiler must assume it may alias).
   bool err = false;
@@ -237,7 +243,7 @@ __global__ void __launch_bounds__(256)
       valid_m[u] = ballot(valid);
     }
     for (uint32_t k = k0; k < a.num_pairs; ++k) {
-      if (k > k0 && uniform(a.pair_quad[k]) != quad) break;
+      if (k > k0 && (k % kPairsPerWave == 0 || uniform(a.pair_quad[k]) != quad)) break;
       uint32_t const poly = uniform(a.pair_poly[k]);
       uint64_t hit_m[4]   = {0, 0, 0, 0};
       switch (nu) {

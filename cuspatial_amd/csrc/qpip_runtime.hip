@@ -5,8 +5,16 @@
 // (cpp/benchmarks/fixture/rmm_pool_raii.hpp): hot-path temporaries are carved
 // from cached device blocks instead of hipMalloc/hipFree per call. Blocks are
 // binned by size; a cached block is reused for a request of at least half its
-// size. Every entry point synchronizes its stream before returning, so a block
-// handed back to the pool is idle and may be reused by any stream.
+// size.
+//
+// A temporary is handed back when it leaves scope, which is often before the
+// kernels queued on the call's stream have finished with it. Reusing it in the
+// same call is safe (the same stream orders the work), and so is reusing it in a
+// later call of the same host thread (every entry point synchronizes its stream
+// before returning). Another thread, on another stream, must not get it: its
+// kernels could overwrite the block while the first stream still reads it. So
+// free blocks are kept per host thread; when a thread exits, its blocks are idle
+// and move to a shared list that any thread may draw from.
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -21,27 +29,66 @@ namespace qpip {
 namespace {
 thread_local std::string g_last_error;
 
+using FreeList = std::multimap<size_t, void*>;  // size -> block
+
 struct DevicePool {
   std::mutex mu;
-  std::multimap<size_t, void*> free_blocks;  // size -> block
-  std::unordered_map<void*, size_t> live;    // block -> size
+  std::unordered_map<void*, size_t> live;  // block -> size
+  FreeList idle;                           // blocks of exited threads
   size_t live_bytes   = 0;
   size_t cached_bytes = 0;
 };
 
 std::mutex g_pools_mu;
-std::vector<DevicePool*> g_pools;
-size_t g_limit = 0;  // 0: no cap
+std::vector<DevicePool*> g_pools;  // never freed: used by thread exit handlers
+size_t g_limit = 0;                // 0: no cap
 
-DevicePool& pool_for_current_device()
+DevicePool& pool_for_device(int dev)
 {
-  int dev = 0;
-  QPIP_HIP(hipGetDevice(&dev));
   std::lock_guard<std::mutex> g(g_pools_mu);
   if (static_cast<size_t>(dev) >= g_pools.size()) g_pools.resize(dev + 1, nullptr);
   if (!g_pools[dev]) g_pools[dev] = new DevicePool();  // lives until process exit
   return *g_pools[dev];
 }
+
+int current_device()
+{
+  int dev = 0;
+  QPIP_HIP(hipGetDevice(&dev));
+  return dev;
+}
+
+// For pool_free, which runs in destructors and must not throw: after a device
+// error the runtime may report it here too; the block is then looked up on
+// device 0's pool (a single-device process) and the error is left for the
+// caller's next checked call.
+int current_device_nothrow()
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  return dev;
+}
+
+// This thread's free blocks, per device. Touched only by its own thread; the
+// byte counters it shares are updated under the device pool's mutex.
+struct ThreadCache {
+  std::vector<std::pair<DevicePool*, FreeList>> lists;  // index: device
+  FreeList& list(int dev, DevicePool* p)
+  {
+    if (static_cast<size_t>(dev) >= lists.size()) lists.resize(dev + 1, {nullptr, FreeList()});
+    lists[dev].first = p;
+    return lists[dev].second;
+  }
+  ~ThreadCache()
+  {
+    for (auto& [p, fl] : lists) {
+      if (!p || fl.empty()) continue;
+      std::lock_guard<std::mutex> g(p->mu);
+      p->idle.insert(fl.begin(), fl.end());
+    }
+  }
+};
+thread_local ThreadCache t_cache;
 
 size_t round_size(size_t bytes)
 {
@@ -50,11 +97,29 @@ size_t round_size(size_t bytes)
   return (bytes + gran - 1) / gran * gran;
 }
 
-void release_cached(DevicePool& p)
+bool take(FreeList& fl, size_t size, void*& blk, size_t& bsz)
 {
-  for (auto& kv : p.free_blocks) (void)hipFree(kv.second);
-  p.free_blocks.clear();
-  p.cached_bytes = 0;
+  auto it = fl.lower_bound(size);
+  if (it == fl.end() || it->first > 2 * size) return false;
+  blk = it->second;
+  bsz = it->first;
+  fl.erase(it);
+  return true;
+}
+
+// Frees this thread's cached blocks and the shared idle ones (caller holds p.mu).
+void release_cached(DevicePool& p, FreeList& mine)
+{
+  for (auto& kv : mine) {
+    (void)hipFree(kv.second);
+    p.cached_bytes -= kv.first;
+  }
+  mine.clear();
+  for (auto& kv : p.idle) {
+    (void)hipFree(kv.second);
+    p.cached_bytes -= kv.first;
+  }
+  p.idle.clear();
 }
 }  // namespace
 
@@ -64,27 +129,26 @@ std::string const& last_error() { return g_last_error; }
 void* pool_alloc(size_t bytes)
 {
   if (bytes == 0) return nullptr;
-  DevicePool& p     = pool_for_current_device();
+  int const dev     = current_device();
+  DevicePool& p     = pool_for_device(dev);
+  FreeList& mine    = t_cache.list(dev, &p);
   size_t const size = round_size(bytes);
   std::lock_guard<std::mutex> g(p.mu);
   if (g_limit && p.live_bytes + size > g_limit)
     throw oom_error("out of memory: library memory limit of " + std::to_string(g_limit) +
                     " bytes reached (requested " + std::to_string(size) + ")");
-  auto it = p.free_blocks.lower_bound(size);
-  if (it != p.free_blocks.end() && it->first <= 2 * size) {
-    void* blk        = it->second;
-    size_t const bsz = it->first;
-    p.free_blocks.erase(it);
+  void* blk  = nullptr;
+  size_t bsz = 0;
+  if (take(mine, size, blk, bsz) || take(p.idle, size, blk, bsz)) {
     p.cached_bytes -= bsz;
     p.live[blk] = bsz;
     p.live_bytes += bsz;
     return blk;
   }
-  void* blk      = nullptr;
   hipError_t err = hipMalloc(&blk, size);
   if (err != hipSuccess) {
     (void)hipGetLastError();
-    release_cached(p);
+    release_cached(p, mine);
     err = hipMalloc(&blk, size);
   }
   if (err != hipSuccess) {
@@ -99,14 +163,16 @@ void* pool_alloc(size_t bytes)
 void pool_free(void* blk)
 {
   if (!blk) return;
-  DevicePool& p = pool_for_current_device();
+  int const dev  = current_device_nothrow();
+  DevicePool& p  = pool_for_device(dev);
+  FreeList& mine = t_cache.list(dev, &p);
   std::lock_guard<std::mutex> g(p.mu);
   auto it = p.live.find(blk);
   if (it == p.live.end()) return;  // not ours
   size_t const sz = it->second;
   p.live.erase(it);
   p.live_bytes -= sz;
-  p.free_blocks.emplace(sz, blk);
+  mine.emplace(sz, blk);
   p.cached_bytes += sz;
 }
 
@@ -114,14 +180,16 @@ void pool_set_limit(size_t bytes) { g_limit = bytes; }
 
 void pool_trim()
 {
-  DevicePool& p = pool_for_current_device();
+  int const dev  = current_device();
+  DevicePool& p  = pool_for_device(dev);
+  FreeList& mine = t_cache.list(dev, &p);
   std::lock_guard<std::mutex> g(p.mu);
-  release_cached(p);
+  release_cached(p, mine);
 }
 
 size_t pool_bytes_in_use()
 {
-  DevicePool& p = pool_for_current_device();
+  DevicePool& p = pool_for_device(current_device());
   std::lock_guard<std::mutex> g(p.mu);
   return p.live_bytes;
 }

@@ -351,7 +351,7 @@ class LearnedLocator:
                 first[1:] = orig[1:] != orig[:-1]
             out[orig[first]] = poly[first].to(torch.int32)
             extra = (poly[~first], orig[~first])
-        torch.cuda.synchronize(dev)
+        torch.cuda.current_stream(dev).synchronize()
         t2 = time.perf_counter()
         return LocateResult(out, tests, flags, int(fb.numel()), extra,
                             {"learned": t1 - t0, "fallback": t2 - t1})
