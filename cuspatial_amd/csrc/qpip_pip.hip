@@ -86,12 +86,23 @@ struct PipArgs {
   uint64_t num_nodes;
   double const* xy;  // interleaved x, y pairs instead of x / y (null: two arrays)
   bool xy_vec;       // xy is 16-byte aligned
+  uint32_t pairs_per_wave;  // longest segment of a leaf's pairs one wave takes
 };
 
+// Segments of at most max(16, pairs / 65536) pairs: few pairs (a small join, or
+// leaves that overlap thousands of polygons) are spread over many waves; with
+// many pairs the GPU is full anyway and whole runs save re-gathering a leaf's
+// points per segment.
+inline uint32_t pairs_per_wave(uint64_t num_pairs)
+{
+  uint64_t const g = num_pairs >> 16;
+  return static_cast<uint32_t>(g < 16 ? 16 : (g > (1u << 30) ? (1u << 30) : g));
+}
+
 // --- 3. test ------------------------------------------------------------------------
-// One wave per segment of at most kPairsPerWave pairs that share a leaf (the join
+// One wave per segment of at most pairs_per_wave pairs that share a leaf (the join
 // orders pairs by leaf offset, so such pairs are adjacent; segments start at a run's
-// first pair and at every multiple of kPairsPerWave inside a run): the segment's
+// first pair and at every multiple of pairs_per_wave inside a run): the segment's
 // first wave gathers the leaf's points up to 4 chunks (256 points, U per lane) at a
 // time and tests them against every polygon of the segment; the other waves return
 // at once. Masks are stored per (pair, chunk) in pair order: chunk j of pair k is
@@ -182,8 +193,6 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
   }
 }
 
-constexpr uint32_t kPairsPerWave = 16;
-
 __global__ void __launch_bounds__(256)
   pip_test_kernel(PipArgs a, uint64_t* __restrict__ masks, uint32_t* __restrict__ hit_counts,
                   uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
@@ -192,9 +201,10 @@ __global__ void __launch_bounds__(256)
   if (k0 >= a.num_pairs) return;
   uint32_t const quad = uniform(a.pair_quad[k0]);
   // A wave takes the pairs of one leaf from k0 up to the run's end or the next
-  // multiple of kPairsPerWave, whichever comes first: a leaf that overlaps many
+  // multiple of pairs_per_wave, whichever comes first: a leaf that overlaps many
   // polygons is spread over several waves instead of being walked by one.
-  if (k0 > 0 && k0 % kPairsPerWave != 0 && uniform(a.pair_quad[k0 - 1]) == quad) return;
+  uint32_t const G = a.pairs_per_wave;
+  if (k0 > 0 && k0 % G != 0 && uniform(a.pair_quad[k0 - 1]) == quad) return;
   // Errors are recorded at the end: a global store before the edge loads would
   // keep them off the scalar unit (the compiler must assume it may alias).
   bool err = false;
@@ -243,7 +253,7 @@ __global__ void __launch_bounds__(256)
       valid_m[u] = ballot(valid);
     }
     for (uint32_t k = k0; k < a.num_pairs; ++k) {
-      if (k > k0 && (k % kPairsPerWave == 0 || uniform(a.pair_quad[k]) != quad)) break;
+      if (k > k0 && (k % G == 0 || uniform(a.pair_quad[k]) != quad)) break;
       uint32_t const poly = uniform(a.pair_poly[k]);
       uint64_t hit_m[4]   = {0, 0, 0, 0};
       switch (nu) {
@@ -482,7 +492,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                  static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
                  ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), T,
                  static_cast<uint64_t>(num_nodes), xy,
-                 xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0};
+                 xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
+                 pairs_per_wave(K)};
     // 3. test
     dbuf<uint64_t> masks(T);
     dbuf<uint32_t> hit_counts(T);
