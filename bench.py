@@ -329,22 +329,31 @@ def bench_batched(args, dev):
     value = n / (ms / 1e3)
     log(f"[bench] batched: {ms:.2f} ms per step, {value:.4g} points/s, hits={out.num_hits}")
 
-    # the plain path on the same inputs, job by job (the same device arrays)
+    # the plain path on the same inputs, job by job: each job's points and its own
+    # polygons (offsets rebased to the job, vertex arrays sliced), as a caller that
+    # owns one job's data passes them; built before the timed loop
     log("[bench] plain per-job calls")
+    xy = w.xy
+    part_h = np.asarray(w.part_offset, np.int64)
+    ring_h = np.asarray(w.ring_offset, np.int64)
+    ring_d = torch.as_tensor(w.ring_offset, device=dev)
+    part_d = torch.as_tensor(w.part_offset, device=dev)
+    jobs = []
+    for j in range(w.num_jobs):
+        a, b = int(w.job_point_offsets[j]), int(w.job_point_offsets[j + 1])
+        q0, q1 = int(w.job_polygon_offsets[j]), int(w.job_polygon_offsets[j + 1])
+        r0, r1 = int(part_h[q0]), int(part_h[q1])
+        v0, v1 = int(ring_h[r0]), int(ring_h[r1])
+        P = cs.Polygons((part_d[q0:q1 + 1] - r0).to(torch.int32),
+                        (ring_d[r0:r1 + 1] - v0).to(torch.int32), polys.x[v0:v1], polys.y[v0:v1])
+        jobs.append((cs.Points.from_points_xy(xy[2 * a:2 * b]), P, w.bboxes[j],
+                     int(w.max_depths[j]), int(w.max_sizes[j]), float(w.scales[j])))
     torch.cuda.synchronize()
     t = time.perf_counter()
     plain_hits = []
-    xy = w.xy
-    pp = torch.as_tensor(w.part_offset, device=dev)
-    for j in range(w.num_jobs):
-        a, b = int(w.job_point_offsets[j]), int(w.job_point_offsets[j + 1])
-        pj = cs.Points.from_points_xy(xy[2 * a:2 * b])
-        q0, q1 = int(w.job_polygon_offsets[j]), int(w.job_polygon_offsets[j + 1])
-        P = cs.Polygons(pp[q0:q1 + 1], polys.ring_offset, polys.x, polys.y)
-        x0, x1, y0, y1 = w.bboxes[j]
-        d, msz, sc = int(w.max_depths[j]), int(w.max_sizes[j]), float(w.scales[j])
+    for pj, P, (x0, x1, y0, y1), d, msz, sc in jobs:
         pi, tree = cs.quadtree_on_points(pj, x0, x1, y0, y1, sc, d, msz)
-        bb = _job_boxes(cs, P)
+        bb = cs.polygon_bounding_boxes(P)
         pairs = cs.join_quadtree_and_bounding_boxes(tree, bb, x0, x1, y0, y1, sc, d)
         res = cs.quadtree_point_in_polygon(pairs, tree, pi, pj, P)
         plain_hits.append(len(res))
@@ -600,21 +609,6 @@ def bench_serving(args, dev):
         "sweep": sweep,
     }
     print(json.dumps(line), flush=True)
-
-
-def _job_boxes(cs, P):
-    """Bounding boxes of a job's polygons given as a slice of the global polygon
-    arrays (offsets not rebased): computed on the rebased copy, as a caller that
-    owns the job's polygons would."""
-    import torch
-
-    part = P.part_offset
-    r0, r1 = int(part[0]), int(part[-1])
-    ring = P.ring_offset[r0:r1 + 1]
-    v0, v1 = int(ring[0]), int(ring[-1])
-    Q = cs.Polygons((part - r0).to(torch.int32), (ring - v0).to(torch.int32), P.x[v0:v1],
-                    P.y[v0:v1])
-    return cs.polygon_bounding_boxes(Q)
 
 
 if __name__ == "__main__":
