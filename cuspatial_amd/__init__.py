@@ -8,17 +8,20 @@ the C ABI in include/cuspatial_amd.h.
 
 from .spatial import (  # noqa: F401
     BatchResult,
+    Linestrings,
     Points,
     Polygons,
     Table,
     batched_quadtree_point_in_polygon,
     join_quadtree_and_bounding_boxes,
+    linestring_bounding_boxes,
     pairwise_point_in_polygon,
     point_in_polygon,
     point_in_polygon_bitmask,
     polygon_bounding_boxes,
     quadtree_on_points,
     quadtree_point_in_polygon,
+    quadtree_point_to_nearest_linestring,
 )
 from .binpreds import contains_properly, polygon_contains_properly  # noqa: F401
 from .ranker import CandidateGrid, LearnedLocator, Ranker  # noqa: F401
@@ -27,6 +30,9 @@ __all__ = [
     "Points",
     "Polygons",
     "Table",
+    "Linestrings",
+    "linestring_bounding_boxes",
+    "quadtree_point_to_nearest_linestring",
     "quadtree_on_points",
     "polygon_bounding_boxes",
     "join_quadtree_and_bounding_boxes",

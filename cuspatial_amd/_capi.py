@@ -62,6 +62,8 @@ EXPORTED = (
     "qpip_point_in_polygon",
     "qpip_pairwise_point_in_polygon",
     "qpip_batched_quadtree_point_in_polygon",
+    "qpip_linestring_bounding_boxes",
+    "qpip_quadtree_point_to_nearest_linestring",
     "qpip_ranker_num_params",
     "qpip_ranker_features",
     "qpip_ranker_logits",
@@ -119,6 +121,13 @@ def lib():
         L.qpip_batched_quadtree_point_in_polygon.argtypes = [
             _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
             _i64, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), _vp, P(_i64)]
+        L.qpip_linestring_bounding_boxes.argtypes = [
+            _vp, _i64, _vp, _vp, _i64, _i64, _d, P(qpip_resource), _vp,
+            P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]
+        L.qpip_quadtree_point_to_nearest_linestring.argtypes = [
+            _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
+            _vp, _i64, _vp, _vp, _i64, _i64, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp),
+            P(_i64)]
         L.qpip_ranker_num_params.argtypes = [_i64]
         L.qpip_ranker_num_params.restype = _i64
         L.qpip_ranker_features.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp]
@@ -129,6 +138,7 @@ def lib():
         L.qpip_ranker_locate.argtypes = [_vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _vp, _i64,
                                          ctypes.c_int32, _vp, _vp, _vp, _vp]
         for name in (
+            "qpip_linestring_bounding_boxes", "qpip_quadtree_point_to_nearest_linestring",
             "qpip_ranker_features", "qpip_ranker_logits", "qpip_ranker_gradients",
             "qpip_ranker_adam", "qpip_ranker_locate",
             "qpip_batched_quadtree_point_in_polygon",

@@ -127,6 +127,26 @@ class Polygons:
         return max(int(self.part_offset.numel()) - 1, 0)
 
 
+class Linestrings:
+    """Linestrings as GeoArrow offsets (the part_offset / x / y of a linestring
+    GeoSeries' ``lines`` accessor); ``geometry_offset`` groups parts into
+    multilinestrings."""
+
+    def __init__(self, part_offset, x, y, geometry_offset=None, device=None):
+        dev = _device(device if device is not None else
+                      (x.device if isinstance(x, torch.Tensor) and x.is_cuda else None))
+        self.part_offset = _as_tensor(part_offset, torch.int32, dev, "part_offset")
+        self.x = _as_tensor(x, torch.float64, dev, "x")
+        self.y = _as_tensor(y, torch.float64, dev, "y")
+        self.geometry_offset = (None if geometry_offset is None else
+                                _as_tensor(geometry_offset, torch.int32, dev, "geometry_offset"))
+        self.device = dev
+
+    def __len__(self):
+        g = self.geometry_offset
+        return max(int((g if g is not None else self.part_offset).numel()) - 1, 0)
+
+
 class Table(OrderedDict):
     """An ordered set of equal-length device columns (what the reference returns
     as a cudf.DataFrame)."""
@@ -446,3 +466,74 @@ def batched_quadtree_point_in_polygon(points: Points, job_point_offsets, polygon
     return BatchResult(res.take(pi.value, n if J > 0 else 0, torch.uint32),
                        res.take(po.value, h.value, torch.uint32),
                        res.take(pt.value, h.value, torch.uint32), jpo, hits)
+
+
+# ---------------------------------------------------------------------------
+# Linestrings
+# ---------------------------------------------------------------------------
+def linestring_bounding_boxes(linestrings: Linestrings, expansion_radius: float):
+    """Bounding box of each linestring, every vertex expanded by expansion_radius
+    (bounding.py:83-135). For multilinestrings the box covers all parts."""
+    cols = ["minx", "miny", "maxx", "maxy"]
+    if len(linestrings) == 0:
+        out = Table()
+        for c in cols:
+            out[c] = torch.zeros(0, dtype=torch.float64, device=linestrings.device)
+        return out
+    if not isinstance(linestrings, Linestrings):
+        raise ValueError("Geoseries must contain only linestrings.")
+    offs = linestrings.part_offset
+    if linestrings.geometry_offset is not None:  # part_offset.take(geometry_offset)
+        offs = offs[linestrings.geometry_offset.to(torch.int64)].contiguous()
+    dev = linestrings.device
+    res = _capi.TorchResource(dev)
+    o = _outs(4)
+    n = ctypes.c_int64()
+    _capi.check(_capi.lib().qpip_linestring_bounding_boxes(
+        _p(offs), offs.numel(), _p(linestrings.x), _p(linestrings.y), linestrings.x.numel(),
+        linestrings.y.numel(), float(expansion_radius), ctypes.byref(res.struct),
+        _capi.stream_handle(dev), *[ctypes.byref(v) for v in o], ctypes.byref(n)))
+    out = Table()
+    for c, v in zip(cols, o):
+        out[c] = res.take(v.value, n.value, torch.float64)
+    return out
+
+
+def quadtree_point_to_nearest_linestring(linestring_quad_pairs, quadtree, point_indices,
+                                         points: Points, linestrings: Linestrings):
+    """Nearest linestring of each point among those paired with its quadrant
+    (join.py:265-355). Returns columns point_index (a position in
+    ``point_indices``), linestring_index and distance, one row per point."""
+    if not isinstance(points, Points):
+        raise ValueError("`point` Geoseries must contains only point geometries.")
+    if not isinstance(linestrings, Linestrings):
+        raise ValueError("`linestrings` Geoseries must contains only linestring geometries.")
+    if linestrings.geometry_offset is not None and \
+            linestrings.geometry_offset.numel() != linestrings.part_offset.numel():
+        raise ValueError("GeoSeries cannot contain multilinestrings.")
+    pcols = list(linestring_quad_pairs.values()) if isinstance(linestring_quad_pairs, dict) \
+        else list(linestring_quad_pairs)
+    if len(pcols) != 2:
+        raise RuntimeError("cuSpatial failure: a quadrant-linestring table must have 2 columns")
+    key, level, internal, length, offset = _tree_columns(quadtree)
+    dev = points.device
+    pl = _u32(pcols[0], "linestring_quad_pairs")
+    pq = _u32(pcols[1], "linestring_quad_pairs")
+    pi = _u32(point_indices, "point_indices")
+    x = points.x.contiguous()
+    y = points.y.contiguous()
+    lo = _u32(linestrings.part_offset, "linestring_offsets")
+    res = _capi.TorchResource(dev)
+    o = _outs(3)
+    n = ctypes.c_int64()
+    _capi.check(_capi.lib().qpip_quadtree_point_to_nearest_linestring(
+        _p(pl), _p(pq), pl.numel(), _p(key), _p(level), _p(internal), _p(length), _p(offset),
+        key.numel(), _p(pi), pi.numel(), _p(x), _p(y), x.numel(), y.numel(), _p(lo), lo.numel(),
+        _p(linestrings.x), _p(linestrings.y), linestrings.x.numel(), linestrings.y.numel(),
+        ctypes.byref(res.struct), _capi.stream_handle(dev), *[ctypes.byref(v) for v in o],
+        ctypes.byref(n)))
+    out = Table()
+    out["point_index"] = res.take(o[0].value, n.value, torch.uint32)
+    out["linestring_index"] = res.take(o[1].value, n.value, torch.uint32)
+    out["distance"] = res.take(o[2].value, n.value, torch.float64)
+    return out

@@ -384,6 +384,77 @@ QPIP_API int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
                                                     int64_t* num_hits);
 
 /*
+ * linestring_bounding_boxes
+ * Replaces cuspatial::linestring_bounding_boxes
+ *   (cpp/include/cuspatial/bounding_boxes.hpp:52-57,
+ *    cpp/src/bounding_boxes/linestring_bounding_boxes.cu:127-154;
+ *    algorithm detail/bounding_boxes.cuh:97-131).
+ * Box of each linestring's vertices, each vertex expanded by expansion_radius.
+ * Outputs minx, miny, maxx, maxy (FLOAT64, *num_boxes = num_linestring_offsets - 1).
+ * Errors: "x and y must be the same size", "expansion radius must be greater or
+ * equal than 0", "all linestrings must have at least 2 vertices".
+ */
+QPIP_API int qpip_linestring_bounding_boxes(const int32_t* linestring_offsets,
+                                            int64_t num_linestring_offsets,
+                                            const double* x,
+                                            const double* y,
+                                            int64_t num_x,
+                                            int64_t num_y,
+                                            double expansion_radius,
+                                            const qpip_resource* mr,
+                                            void* stream,
+                                            double** minx,
+                                            double** miny,
+                                            double** maxx,
+                                            double** maxy,
+                                            int64_t* num_boxes);
+
+/*
+ * quadtree_point_to_nearest_linestring
+ * Replaces cuspatial::quadtree_point_to_nearest_linestring
+ *   (cpp/include/cuspatial/spatial_join.hpp:166-175,
+ *    cpp/src/join/quadtree_point_to_nearest_linestring.cu:128-189;
+ *    algorithm detail/join/quadtree_point_to_nearest_linestring.cuh:143-314).
+ * Inputs: the (linestring_index, quad_index) pairs of
+ * qpip_join_quadtree_and_bounding_boxes over linestring boxes, the quadtree,
+ * point_indices and x, y, and the linestrings (offsets, x, y; one linestring per
+ * multilinestring). Outputs (num_point_indices rows): point_index (UINT32, the
+ * row's position in point_indices), linestring_index (UINT32) and distance
+ * (FLOAT64) of the nearest linestring among those paired with the point's leaf.
+ * Rows of points in leaves with no pair: (p, 0xffffffff, 0.0) (the reference
+ * leaves the index columns of these rows uninitialized).
+ * Errors: "number of points must be the same for both x and y columns",
+ * "numbers of vertices must be the same for both x and y columns".
+ */
+QPIP_API int qpip_quadtree_point_to_nearest_linestring(const uint32_t* linestring_index,
+                                                       const uint32_t* quad_index,
+                                                       int64_t num_pairs,
+                                                       const uint32_t* key,
+                                                       const uint8_t* level,
+                                                       const uint8_t* is_internal_node,
+                                                       const uint32_t* length,
+                                                       const uint32_t* offset,
+                                                       int64_t num_nodes,
+                                                       const uint32_t* point_indices,
+                                                       int64_t num_point_indices,
+                                                       const double* x,
+                                                       const double* y,
+                                                       int64_t num_x,
+                                                       int64_t num_y,
+                                                       const uint32_t* linestring_offsets,
+                                                       int64_t num_linestring_offsets,
+                                                       const double* linestring_x,
+                                                       const double* linestring_y,
+                                                       int64_t num_linestring_x,
+                                                       int64_t num_linestring_y,
+                                                       const qpip_resource* mr,
+                                                       void* stream,
+                                                       uint32_t** point_index,
+                                                       uint32_t** linestring_index_out,
+                                                       double** distance,
+                                                       int64_t* num_rows);
+
+/*
  * Learned candidate ranking over a coarse grid (no counterpart in the
  * reference; an addition for point location over non-overlapping polygons).
  *

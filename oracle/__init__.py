@@ -87,6 +87,11 @@ def lib():
             _c_double_p, _c_double_p, i64, i64, _c_u32_p, i64, _c_u32_p, i64,
             _c_double_p, _c_double_p, i64, i64]
         L.oq_quadtree_point_in_polygon.restype = vp
+        L.oq_quadtree_point_to_nearest_linestring.argtypes = [
+            _c_u32_p, _c_u32_p, i64, _c_u32_p, _c_u32_p, i64, _c_u32_p, i64,
+            _c_double_p, _c_double_p, i64, i64, _c_u32_p, i64,
+            _c_double_p, _c_double_p, i64, i64]
+        L.oq_quadtree_point_to_nearest_linestring.restype = vp
         _lib = L
     return _lib
 
@@ -265,3 +270,35 @@ def quadtree_point_in_polygon(pairs, tree, point_indices, x, y, poly_offsets, ri
         len(x), len(y), _ptr(po, ctypes.c_uint32), len(po), _ptr(ro, ctypes.c_uint32), len(ro),
         _ptr(vx, ctypes.c_double), _ptr(vy, ctypes.c_double), len(vx), len(vy))
     return tuple(_take(h, [np.uint32, np.uint32]))
+
+
+def linestring_bounding_boxes(linestring_offsets, x, y, expansion_radius=0.0):
+    """Box per linestring (detail/bounding_boxes.cuh:97-131): the same per-geometry
+    reduction as polygon_bounding_boxes with one ring per polygon. Returns
+    (minx, miny, maxx, maxy)."""
+    lo = _arr(linestring_offsets, np.int32)
+    n = max(len(lo) - 1, 0)
+    return polygon_bounding_boxes(np.arange(n + 1, dtype=np.int32), lo, x, y, expansion_radius)
+
+
+def quadtree_point_to_nearest_linestring(pairs, tree, point_indices, x, y, linestring_offsets,
+                                         lx, ly):
+    """pairs = (linestring_index, quad_index). Returns (point_index, linestring_index,
+    distance), one row per entry of point_indices."""
+    pl = _arr(pairs[0], np.uint32)
+    pq = _arr(pairs[1], np.uint32)
+    length = _arr(tree[3], np.uint32)
+    offset = _arr(tree[4], np.uint32)
+    pi = _arr(point_indices, np.uint32)
+    x = _arr(x, np.float64)
+    y = _arr(y, np.float64)
+    lo = _arr(linestring_offsets, np.uint32)
+    lx = _arr(lx, np.float64)
+    ly = _arr(ly, np.float64)
+    h = lib().oq_quadtree_point_to_nearest_linestring(
+        _ptr(pl, ctypes.c_uint32), _ptr(pq, ctypes.c_uint32), len(pl),
+        _ptr(length, ctypes.c_uint32), _ptr(offset, ctypes.c_uint32), len(length),
+        _ptr(pi, ctypes.c_uint32), len(pi), _ptr(x, ctypes.c_double), _ptr(y, ctypes.c_double),
+        len(x), len(y), _ptr(lo, ctypes.c_uint32), len(lo), _ptr(lx, ctypes.c_double),
+        _ptr(ly, ctypes.c_double), len(lx), len(ly))
+    return tuple(_take(h, [np.uint32, np.uint32, np.float64]))

@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -882,6 +883,139 @@ void* oq_quadtree_point_in_polygon(const uint32_t* pair_poly,
   }
   r->add(out_poly);
   r->add(out_point);
+  return r;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// quadtree_point_to_nearest_linestring
+// (detail/join/quadtree_point_to_nearest_linestring.cuh:143-314, step by step)
+// ---------------------------------------------------------------------------
+namespace {
+
+// dot(a, b) = a.x * b.x + a.y * b.y contracted to fma(a.x, b.x, a.y * b.y): the
+// reference's expected distances (quadtree_point_to_nearest_linestring_test_small.cu:
+// 173-192) are reproduced bit for bit by this form and missed by up to 45 ULP by
+// the unfused one.
+double dot2(double ax, double ay, double bx, double by) { return std::fma(ax, bx, ay * by); }
+
+// point_linestring_distance (detail/algorithm/point_linestring_distance.cuh:33-51).
+double point_linestring_distance(double px, double py, const double* lx, const double* ly,
+                                 uint32_t v0, uint32_t v1)
+{
+  double dsq = std::numeric_limits<double>::max();
+  for (uint32_t i = v0; i + 1 < v1; ++i) {
+    double const v1x = px - lx[i], v1y = py - ly[i];
+    double const v2x = px - lx[i + 1], v2y = py - ly[i + 1];
+    double const ex = lx[i + 1] - lx[i], ey = ly[i + 1] - ly[i];
+    double const d0 = dot2(v1x, v1y, v1x, v1y);
+    double const d1 = dot2(v2x, v2y, v2x, v2y);
+    double const d2 = dot2(ex, ey, ex, ey);
+    double const d3 = dot2(v1x, v1y, ex, ey);  // proj2: dot(point - v1, v2 - v1)
+    double const r  = d3 * d3 / d2;
+    double const d  = (d3 <= 0 || r >= d2) ? std::fmin(d0, d1) : d0 - r;
+    dsq             = std::fmin(dsq, d);
+  }
+  return std::sqrt(dsq);
+}
+
+struct PLD {
+  uint32_t point, linestring;
+  double distance;
+};
+
+// The reduce_by_key operator (:279-294).
+PLD nearest_op(PLD const& lhs, PLD const& rhs)
+{
+  if (lhs.distance == 0) return rhs;
+  if (rhs.distance == 0) return lhs;
+  if (lhs.distance == rhs.distance) return lhs.linestring < rhs.linestring ? lhs : rhs;
+  return lhs.distance < rhs.distance ? lhs : rhs;
+}
+
+}  // namespace
+
+extern "C" {
+
+void* oq_quadtree_point_to_nearest_linestring(const uint32_t* pair_ls,
+                                              const uint32_t* pair_quad,
+                                              int64_t k,
+                                              const uint32_t* length,
+                                              const uint32_t* offset,
+                                              int64_t m,
+                                              const uint32_t* point_indices,
+                                              int64_t n_point_indices,
+                                              const double* x,
+                                              const double* y,
+                                              int64_t nx,
+                                              int64_t ny,
+                                              const uint32_t* ls_offsets,
+                                              int64_t n_ls_offsets,
+                                              const double* lx,
+                                              const double* ly,
+                                              int64_t nlx,
+                                              int64_t nly)
+{
+  // cpp/src/join/quadtree_point_to_nearest_linestring.cu:153-176
+  if (!(n_point_indices == nx && nx == ny))
+    return fail("number of points must be the same for both x and y columns");
+  if (nlx != nly) return fail("numbers of vertices must be the same for both x and y columns");
+  auto* r = new Result;
+  std::vector<uint32_t> out_point, out_ls;
+  std::vector<double> out_dist;
+  if (!(k == 0 || m == 0 || n_point_indices == 0 || n_ls_offsets == 0)) {
+    // local point offsets: inclusive scan of the paired quadrants' lengths (:204-217)
+    std::vector<uint64_t> offs(k + 1, 0);
+    for (int64_t i = 0; i < k; ++i) offs[i + 1] = offs[i] + length[pair_quad[i]];
+    uint64_t const total = offs[k];
+    std::vector<uint32_t> quad_off(k);
+    for (int64_t i = 0; i < k; ++i) quad_off[i] = offset[pair_quad[i]];
+    // the transposed enumeration (:57-91, 106-118): within a run of pairs on one
+    // quadrant, entries go point by point, each point against the run's linestrings
+    std::vector<PLD> seq(total);
+    for (uint64_t g = 0; g < total; ++g) {
+      int64_t const pair =
+        std::upper_bound(offs.begin(), offs.end(), g) - offs.begin() - 1;  // quad-linestring index
+      uint64_t const local_point = g - offs[pair];
+      uint32_t const qo          = quad_off[pair];
+      int64_t const lhs = std::lower_bound(quad_off.begin(), quad_off.begin() + pair, qo) -
+                          quad_off.begin();
+      int64_t const rhs =
+        std::upper_bound(quad_off.begin() + pair, quad_off.end(), qo) - quad_off.begin();
+      uint64_t const local_ls   = static_cast<uint64_t>(pair - lhs);
+      uint64_t const ls_in_quad = static_cast<uint64_t>(rhs - lhs);
+      uint64_t const npts       = length[pair_quad[pair]];
+      uint64_t const start      = local_ls * npts + local_point;
+      uint32_t const point_id   = static_cast<uint32_t>(start / ls_in_quad + qo);
+      int64_t const ls_pair     = static_cast<int64_t>(start % ls_in_quad) + lhs;
+      uint32_t const ls         = pair_ls[ls_pair];
+      uint32_t const idx        = point_indices[point_id];
+      seq[g] = PLD{point_id, ls,
+                   point_linestring_distance(x[idx], y[idx], lx, ly, ls_offsets[ls],
+                                             ls_offsets[ls + 1])};
+    }
+    // reduce_by_key over equal consecutive point ids, then scatter to row point_id
+    // (:266-312); rows never written: index columns left unset by the reference,
+    // (p, 0xffffffff, 0) here
+    out_point.resize(n_point_indices);
+    out_ls.assign(n_point_indices, 0xffffffffu);
+    out_dist.assign(n_point_indices, 0.0);
+    for (int64_t p = 0; p < n_point_indices; ++p) out_point[p] = static_cast<uint32_t>(p);
+    uint64_t g = 0;
+    while (g < total) {
+      PLD acc     = seq[g];
+      uint64_t e  = g + 1;
+      while (e < total && seq[e].point == acc.point) acc = nearest_op(acc, seq[e++]);
+      out_point[acc.point] = acc.point;
+      out_ls[acc.point]    = acc.linestring;
+      out_dist[acc.point]  = acc.distance;
+      g                    = e;
+    }
+  }
+  r->add(out_point);
+  r->add(out_ls);
+  r->add(out_dist);
   return r;
 }
 

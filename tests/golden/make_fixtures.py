@@ -288,6 +288,34 @@ def contains_properly_kats(ref):
     return cases
 
 
+def nearest_linestring_small(ref):
+    """cpp/tests/join/quadtree_point_to_nearest_linestring_test_small.cu:36-209: 71
+    points in [0, 8)^2, a depth-3 quadtree (scale 1, max_size 12), 4 closed
+    linestrings with boxes expanded by 2; the expected join pairs and, per point
+    position, the nearest linestring and its distance (float64 branch)."""
+    src = "cpp/tests/join/quadtree_point_to_nearest_linestring_test_small.cu"
+    points = pairs(lines(ref, src, 48, 84))
+    verts = pairs(lines(ref, src, 96, 117))
+
+    def ints(a, b):
+        return [int(v) for v in re.findall(r"\d+", lines(ref, src, a, b).split("(", 1)[1])]
+
+    dist_text = lines(ref, src, 173, 192)
+    return {
+        "source": src + ":36-209",
+        "points": points,
+        "bbox": [0.0, 8.0, 0.0, 8.0], "scale": 1.0, "max_depth": 3, "max_size": 12,
+        "linestring_offsets": [0, 4, 10, 14, 19],
+        "vertices": verts,
+        "expansion_radius": 2.0,
+        "expected_join_linestring_index": ints(138, 139),
+        "expected_join_quad_index": ints(140, 141),
+        "expected_point_index": ints(196, 199),
+        "expected_linestring_index": ints(201, 204),
+        "expected_distance": [float(v) for v in re.findall(NUM, dist_text.split("make_device_vector<T>(", 1)[1])],
+    }
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     with open(os.path.join(HERE, "quadtree_cases.json"), "w") as f:
@@ -298,6 +326,8 @@ def main():
         json.dump(pip_kats(), f, indent=1)
     with open(os.path.join(HERE, "pairwise_pip_kats.json"), "w") as f:
         json.dump(pairwise_pip_kats(), f, indent=1)
+    with open(os.path.join(HERE, "nearest_linestring_small.json"), "w") as f:
+        json.dump(nearest_linestring_small(ref), f, indent=1)
     with open(os.path.join(HERE, "contains_properly_kats.json"), "w") as f:
         json.dump(contains_properly_kats(ref), f, indent=1)
     print("fixtures written to", HERE)

@@ -33,7 +33,7 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 23
+    assert len(names) == 25
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)
@@ -283,3 +283,39 @@ def test_ranker_preconditions(capi):
     assert L.qpip_ranker_logits(a(m), a(g), None, 0, None, None) == capi.QPIP_SUCCESS
     assert L.qpip_ranker_locate(a(m), a(g), None, 1, None, 1, None, None, 0, None, 0, 16, None,
                                 None, None, None) == capi.QPIP_SUCCESS
+
+
+def test_linestring_preconditions(capi):
+    import numpy as np
+
+    L = capi.lib()
+    outs = _outs(4)
+    n = ctypes.c_int64(-1)
+    offs = np.array([0, 2, 3], np.int32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    rc = L.qpip_linestring_bounding_boxes(p(offs), 3, None, None, 3, 2, 0.0, None, None,
+                                          *[ctypes.byref(o) for o in outs], ctypes.byref(n))
+    assert rc == capi.QPIP_ERROR_LOGIC and "x and y must be the same size" in \
+        L.qpip_last_error_message().decode()
+    rc = L.qpip_linestring_bounding_boxes(p(offs), 3, None, None, 3, 3, -0.5, None, None,
+                                          *[ctypes.byref(o) for o in outs], ctypes.byref(n))
+    assert rc == capi.QPIP_ERROR_LOGIC and "expansion radius" in \
+        L.qpip_last_error_message().decode()
+    rc = L.qpip_linestring_bounding_boxes(p(offs), 3, None, None, 3, 3, 0.0, None, None,
+                                          *[ctypes.byref(o) for o in outs], ctypes.byref(n))
+    assert rc == capi.QPIP_ERROR_LOGIC and "at least 2 vertices" in \
+        L.qpip_last_error_message().decode()
+    rc = L.qpip_linestring_bounding_boxes(None, 0, None, None, 0, 0, 0.0, None, None,
+                                          *[ctypes.byref(o) for o in outs], ctypes.byref(n))
+    assert rc == capi.QPIP_SUCCESS and n.value == 0
+    o3 = _outs(3)
+    rc = L.qpip_quadtree_point_to_nearest_linestring(
+        None, None, 1, None, None, None, None, None, 1, None, 5, None, None, 5, 4, None, 2, None,
+        None, 3, 3, None, None, *[ctypes.byref(o) for o in o3], ctypes.byref(n))
+    assert rc == capi.QPIP_ERROR_LOGIC and \
+        "number of points must be the same for both x and y columns" in \
+        L.qpip_last_error_message().decode()
+    rc = L.qpip_quadtree_point_to_nearest_linestring(
+        None, None, 0, None, None, None, None, None, 0, None, 0, None, None, 0, 0, None, 0, None,
+        None, 0, 0, None, None, *[ctypes.byref(o) for o in o3], ctypes.byref(n))
+    assert rc == capi.QPIP_SUCCESS and n.value == 0

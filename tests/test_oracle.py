@@ -380,3 +380,89 @@ def test_quadtree_pip_equals_bruteforce(oracle_mod, seed):
     else:  # polygons poking out of the bbox may miss points outside it
         assert set(got) <= set(want)
     assert len(got) > 50
+
+
+def _ulps(a, b):
+    a = np.asarray(a, np.float64).view(np.int64)
+    b = np.asarray(b, np.float64).view(np.int64)
+    return np.abs(a - b)
+
+
+def test_nearest_linestring_golden(oracle_mod):
+    """quadtree_point_to_nearest_linestring_test_small.cu: the join over expanded
+    linestring boxes and, per point, the nearest linestring and the distance,
+    both exactly (the reference test allows 4 ULP; the fma-contracted dot
+    products reproduce its values bit for bit)."""
+    g = load_golden("nearest_linestring_small.json")
+    p = np.array(g["points"], dtype=np.float64)
+    v = np.array(g["vertices"], dtype=np.float64)
+    x0, x1, y0, y1 = g["bbox"]
+    pi, key, level, internal, length, offset = oracle_mod.quadtree_on_points(
+        p[:, 0], p[:, 1], x0, x1, y0, y1, g["scale"], g["max_depth"], g["max_size"])
+    tree = (key, level, internal, length, offset)
+    bb = oracle_mod.linestring_bounding_boxes(g["linestring_offsets"], v[:, 0], v[:, 1],
+                                              g["expansion_radius"])
+    jl, jq = oracle_mod.join_quadtree_and_bounding_boxes(tree, bb, x0, x1, y0, y1, g["scale"],
+                                                         g["max_depth"])
+    assert jl.tolist() == g["expected_join_linestring_index"]
+    assert jq.tolist() == g["expected_join_quad_index"]
+    pt, ls, d = oracle_mod.quadtree_point_to_nearest_linestring(
+        (jl, jq), tree, pi, p[:, 0], p[:, 1], g["linestring_offsets"], v[:, 0], v[:, 1])
+    assert pt.tolist() == g["expected_point_index"]
+    assert ls.tolist() == g["expected_linestring_index"]
+    assert _ulps(d, g["expected_distance"]).max() == 0
+
+
+def test_nearest_linestring_bruteforce(oracle_mod):
+    """Random linestrings and points: each point's row is the nearest of the
+    linestrings paired with its leaf (brute force over those candidates, with the
+    reference's zero-distance and tie rules), by an independent numpy distance."""
+    rng = np.random.default_rng(17)
+    n, L = 3000, 40
+    x, y = rng.uniform(0, 100, n), rng.uniform(0, 100, n)
+    offs, lx, ly = [0], [], []
+    for _ in range(L):
+        k = int(rng.integers(2, 7))
+        cx, cy = rng.uniform(0, 100, 2)
+        lx += list(cx + rng.normal(0, 6, k))
+        ly += list(cy + rng.normal(0, 6, k))
+        offs.append(len(lx))
+    lx, ly = np.array(lx), np.array(ly)
+    pi, key, level, internal, length, offset = oracle_mod.quadtree_on_points(
+        x, y, 0, 100, 0, 100, 100 / 32, 5, 40)
+    tree = (key, level, internal, length, offset)
+    bb = oracle_mod.linestring_bounding_boxes(offs, lx, ly, 8.0)
+    jl, jq = oracle_mod.join_quadtree_and_bounding_boxes(tree, bb, 0, 100, 0, 100, 100 / 32, 5)
+    pt, ls, d = oracle_mod.quadtree_point_to_nearest_linestring(
+        (jl, jq), tree, pi, x, y, offs, lx, ly)
+    assert pt.tolist() == list(range(n))
+
+    def dist(px, py, j):
+        best = np.inf
+        for i in range(offs[j], offs[j + 1] - 1):
+            ax, ay, bx, by = lx[i], ly[i], lx[i + 1], ly[i + 1]
+            t = ((px - ax) * (bx - ax) + (py - ay) * (by - ay)) / ((bx - ax) ** 2 + (by - ay) ** 2)
+            t = min(max(t, 0.0), 1.0)
+            best = min(best, (px - ax - t * (bx - ax)) ** 2 + (py - ay - t * (by - ay)) ** 2)
+        return np.sqrt(best)
+
+    cand = {}
+    for l_, q in zip(jl, jq):
+        cand.setdefault(int(q), []).append(int(l_))
+    leaf_of = np.full(n, -1)
+    for q in cand:
+        leaf_of[offset[q]:offset[q] + length[q]] = q
+    covered = 0
+    for pos in range(n):
+        q = leaf_of[pos]
+        if q < 0:
+            assert ls[pos] == 0xFFFFFFFF and d[pos] == 0.0
+            continue
+        covered += 1
+        idx = pi[pos]
+        ds = [(dist(x[idx], y[idx], j), j) for j in cand[q]]
+        nz = [t for t in ds if t[0] != 0]
+        want = min(nz) if nz else ds[-1]
+        assert ls[pos] == want[1], pos
+        assert abs(d[pos] - want[0]) <= 1e-9 * max(1.0, want[0])
+    assert covered > n // 2
