@@ -98,6 +98,9 @@ def main():
                     help="labelled points the ranker is trained on (voronoi_learned)")
     ap.add_argument("--epochs", type=int, default=3, help="ranker training epochs")
     ap.add_argument("--jobs", type=int, default=4096, help="jobs of the batched workload")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched_jobs: pick the PIP segment parameters with the autotuner "
+                         "(cuspatial_amd/autotune.py) on this workload before timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the known "
                          "traffic that scales the HBM counters) and exit")
@@ -309,6 +312,13 @@ def bench_batched(args, dev):
                                                     w.job_polygon_offsets, w.bboxes, w.scales,
                                                     w.max_depths, w.max_sizes)
 
+    tuned = None
+    if args.autotune:
+        from cuspatial_amd import autotune
+
+        tuned = autotune.tune(step, repeats=1, log=lambda s: log("[bench] " + s))
+        log(f"[bench] autotune chose {tuned['config']}: pip_test {tuned['baseline_ms']:.2f} -> "
+            f"{tuned['best_ms']:.2f} ms")
     for i in range(args.warmup):
         log(f"[bench] warmup {i}")
         out = step()
@@ -385,6 +395,9 @@ def bench_batched(args, dev):
         "plain_per_job": {"value": plain_value, "unit": "points/s", "seconds": plain_s,
                           "jobs_with_equal_hit_counts": sum(same)},
         "speedup_vs_plain": value / plain_value,
+        "autotune": None if tuned is None else {
+            "config": tuned["config"], "pip_test_ms_before": tuned["baseline_ms"],
+            "pip_test_ms_after": tuned["best_ms"], "trials": len(tuned["trials"])},
         "kernels": kernels,
         "p_ms": {"min": 1e3 * min(times), "max": 1e3 * max(times)},
     }

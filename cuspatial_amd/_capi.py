@@ -53,6 +53,8 @@ EXPORTED = (
     "qpip_profile_enable",
     "qpip_profile_reset",
     "qpip_profile_report",
+    "qpip_tuning_set",
+    "qpip_tuning_get",
     "qpip_quadtree_on_points",
     "qpip_quadtree_on_points_xy",
     "qpip_polygon_bounding_boxes",
@@ -94,6 +96,10 @@ def lib():
         L.qpip_profile_enable.argtypes = [ctypes.c_int]
         L.qpip_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
         L.qpip_profile_report.restype = ctypes.c_int64
+        L.qpip_tuning_set.argtypes = [ctypes.c_char_p, _i64]
+        L.qpip_tuning_set.restype = ctypes.c_int
+        L.qpip_tuning_get.argtypes = [ctypes.c_char_p, P(_i64)]
+        L.qpip_tuning_get.restype = ctypes.c_int
         L.qpip_quadtree_on_points.argtypes = [
             _vp, _vp, _i64, _i64, _d, _d, _d, _d, _d, ctypes.c_int8, ctypes.c_int32,
             P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]

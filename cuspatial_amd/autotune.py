@@ -1,0 +1,106 @@
+"""Autotuner for the library's launch parameters.
+
+The tunables are the library's runtime tuning parameters (qpip_tuning_set /
+qpip_tuning_get in include/cuspatial_amd.h); results never depend on them, only
+the time. ``tune`` runs a caller-supplied step (one pass of the real workload)
+under each candidate value, one parameter at a time (coordinate descent from the
+current values), timing the kernels the parameter affects with the library's
+HIP-event timers, and keeps the fastest. The chosen values are applied, and can
+be saved to and loaded from a JSON file next to the workload that chose them.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import time
+
+from . import _capi
+
+# parameter -> candidates tried by default, and the timer(s) it is judged by
+PARAMETERS = {
+    "pip_min_pairs_per_wave": ([4, 8, 16, 32, 64, 256, 1024], ("pip_test",)),
+    "pip_pairs_shift": ([10, 12, 14, 16, 18, 20, 24], ("pip_test",)),
+}
+
+
+def get(name: str) -> int:
+    v = ctypes.c_int64()
+    _capi.check(_capi.lib().qpip_tuning_get(name.encode(), ctypes.byref(v)))
+    return int(v.value)
+
+
+def set(name: str, value: int) -> None:  # noqa: A001 - mirrors qpip_tuning_set
+    _capi.check(_capi.lib().qpip_tuning_set(name.encode(), int(value)))
+
+
+def current() -> dict:
+    return {k: get(k) for k in PARAMETERS}
+
+
+def apply(config: dict) -> None:
+    for k, v in config.items():
+        set(k, v)
+
+
+def _measure(step, timers, repeats):
+    import torch
+
+    step()  # warm: allocations, caches
+    torch.cuda.synchronize()
+    _capi.profile_reset()
+    _capi.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        step()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / repeats
+    _capi.profile_enable(False)
+    prof = _capi.profile_report()
+    kern = sum(prof.get(t, (0, 0.0))[1] for t in timers) / repeats
+    return kern, 1e3 * wall
+
+
+def tune(step, parameters=None, repeats: int = 2, rounds: int = 1, log=None) -> dict:
+    """Coordinate descent over `parameters` (name -> candidates; default
+    PARAMETERS). Returns {"config": best values, "trials": [...], "baseline_ms",
+    "best_ms"}; the best values are left applied."""
+    params = parameters or {k: v[0] for k, v in PARAMETERS.items()}
+    timers = sorted({t for k in params for t in PARAMETERS.get(k, ([], ("pip_test",)))[1]})
+    best = current()
+    base_kern, base_wall = _measure(step, timers, repeats)
+    best_kern = base_kern
+    trials = [{"config": dict(best), "kernel_ms": base_kern, "step_ms": base_wall}]
+    if log:
+        log(f"autotune: start {best} kernel {base_kern:.3f} ms, step {base_wall:.3f} ms")
+    for _ in range(rounds):
+        for name, cands in params.items():
+            for v in cands:
+                if v == best[name]:
+                    continue
+                cfg = dict(best, **{name: v})
+                apply(cfg)
+                kern, wall = _measure(step, timers, repeats)
+                trials.append({"config": cfg, "kernel_ms": kern, "step_ms": wall})
+                if log:
+                    log(f"autotune: {name}={v}: kernel {kern:.3f} ms, step {wall:.3f} ms")
+                if kern < best_kern:
+                    best, best_kern = cfg, kern
+            apply(best)
+    return {"config": best, "timers": timers, "baseline_ms": base_kern, "best_ms": best_kern,
+            "trials": trials}
+
+
+def save(path, result: dict, workload: str) -> None:
+    with open(path, "w") as f:
+        json.dump({"workload": workload, "config": result["config"],
+                   "baseline_ms": result["baseline_ms"], "best_ms": result["best_ms"]}, f,
+                  indent=1)
+
+
+def load(path) -> dict:
+    """Applies a saved configuration; returns it."""
+    with open(path) as f:
+        cfg = json.load(f)["config"]
+    apply(cfg)
+    return cfg

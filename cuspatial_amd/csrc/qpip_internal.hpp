@@ -58,6 +58,11 @@ struct oom_error : std::runtime_error {
 
 void set_last_error(std::string const& msg);
 
+// Runtime tuning parameters (qpip_tuning_set / qpip_tuning_get), read at launch.
+enum class Tuning : int { PipMinPairsPerWave = 0, PipPairsShift = 1 };
+constexpr int kNumTuning = 2;
+int64_t tuning(Tuning t);
+
 // ----------------------------------------------------------------------------
 // Caching device pool (library-owned temporaries and default-resource outputs).
 // ----------------------------------------------------------------------------

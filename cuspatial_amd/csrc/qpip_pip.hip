@@ -89,14 +89,16 @@ struct PipArgs {
   uint32_t pairs_per_wave;  // longest segment of a leaf's pairs one wave takes
 };
 
-// Segments of at most max(16, pairs / 65536) pairs: few pairs (a small join, or
-// leaves that overlap thousands of polygons) are spread over many waves; with
-// many pairs the GPU is full anyway and whole runs save re-gathering a leaf's
-// points per segment.
+// Segments of at most max(m, pairs >> s) pairs (m = 16, s = 16 by default; both
+// are tuning parameters): few pairs (a small join, or leaves that overlap
+// thousands of polygons) are spread over many waves; with many pairs the GPU is
+// full anyway and whole runs save re-gathering a leaf's points per segment.
 inline uint32_t pairs_per_wave(uint64_t num_pairs)
 {
-  uint64_t const g = num_pairs >> 16;
-  return static_cast<uint32_t>(g < 16 ? 16 : (g > (1u << 30) ? (1u << 30) : g));
+  uint64_t const m = static_cast<uint64_t>(tuning(Tuning::PipMinPairsPerWave));
+  uint64_t const g = num_pairs >> tuning(Tuning::PipPairsShift);
+  uint64_t const v = g < m ? m : g;
+  return static_cast<uint32_t>(v > (1u << 30) ? (1u << 30) : v);
 }
 
 // --- 3. test ------------------------------------------------------------------------

@@ -15,6 +15,7 @@
 // kernels could overwrite the block while the first stream still reads it. So
 // free blocks are kept per host thread; when a thread exits, its blocks are idle
 // and move to a shared list that any thread may draw from.
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -275,9 +276,52 @@ void OutputAllocator::deallocate(void* p, size_t bytes)
     mr_->deallocate(mr_->ctx, p, bytes, stream_);
 }
 
+// --- tuning parameters ---------------------------------------------------------
+namespace {
+struct TuningParam {
+  char const* name;
+  std::atomic<int64_t> value;
+  int64_t lo, hi;
+};
+TuningParam g_tuning[kNumTuning] = {
+  {"pip_min_pairs_per_wave", {16}, 1, int64_t{1} << 30},
+  {"pip_pairs_shift", {16}, 0, 40},
+};
+}  // namespace
+
+int64_t tuning(Tuning t) { return g_tuning[static_cast<int>(t)].value.load(std::memory_order_relaxed); }
+
 }  // namespace qpip
 
 extern "C" {
+
+int qpip_tuning_set(const char* name, int64_t value)
+{
+  for (auto& p : qpip::g_tuning) {
+    if (name && strcmp(name, p.name) == 0) {
+      if (value < p.lo || value > p.hi) {
+        qpip::set_last_error(std::string("tuning value out of range for ") + p.name);
+        return QPIP_ERROR_LOGIC;
+      }
+      p.value.store(value, std::memory_order_relaxed);
+      return QPIP_SUCCESS;
+    }
+  }
+  qpip::set_last_error(std::string("unknown tuning parameter: ") + (name ? name : "(null)"));
+  return QPIP_ERROR_LOGIC;
+}
+
+int qpip_tuning_get(const char* name, int64_t* value)
+{
+  for (auto& p : qpip::g_tuning) {
+    if (name && strcmp(name, p.name) == 0) {
+      if (value) *value = p.value.load(std::memory_order_relaxed);
+      return QPIP_SUCCESS;
+    }
+  }
+  qpip::set_last_error(std::string("unknown tuning parameter: ") + (name ? name : "(null)"));
+  return QPIP_ERROR_LOGIC;
+}
 
 const char* qpip_last_error_message(void) { return qpip::last_error().c_str(); }
 

@@ -67,6 +67,17 @@ QPIP_API void qpip_set_memory_limit(size_t bytes);
 /* Returns cached, unused pool memory to the device. */
 QPIP_API void qpip_trim(void);
 
+/* Tuning parameters (no counterpart in the reference; set by the autotuner,
+ * cuspatial_amd/autotune.py). Process-wide, read when a kernel is launched:
+ *   pip_min_pairs_per_wave  shortest segment of a leaf's pairs that one wave of
+ *                           the point-in-polygon test takes (default 16)
+ *   pip_pairs_shift         segments are at most max(min, pairs >> shift) pairs
+ *                           long (default 16)
+ * Results do not depend on them, only the time. Errors: unknown name, value out
+ * of range. */
+QPIP_API int qpip_tuning_set(const char* name, int64_t value);
+QPIP_API int qpip_tuning_get(const char* name, int64_t* value);
+
 /* Kernel timing (no counterpart in the reference; for benchmarks). While
  * enabled, every kernel launch is bracketed by HIP events on its stream.
  * qpip_profile_report writes {"kernel name": [launches, total ms], ...} as JSON

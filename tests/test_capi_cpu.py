@@ -33,7 +33,7 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 25
+    assert len(names) == 27
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)
@@ -319,3 +319,19 @@ def test_linestring_preconditions(capi):
         None, None, 0, None, None, None, None, None, 0, None, 0, None, None, 0, 0, None, 0, None,
         None, 0, 0, None, None, *[ctypes.byref(o) for o in o3], ctypes.byref(n))
     assert rc == capi.QPIP_SUCCESS and n.value == 0
+
+
+def test_tuning_parameters(capi):
+    """The autotuner's knobs: defaults, range checks, unknown names (host only)."""
+    from cuspatial_amd import autotune
+
+    assert autotune.get("pip_min_pairs_per_wave") == 16
+    assert autotune.get("pip_pairs_shift") == 16
+    autotune.set("pip_min_pairs_per_wave", 64)
+    assert autotune.current()["pip_min_pairs_per_wave"] == 64
+    autotune.apply({"pip_min_pairs_per_wave": 16})
+    with pytest.raises(RuntimeError, match="out of range"):
+        autotune.set("pip_pairs_shift", 99)
+    with pytest.raises(RuntimeError, match="unknown tuning parameter"):
+        autotune.set("no_such_knob", 1)
+    assert autotune.get("pip_pairs_shift") == 16
