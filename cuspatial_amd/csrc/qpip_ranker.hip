@@ -11,7 +11,8 @@
 //     h1 = relu(W1 x + b1)  (128)    h2 = relu(W2 h1 + b2)  (128)
 //     logits = W3 h2 + b3   (16, one per candidate slot)
 //
-// bf16 operands, fp32 accumulation, on MFMA (v_mfma_f32_16x16x16_bf16). The
+// bf16 operands, fp32 accumulation, on MFMA (v_mfma_f32_16x16x32_bf16 for the
+// locator, v_mfma_f32_16x16x16_bf16 in the training GEMM). The
 // locator tests the candidates in the order of their logits with the exact FP64
 // point-in-polygon test of the quadtree path (same edge table, same arithmetic)
 // and stops at the first polygon that contains the point away from its edges.
@@ -299,10 +300,12 @@ __global__ void __launch_bounds__(256) fill_f32_kernel(float* __restrict__ p, in
 // Locator: MLP on MFMA + exact test in predicted order
 // ---------------------------------------------------------------------------------
 constexpr int kLocTiles   = 4;  // 16-point tiles per wave round: 64 points, one per lane
-constexpr int kW1Frags    = (kHid / 16) * (kIn / 16);    // 48
-constexpr int kW2Frags    = (kHid / 16) * (kHid / 16);   // 64
-constexpr int kW3Frags    = (kCand / 16) * (kHid / 16);  // 8
+constexpr int kKT         = 32;  // K per v_mfma_f32_16x16x32_bf16
+constexpr int kW1Frags    = (kHid / 16) * (kIn / kKT);    // 24
+constexpr int kW2Frags    = (kHid / 16) * (kHid / kKT);   // 32
+constexpr int kW3Frags    = (kCand / 16) * (kHid / kKT);  // 4
 constexpr int kFrags      = kW1Frags + kW2Frags + kW3Frags;
+constexpr int kFragElems  = 8;  // bf16 per lane per fragment
 constexpr int kLogitPitch = kCand + 1;  // LDS row pitch of the logit transpose
 constexpr int kLocWaves   = 4;
 
@@ -411,35 +414,60 @@ __device__ __forceinline__ void exact_test(PolyView const& P, uint32_t poly, dou
   near_out   = near;
 }
 
-__device__ __forceinline__ bf16x4 load_frag(uint16_t const* lds, int frag, int lane)
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// D = A B + C on one 16x16x32 bf16 tile (gfx950). Lane l supplies
+// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16] (j = 0..7) and holds
+// D[4 (l / 16) + j][l % 16] (j = 0..3).
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
 {
-  return *reinterpret_cast<bf16x4 const*>(lds + (static_cast<int64_t>(frag) * kWave + lane) * 4);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
+
+__device__ __forceinline__ bf16x8 load_frag(uint16_t const* lds, int frag, int lane)
+{
+  return *reinterpret_cast<bf16x8 const*>(lds +
+                                          (static_cast<int64_t>(frag) * kWave + lane) * kFragElems);
+}
+
+// Input column of k-slot j of K tile c for lane group g of a layer fed by the
+// previous layer's output tiles: the B operand of that tile is output tiles 2c
+// (slots 0..3: rows 4g + j) and 2c + 1 (slots 4..7: rows 16 + 4g + j - 4), so the
+// weights of slot j sit in that column. The first layer's input is in natural
+// order (32c + 8g + j).
+__host__ __device__ __forceinline__ int chained_col(int c, int g, int j)
+{
+  return 32 * c + (j < 4 ? 4 * g + j : 16 + 4 * g + (j - 4));
+}
+
+__device__ __forceinline__ __bf16 bf(float f) { return static_cast<__bf16>(f); }
 
 __global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
 {
-  __shared__ __attribute__((aligned(16))) uint16_t w_lds[kFrags * kWave * 4];
+  __shared__ __attribute__((aligned(16))) uint16_t w_lds[kFrags * kWave * kFragElems];
   __shared__ float b_lds[kHid + kHid + kCand];
   __shared__ float logit_lds[kLocWaves][kWave * kLogitPitch];
 
-  // Stage W1, W2, W3 in fragment order: fragment f, lane l, element j holds
-  // W[16 mt + l % 16][16 kt + 4 (l / 16) + j].
+  // Stage W1, W2, W3 in fragment order: fragment f (M tile mt, K tile c), lane l,
+  // element j holds W[16 mt + l % 16][column of slot j of lane group l / 16].
   for (int t = threadIdx.x; t < kFrags * kWave; t += blockDim.x) {
-    int const f = t / kWave, l = t % kWave;
+    int const f = t / kWave, l = t % kWave, g = l >> 4;
     uint16_t const* W;
-    int in, mt, kt;
+    int in, mt, c;
+    bool chained;
     if (f < kW1Frags) {
-      W = a.pb + a.lay.w1, in = kIn, mt = f / (kIn / 16), kt = f % (kIn / 16);
+      W = a.pb + a.lay.w1, in = kIn, mt = f / (kIn / kKT), c = f % (kIn / kKT), chained = false;
     } else if (f < kW1Frags + kW2Frags) {
       int const ff = f - kW1Frags;
-      W = a.pb + a.lay.w2, in = kHid, mt = ff / (kHid / 16), kt = ff % (kHid / 16);
+      W = a.pb + a.lay.w2, in = kHid, mt = ff / (kHid / kKT), c = ff % (kHid / kKT), chained = true;
     } else {
       int const ff = f - kW1Frags - kW2Frags;
-      W = a.pb + a.lay.w3, in = kHid, mt = ff / (kHid / 16), kt = ff % (kHid / 16);
+      W = a.pb + a.lay.w3, in = kHid, mt = ff / (kHid / kKT), c = ff % (kHid / kKT), chained = true;
     }
-    uint16_t const* src = W + static_cast<int64_t>(16 * mt + (l & 15)) * in + 16 * kt + 4 * (l >> 4);
+    uint16_t const* row = W + static_cast<int64_t>(16 * mt + (l & 15)) * in;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) w_lds[t * 4 + j] = src[j];
+    for (int j = 0; j < kFragElems; ++j)
+      w_lds[t * kFragElems + j] = row[chained ? chained_col(c, g, j) : kKT * c + 8 * g + j];
   }
   for (int t = threadIdx.x; t < kHid + kHid + kCand; t += blockDim.x)
     b_lds[t] = t < kHid ? a.p[a.lay.b1 + t]
@@ -451,13 +479,13 @@ __global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
   float* lg   = logit_lds[wave];
   int64_t const waves = static_cast<int64_t>(gridDim.x) * kLocWaves;
   uint16_t const* W1s = w_lds;
-  uint16_t const* W2s = w_lds + kW1Frags * kWave * 4;
-  uint16_t const* W3s = w_lds + (kW1Frags + kW2Frags) * kWave * 4;
+  uint16_t const* W2s = w_lds + kW1Frags * kWave * kFragElems;
+  uint16_t const* W3s = w_lds + (kW1Frags + kW2Frags) * kWave * kFragElems;
 
   for (int64_t base = (static_cast<int64_t>(blockIdx.x) * kLocWaves + wave) * kWave; base < a.n;
        base += waves * kWave) {
     uint32_t cell[kLocTiles];
-    bf16x4 X[kLocTiles][kIn / 16];
+    bf16x8 X[kLocTiles][kIn / kKT];
 #pragma unroll
     for (int t = 0; t < kLocTiles; ++t) {
       int64_t const p = base + 16 * t + r;
@@ -466,54 +494,64 @@ __global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
       cell[t]         = L.cell;
       uint16_t const* e = a.pb + a.lay.e + static_cast<int64_t>(L.cell) * kEmb;
 #pragma unroll
-      for (int c = 0; c < kEmb / 16; ++c) X[t][c] = *reinterpret_cast<bf16x4 const*>(e + 16 * c + 4 * g);
+      for (int c = 0; c < kEmb / kKT; ++c)
+        X[t][c] = *reinterpret_cast<bf16x8 const*>(e + kKT * c + 8 * g);
+      // features 64 + 8g + j: frequencies 2g (j < 4) and 2g + 1 (j >= 4)
+      float f0[4], f1[4];
+      fourier4(L.u, L.v, 2 * g, f0);
+      fourier4(L.u, L.v, 2 * g + 1, f1);
 #pragma unroll
-      for (int c = 0; c < kFour / 16; ++c) {
-        float f[4];
-        fourier4(L.u, L.v, 4 * c + g, f);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) X[t][kEmb / 16 + c][j] = static_cast<short>(to_bf16(f[j]));
+      for (int j = 0; j < 4; ++j) {
+        X[t][kEmb / kKT][j]     = bf(f0[j]);
+        X[t][kEmb / kKT][j + 4] = bf(f1[j]);
       }
     }
-    // layer 1: h1 = relu(W1 x + b1); output tile mt is layer 2's K tile mt
-    bf16x4 H[kLocTiles][kHid / 16];
+    // layer 1: output tile mt holds rows 16 mt + 4g + j; tiles 2c and 2c + 1 form
+    // layer 2's K tile c (chained_col)
+    f32x4 H[kLocTiles][kHid / 16];
 #pragma unroll
     for (int mt = 0; mt < kHid / 16; ++mt) {
-      f32x4 acc[kLocTiles];
 #pragma unroll
       for (int t = 0; t < kLocTiles; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[t][j] = b_lds[16 * mt + 4 * g + j];
+        for (int j = 0; j < 4; ++j) H[t][mt][j] = b_lds[16 * mt + 4 * g + j];
 #pragma unroll
-      for (int c = 0; c < kIn / 16; ++c) {
-        bf16x4 const w = load_frag(W1s, mt * (kIn / 16) + c, l);
+      for (int c = 0; c < kIn / kKT; ++c) {
+        bf16x8 const w = load_frag(W1s, mt * (kIn / kKT) + c, l);
 #pragma unroll
-        for (int t = 0; t < kLocTiles; ++t) acc[t] = mfma16(w, X[t][c], acc[t]);
+        for (int t = 0; t < kLocTiles; ++t) H[t][mt] = mfma32(w, X[t][c], H[t][mt]);
       }
-#pragma unroll
-      for (int t = 0; t < kLocTiles; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) H[t][mt][j] = static_cast<short>(to_bf16(fmaxf(acc[t][j], 0.f)));
     }
+    auto chain = [&](f32x4 const (&h)[kHid / 16], int c) {
+      bf16x8 b;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        b[j]     = bf(fmaxf(h[2 * c][j], 0.f));
+        b[j + 4] = bf(fmaxf(h[2 * c + 1][j], 0.f));
+      }
+      return b;
+    };
     // layer 2
-    bf16x4 H2[kLocTiles][kHid / 16];
-#pragma unroll
-    for (int mt = 0; mt < kHid / 16; ++mt) {
-      f32x4 acc[kLocTiles];
+    f32x4 H2[kLocTiles][kHid / 16];
+    {
+      bf16x8 B1[kLocTiles][kHid / kKT];
 #pragma unroll
       for (int t = 0; t < kLocTiles; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[t][j] = b_lds[kHid + 16 * mt + 4 * g + j];
+        for (int c = 0; c < kHid / kKT; ++c) B1[t][c] = chain(H[t], c);
 #pragma unroll
-      for (int c = 0; c < kHid / 16; ++c) {
-        bf16x4 const w = load_frag(W2s, mt * (kHid / 16) + c, l);
+      for (int mt = 0; mt < kHid / 16; ++mt) {
 #pragma unroll
-        for (int t = 0; t < kLocTiles; ++t) acc[t] = mfma16(w, H[t][c], acc[t]);
+        for (int t = 0; t < kLocTiles; ++t)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) H2[t][mt][j] = b_lds[kHid + 16 * mt + 4 * g + j];
+#pragma unroll
+        for (int c = 0; c < kHid / kKT; ++c) {
+          bf16x8 const w = load_frag(W2s, mt * (kHid / kKT) + c, l);
+#pragma unroll
+          for (int t = 0; t < kLocTiles; ++t) H2[t][mt] = mfma32(w, B1[t][c], H2[t][mt]);
+        }
       }
-#pragma unroll
-      for (int t = 0; t < kLocTiles; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) H2[t][mt][j] = static_cast<short>(to_bf16(fmaxf(acc[t][j], 0.f)));
     }
     // layer 3: logits, then transpose through LDS (lane l -> point base + l)
     {
@@ -523,10 +561,10 @@ __global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[t][j] = b_lds[2 * kHid + 4 * g + j];
 #pragma unroll
-      for (int c = 0; c < kHid / 16; ++c) {
-        bf16x4 const w = load_frag(W3s, c, l);
+      for (int c = 0; c < kHid / kKT; ++c) {
+        bf16x8 const w = load_frag(W3s, c, l);
 #pragma unroll
-        for (int t = 0; t < kLocTiles; ++t) acc[t] = mfma16(w, H2[t][c], acc[t]);
+        for (int t = 0; t < kLocTiles; ++t) acc[t] = mfma32(w, chain(H2[t], c), acc[t]);
       }
 #pragma unroll
       for (int t = 0; t < kLocTiles; ++t)

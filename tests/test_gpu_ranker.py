@@ -225,3 +225,31 @@ def test_locator_partition_with_holes(env):
     res = loc.locate(cs.Points.from_points_xy(xy))
     poly, pt = res.pairs()
     assert torch.equal(poly, want_poly) and torch.equal(pt, want_pt)
+
+
+def test_fused_ranking_matches_gemm_logits(env):
+    """The locator's fused MLP (rank_kernel, 16x16x32 MFMA, weights chained in
+    registers) ranks like the training path's logits (qpip_ranker_logits, checked
+    against the oracle above): its first prediction is the argmax of those logits
+    over the cell's candidates, up to near-ties (accumulation order differs)."""
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 100000, 400, 15, 32)
+    model = ranker.Ranker(grid.num_cells, seed=16)
+    z = model.logits(grid, xy)
+    c = grid.cells_of(xy)
+    cnt = grid.counts[c].to(torch.int64)
+    mask = torch.arange(16, device="cuda")[None, :] < cnt[:, None]
+    zm = torch.where(mask, z, torch.full_like(z, -float("inf")))
+    top = zm.argmax(1)
+    srt = zm.sort(1, descending=True).values
+    clear = (srt[:, 0] - srt[:, 1]) > 1e-3 * srt[:, 0].abs().clamp(min=1.0)  # no near-tie
+    top_poly = grid.candidates.view(-1, 16)[c, top].to(torch.int64)
+    want_poly, want_pt = exact_pairs(env, P, xy)
+    per_point = ranker.per_point_polygon(want_poly, want_pt, len(x)).to(torch.int64)
+    loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+    res = loc.locate(cs.Points.from_points_xy(xy), max_tests=1)
+    first_right = (res.flags & 2) == 0
+    gemm_right = top_poly == per_point
+    sel = clear & (per_point >= 0) & (cnt > 1) & (cnt <= 16)
+    agree = (first_right[sel] == gemm_right[sel]).to(torch.float64).mean().item()
+    assert sel.sum().item() > 50000 and agree > 0.999, agree
