@@ -60,7 +60,31 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cpp_tests(verbose)
     return LIB
+
+
+CPP_TESTS = os.path.join(os.path.dirname(HERE), "tests", "cpp")
+
+
+def build_cpp_tests(verbose: bool = False) -> None:
+    """Compiles the C++ interface's test driver(s) under tests/cpp (host code
+    linked against the library) into tests/cpp/_build."""
+    srcs = sorted(glob.glob(os.path.join(CPP_TESTS, "*.cpp")))
+    if not srcs:
+        return
+    out_dir = os.path.join(CPP_TESTS, "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    for src in srcs:
+        exe = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
+        cmd = [HIPCC, "-O2", "-std=c++17", src, "-o", exe + ".tmp", f"-L{HERE}", "-lcuspatial_amd",
+               "-Wl,-rpath,$ORIGIN/" + os.path.relpath(HERE, out_dir)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-8000:]}")
+        os.replace(exe + ".tmp", exe)
+        if verbose:
+            print("built", exe)
 
 
 if __name__ == "__main__":
