@@ -6,7 +6,9 @@ quadtree_on_points -> polygon_bounding_boxes -> join_quadtree_and_bounding_boxes
 the C ABI in include/cuspatial_amd.h.
 """
 
+from ._capi import CapacityError  # noqa: F401
 from .spatial import (  # noqa: F401
+    BatchPlan,
     BatchResult,
     Linestrings,
     Points,
@@ -41,7 +43,9 @@ __all__ = [
     "point_in_polygon_bitmask",
     "pairwise_point_in_polygon",
     "batched_quadtree_point_in_polygon",
+    "BatchPlan",
     "BatchResult",
+    "CapacityError",
     "contains_properly",
     "polygon_contains_properly",
     "CandidateGrid",

@@ -27,6 +27,11 @@ QPIP_SUCCESS = 0
 QPIP_ERROR_LOGIC = 1
 QPIP_ERROR_DEVICE = 2
 QPIP_ERROR_OOM = 3
+QPIP_ERROR_CAPACITY = 4
+
+
+class CapacityError(RuntimeError):
+    """A launch of a captured plan outgrew a capacity fixed when the plan was made."""
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -64,6 +69,10 @@ EXPORTED = (
     "qpip_point_in_polygon",
     "qpip_pairwise_point_in_polygon",
     "qpip_batched_quadtree_point_in_polygon",
+    "qpip_batch_plan_create",
+    "qpip_batch_plan_launch",
+    "qpip_batch_plan_result",
+    "qpip_batch_plan_destroy",
     "qpip_linestring_bounding_boxes",
     "qpip_quadtree_point_to_nearest_linestring",
     "qpip_ranker_num_params",
@@ -127,6 +136,13 @@ def lib():
         L.qpip_batched_quadtree_point_in_polygon.argtypes = [
             _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
             _i64, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), _vp, P(_i64)]
+        L.qpip_batch_plan_create.argtypes = [
+            _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
+            _i64, _d, _vp, P(_vp)]
+        L.qpip_batch_plan_launch.argtypes = [_vp, _vp]
+        L.qpip_batch_plan_result.argtypes = [_vp, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp),
+                                             _vp, P(_i64)]
+        L.qpip_batch_plan_destroy.argtypes = [_vp]
         L.qpip_linestring_bounding_boxes.argtypes = [
             _vp, _i64, _vp, _vp, _i64, _i64, _d, P(qpip_resource), _vp,
             P(_vp), P(_vp), P(_vp), P(_vp), P(_i64)]
@@ -147,7 +163,8 @@ def lib():
             "qpip_linestring_bounding_boxes", "qpip_quadtree_point_to_nearest_linestring",
             "qpip_ranker_features", "qpip_ranker_logits", "qpip_ranker_gradients",
             "qpip_ranker_adam", "qpip_ranker_locate",
-            "qpip_batched_quadtree_point_in_polygon",
+            "qpip_batched_quadtree_point_in_polygon", "qpip_batch_plan_create",
+            "qpip_batch_plan_launch", "qpip_batch_plan_result", "qpip_batch_plan_destroy",
             "qpip_point_in_polygon", "qpip_pairwise_point_in_polygon",
             "qpip_quadtree_on_points", "qpip_quadtree_on_points_xy", "qpip_polygon_bounding_boxes",
             "qpip_join_quadtree_and_bounding_boxes", "qpip_quadtree_point_in_polygon",
@@ -187,6 +204,8 @@ def check(rc: int) -> None:
     msg = lib().qpip_last_error_message().decode()
     if rc == QPIP_ERROR_OOM:
         raise MemoryError(msg)
+    if rc == QPIP_ERROR_CAPACITY:
+        raise CapacityError(msg)
     raise RuntimeError(msg)
 
 

@@ -22,10 +22,12 @@
 //   * hits and point_indices are translated back to job-local indices.
 // A group's jobs never interact: every search is confined to its job's key range.
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "qpip_box.hpp"
 #include "qpip_join.hpp"
+#include "qpip_pip.hpp"
 #include "qpip_sort.hpp"
 #include "qpip_tree.hpp"
 
@@ -277,10 +279,15 @@ __global__ void __launch_bounds__(256)
                         uint32_t const* __restrict__ njob, uint32_t const* __restrict__ cfirst,
                         uint32_t const* __restrict__ ccount, JobDev const* __restrict__ jobs,
                         JobTree const* __restrict__ trees, uint64_t m,
+                        ForestMeta const* __restrict__ meta, int levels,
                         uint8_t* __restrict__ internal_out, uint32_t* __restrict__ length_out,
                         uint32_t* __restrict__ offset_out)
 {
   uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (meta) {  // node count on the device: the sum of the level sizes
+    m = 0;
+    for (int l = 0; l < levels; ++l) m += meta->sizes[l];
+  }
   if (i >= m) return;
   uint32_t const j     = njob[i];
   JobDev const& J      = jobs[j];
@@ -566,6 +573,123 @@ int bits_for(uint64_t v)  // bits needed for values 0..v
   return b;
 }
 
+// Device descriptors of jobs [j0, j1), indices relative to the group's first
+// point and polygon.
+std::vector<JobDev> job_descs(std::vector<HostJob> const& hj, int64_t j0, int64_t j1,
+                              int64_t const* jpo, int64_t const* jpoly)
+{
+  std::vector<JobDev> hd(static_cast<size_t>(j1 - j0));
+  for (int64_t k = 0; k < j1 - j0; ++k) {
+    HostJob const& h = hj[j0 + k];
+    JobDev& d        = hd[k];
+    d.min_x = h.min_x, d.min_y = h.min_y, d.max_x = h.max_x, d.max_y = h.max_y;
+    d.scale       = h.scale;
+    d.point_begin = static_cast<uint32_t>(jpo[j0 + k] - jpo[j0]);
+    d.point_end   = static_cast<uint32_t>(jpo[j0 + k + 1] - jpo[j0]);
+    d.poly_begin  = static_cast<uint32_t>(jpoly[j0 + k] - jpoly[j0]);
+    d.poly_end    = static_cast<uint32_t>(jpoly[j0 + k + 1] - jpoly[j0]);
+    d.outside_key = static_cast<uint32_t>((1u << (2 * h.d)) - 1u);
+    d.d           = h.d;
+    d.max_size    = static_cast<uint32_t>(h.max_size);
+    d.shift0      = h.d >= 1 ? 2 * (h.d - 1) : 0;
+  }
+  return hd;
+}
+
+// Upper bound of the forest's node count for jobs [j0, j1): 64 level-0 nodes per
+// job, then per level at most 4 children per internal node of the level above
+// (at most points / (max_size + 1) of them) and at most 4^level nodes.
+uint64_t forest_capacity(std::vector<HostJob> const& hj, int64_t j0, int64_t j1,
+                         int64_t const* jpo)
+{
+  uint64_t cap = 64ull * static_cast<uint64_t>(j1 - j0);
+  for (int64_t j = j0; j < j1; ++j) {
+    uint64_t const nj = static_cast<uint64_t>(jpo[j + 1] - jpo[j]);
+    uint64_t const ib = nj / (static_cast<uint64_t>(hj[j].max_size) + 1);
+    for (int l = 1; l < std::max(hj[j].d, 1); ++l) {
+      uint64_t c = std::min<uint64_t>(nj, 4 * ib);
+      if (2 * l + 6 < 62) c = std::min<uint64_t>(c, uint64_t{64} << (2 * l));
+      cap += c;
+    }
+  }
+  QPIP_EXPECTS(cap < (uint64_t{1} << 32), "batched forest too large");
+  return cap;
+}
+
+// Groups of consecutive jobs whose job number and key fit 32 bits, < 2^31 points.
+std::vector<std::pair<int64_t, int64_t>> job_groups(std::vector<HostJob> const& hj,
+                                                    int64_t const* jpo)
+{
+  std::vector<std::pair<int64_t, int64_t>> out;
+  int64_t const num_jobs = static_cast<int64_t>(hj.size());
+  int64_t j0             = 0;
+  while (j0 < num_jobs) {
+    int64_t j1 = j0 + 1;
+    int kb     = hj[j0].key_bits;
+    while (j1 < num_jobs) {
+      int const kb2 = std::max(kb, hj[j1].key_bits);
+      if (kb2 + bits_for(static_cast<uint64_t>(j1 - j0)) > 32) break;
+      if (jpo[j1 + 1] - jpo[j0] >= (int64_t{1} << 31)) break;
+      kb = kb2;
+      ++j1;
+    }
+    QPIP_EXPECTS(jpo[j1] - jpo[j0] < (int64_t{1} << 31), "a job must have fewer than 2^31 points");
+    out.emplace_back(j0, j1);
+    j0 = j1;
+  }
+  return out;
+}
+
+// Validates the job descriptions and applies the reference's clamps
+// (indexing.py:165-189, detail/point_quadtree.cuh:259-268,
+// quadtree_bbox_filtering.cu:102-106 for jobs that reach the join).
+std::vector<HostJob> prepare_jobs(int64_t num_jobs, int64_t const* job_point_offsets,
+                                  int64_t const* job_polygon_offsets, double const* job_bbox,
+                                  double const* job_scale, int8_t const* job_max_depth,
+                                  int32_t const* job_max_size, int64_t num_points,
+                                  int64_t num_poly_offsets, int64_t num_poly_x,
+                                  int64_t num_poly_y)
+{
+  QPIP_EXPECTS(job_point_offsets && job_polygon_offsets && job_bbox && job_scale &&
+                 job_max_depth && job_max_size,
+               "job descriptions must not be null");
+  QPIP_EXPECTS(num_poly_x == num_poly_y,
+               "numbers of vertices must be the same for both x and y columns");
+  QPIP_EXPECTS(job_point_offsets[0] == 0 && job_point_offsets[num_jobs] == num_points,
+               "job point offsets must start at 0 and end at the number of points");
+  int64_t const num_polys = num_poly_offsets > 0 ? num_poly_offsets - 1 : 0;
+  QPIP_EXPECTS(job_polygon_offsets[0] == 0 && job_polygon_offsets[num_jobs] == num_polys,
+               "job polygon offsets must start at 0 and end at the number of polygons");
+  std::vector<HostJob> hj(num_jobs);
+  for (int64_t j = 0; j < num_jobs; ++j) {
+    QPIP_EXPECTS(job_point_offsets[j + 1] >= job_point_offsets[j] &&
+                   job_polygon_offsets[j + 1] >= job_polygon_offsets[j],
+                 "job offsets must not decrease");
+    HostJob& h      = hj[j];
+    double const x0 = job_bbox[4 * j], x1 = job_bbox[4 * j + 1];
+    double const y0 = job_bbox[4 * j + 2], y1 = job_bbox[4 * j + 3];
+    h.min_x    = std::min(x0, x1);
+    h.max_x    = std::max(x0, x1);
+    h.min_y    = std::min(y0, y1);
+    h.max_y    = std::max(y0, y1);
+    h.max_size = std::max(1, job_max_size[j]);
+    h.d        = std::max(0, std::min(15, static_cast<int>(job_max_depth[j])));
+    h.scale    = std::max(job_scale[j],
+                       std::max(h.max_x - h.min_x, h.max_y - h.min_y) / ((1 << h.d) + 2));
+    if (job_point_offsets[j + 1] > job_point_offsets[j] &&
+        job_polygon_offsets[j + 1] > job_polygon_offsets[j]) {
+      QPIP_EXPECTS(h.scale > 0, "scale must be positive");
+      QPIP_EXPECTS(h.min_x < h.max_x && h.min_y < h.max_y,
+                   "invalid bounding box (x_min, x_max, y_min, y_max)");
+      QPIP_EXPECTS(h.d > 0 && h.d < 16, "maximum depth must be positive and less than 16");
+    }
+    int cbits = 0;
+    while ((uint64_t{1} << cbits) <= (uint64_t{1} << h.d) + 2) ++cbits;
+    h.key_bits = std::min(32, 2 * cbits);
+  }
+  return hj;
+}
+
 struct GroupResult {
   dbuf<uint32_t> poly, point;  // job-local hits of the group
   uint64_t hits = 0;
@@ -592,21 +716,7 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   int const job_bits = bits_for(J - 1);
   int const sort_bits = std::min(32, key_bits + job_bits);
 
-  std::vector<JobDev> hd(J);
-  for (uint32_t k = 0; k < J; ++k) {
-    HostJob const& h = hj[j0 + k];
-    JobDev& d        = hd[k];
-    d.min_x = h.min_x, d.min_y = h.min_y, d.max_x = h.max_x, d.max_y = h.max_y;
-    d.scale       = h.scale;
-    d.point_begin = static_cast<uint32_t>(jpo[j0 + k] - P0);
-    d.point_end   = static_cast<uint32_t>(jpo[j0 + k + 1] - P0);
-    d.poly_begin  = static_cast<uint32_t>(jpoly[j0 + k] - PO0);
-    d.poly_end    = static_cast<uint32_t>(jpoly[j0 + k + 1] - PO0);
-    d.outside_key = static_cast<uint32_t>((1u << (2 * h.d)) - 1u);
-    d.d           = h.d;
-    d.max_size    = static_cast<uint32_t>(h.max_size);
-    d.shift0      = h.d >= 1 ? 2 * (h.d - 1) : 0;
-  }
+  std::vector<JobDev> hd = job_descs(hj, j0, j1, jpo, jpoly);
   dbuf<JobDev> jobs(J);
   QPIP_HIP(hipMemcpyAsync(jobs.get(), hd.data(), J * sizeof(JobDev), hipMemcpyHostToDevice, stream));
   QPIP_HIP(hipStreamSynchronize(stream));  // hd is pageable host memory
@@ -629,21 +739,7 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   vals_tmp.release();
 
   // 3. forest levels
-  uint64_t cap = 64ull * J;
-  {
-    uint64_t per = 0;
-    for (int64_t j = j0; j < j1; ++j) {
-      uint64_t const nj = static_cast<uint64_t>(jpo[j + 1] - jpo[j]);
-      uint64_t const ib = nj / (static_cast<uint64_t>(hj[j].max_size) + 1);
-      for (int l = 1; l < std::max(hj[j].d, 1); ++l) {
-        uint64_t c = std::min<uint64_t>(nj, 4 * ib);
-        if (2 * l + 6 < 62) c = std::min<uint64_t>(c, uint64_t{64} << (2 * l));
-        per += c;
-      }
-    }
-    cap += per;
-  }
-  QPIP_EXPECTS(cap < (uint64_t{1} << 32), "batched forest too large");
+  uint64_t const cap = forest_capacity(hj, j0, j1, jpo);
   dbuf<uint32_t> nkey(cap), nstart(cap), ncount(cap), cfirst(cap), ccount(cap), njob(cap);
   dbuf<uint8_t> nlevel(cap);
   dbuf<uint4> split(cap), staging(static_cast<size_t>(J) * 64);
@@ -687,7 +783,7 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   if (M > 0) {
     batch_finalize_kernel<<<grid_for(M, 256), 256, 0, stream>>>(
       nkey.get(), nstart.get(), ncount.get(), nlevel.get(), njob.get(), cfirst.get(), ccount.get(),
-      jobs.get(), trees.get(), M, f_internal.get(), f_length.get(), f_offset.get());
+      jobs.get(), trees.get(), M, nullptr, 0, f_internal.get(), f_length.get(), f_offset.get());
     QPIP_LAUNCH_CHECK();
   }
   // point_indices (job-local)
@@ -826,6 +922,471 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   for (uint32_t k = 0; k < J; ++k) job_hits[j0 + k] = (k + 1 < J ? hf[k + 1] : h) - hf[k];
 }
 
+// --- captured plan: the same pipeline with every size on the device ----------------
+// A plan fixes the jobs, the polygons and the point buffer's address; each launch
+// replays one HIP graph of the whole pipeline over whatever points the buffer then
+// holds. The polygon boxes and edge table depend on nothing else and are built once
+// at creation. Data-dependent sizes (forest nodes, leaf pairs, PIP chunks, hits)
+// stay on the device: every kernel after them runs on a grid sized by a capacity
+// and reads the actual count, every write is bounded by its capacity, and a count
+// past its capacity sets a flag that the result call reports. Capacities come from
+// sizing runs over the points in the buffer at creation, times a headroom factor.
+
+// Counts of a group whose join is not run (no points or no polygons): no hits.
+__global__ void plan_empty_kernel(DeviceCounts* cnt, DeviceCounts const* prev,
+                                  ForestMeta const* fm, int64_t* job_first, uint32_t num_jobs)
+{
+  uint32_t const j    = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t const base = prev ? prev->hit_base + prev->hits : 0;
+  if (j == 0) {
+    cnt->pairs    = 0;
+    cnt->chunks   = 0;
+    cnt->hits     = 0;
+    cnt->hit_base = base;
+    cnt->flags    = (fm && fm->overflow) ? 2u : 0u;
+  }
+  if (j < num_jobs) job_first[j] = static_cast<int64_t>(base);
+}
+
+// Leaf pairs of the join, clamped to the pair capacity; starts the group's flags.
+__global__ void plan_pairs_kernel(JoinMeta const* jm, int levels, uint64_t cap_out,
+                                  ForestMeta const* fm, DeviceCounts* cnt,
+                                  DeviceCounts const* prev)
+{
+  if (threadIdx.x != 0) return;
+  uint64_t k = 0;
+  for (int l = 0; l < levels; ++l) k += jm->level[l].leaf;
+  uint32_t f = 0;
+  if (jm->bad_index) f |= 1u;
+  if (jm->overflow || fm->overflow || k > cap_out) f |= 2u;
+  cnt->pairs    = k < cap_out ? k : cap_out;
+  cnt->chunks   = 0;
+  cnt->hits     = 0;
+  cnt->hit_base = prev ? prev->hit_base + prev->hits : 0;
+  cnt->flags    = f;
+}
+
+// Sort keys of the leaf pairs (leaf offsets); rows past the pair count get the
+// largest key, so the stable sort leaves them after every real pair.
+__global__ void __launch_bounds__(256)
+  plan_gather_kernel(uint32_t const* __restrict__ out_node, uint32_t const* __restrict__ offset,
+                     DeviceCounts const* cnt, uint64_t cap, uint32_t* __restrict__ keys)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cap) return;
+  keys[i] = i < cnt->pairs ? offset[out_node[i]] : 0xFFFFFFFFu;
+}
+
+__global__ void __launch_bounds__(256)
+  plan_permute_kernel(uint32_t const* __restrict__ perm, uint32_t const* __restrict__ out_node,
+                      uint32_t const* __restrict__ out_bbox, DeviceCounts const* cnt,
+                      uint32_t* __restrict__ bbox_o, uint32_t* __restrict__ quad_o)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cnt->pairs) return;
+  uint32_t const p = perm[i];
+  bbox_o[i]        = out_bbox[p];
+  quad_o[i]        = out_node[p];
+}
+
+// Job-local hits, written at the group's first output row.
+__global__ void __launch_bounds__(256)
+  plan_local_hits_kernel(uint32_t const* __restrict__ poly, uint32_t const* __restrict__ pos,
+                         DeviceCounts* cnt, JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                         uint64_t out_cap, uint32_t* __restrict__ out_poly,
+                         uint32_t* __restrict__ out_point)
+{
+  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cnt->hits) return;
+  uint64_t const o = cnt->hit_base + i;
+  if (o >= out_cap) {
+    atomicOr(&cnt->flags, 2u);
+    return;
+  }
+  uint32_t const p = pos[i];
+  uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
+  out_poly[o]      = poly[i] - jobs[j].poly_begin;
+  out_point[o]     = p - jobs[j].point_begin;
+}
+
+// First output row of every job of the group.
+__global__ void __launch_bounds__(256)
+  plan_job_first_kernel(uint32_t const* __restrict__ pos, DeviceCounts const* cnt,
+                        JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                        int64_t* __restrict__ job_first)
+{
+  uint32_t const j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= num_jobs) return;
+  uint32_t const b = jobs[j].point_begin;
+  uint64_t lo = 0, hi = cnt->hits;
+  while (lo < hi) {
+    uint64_t const mid = lo + ((hi - lo) >> 1);
+    if (pos[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  job_first[j] = static_cast<int64_t>(cnt->hit_base + lo);
+}
+
+template <class T>
+void zero_struct(T* p, hipStream_t stream)
+{
+  static_assert(sizeof(T) % 4 == 0, "cleared as 32-bit words");
+  fill_u32(reinterpret_cast<uint32_t*>(p), sizeof(T) / 4, 0u, stream);
+}
+
+struct PlanCaps {
+  uint64_t pairs = 0, frontier = 0, chunks = 0, hits = 0;
+};
+
+// One group's fixed description and every buffer its launch sequence touches.
+struct PlanGroup {
+  int64_t j0 = 0, j1 = 0;
+  uint32_t J = 0;
+  int64_t P0 = 0, PO0 = 0;
+  uint64_t n = 0, npg = 0;
+  int key_bits = 1, maxd = 0, sort_bits = 1;
+  uint64_t node_cap = 0;
+  bool tree = false, join = false;  // stages that run (host-known)
+  PlanCaps cap;
+  dbuf<JobDev> jobs;
+  // tree
+  dbuf<uint32_t> keys_a, keys_b, vals, vals_tmp, counts0, sort_sums;
+  dbuf<uint32_t> nkey, nstart, ncount, cfirst, ccount, njob, partials;
+  dbuf<uint8_t> nlevel, f_internal;
+  dbuf<uint32_t> f_length, f_offset;
+  dbuf<uint4> split, staging;
+  dbuf<JobTree> trees;
+  dbuf<ForestMeta> fmeta;
+  // boxes (built at creation)
+  dbuf<double> boxes;
+  // join and pair sort (capacity-sized)
+  dbuf<uint64_t> top_off, partials64;
+  dbuf<JoinMeta> jmeta;
+  dbuf<uint32_t> out_node, out_bbox, fa_node, fa_bbox, fb_node, fb_bbox;
+  dbuf<uint32_t> ka, kb, va, vb, pb, pq, pair_counts, pair_sums;
+  // point-in-polygon (capacity-sized)
+  dbuf<uint32_t> chunk_start, hit_counts, hit_poly, hit_pos;
+  dbuf<uint64_t> pip_partials, pip_totals, masks, hit_start;
+  dbuf<uint2> chunk_info;
+  uint32_t pairs_per_wave = 16;
+};
+
+}  // namespace
+
+struct BatchPlan {
+  int device = 0;
+  hipStream_t cstream = nullptr;  // capture stream
+  std::vector<HostJob> hj;
+  std::vector<int64_t> jpo, jpoly;
+  int64_t num_jobs = 0, num_points = 0;
+  double const* points_xy = nullptr;
+  int32_t const* poly_offsets = nullptr;
+  int32_t const* ring_offsets = nullptr;
+  int64_t num_ring_offsets = 0, num_vertices = 0;
+  double const *poly_x = nullptr, *poly_y = nullptr;
+  dbuf<Edge> edges;
+  dbuf<EdgeB> edges_b;
+  dbuf<uint32_t> ring_edges;
+  std::vector<std::unique_ptr<PlanGroup>> groups;
+  dbuf<DeviceCounts> counts;  // one per group
+  dbuf<int64_t> job_first;    // per job: first output row
+  dbuf<uint32_t> out_pi, out_poly, out_point;
+  uint64_t hit_cap = 0;
+  hipGraph_t graph    = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t last_stream = nullptr;
+  bool launched = false;
+
+  ~BatchPlan()
+  {
+    if (launched) (void)hipStreamSynchronize(last_stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cstream) {
+      (void)hipStreamSynchronize(cstream);
+      (void)hipStreamDestroy(cstream);
+    }
+  }
+};
+
+namespace {
+
+// Allocates the capacity-sized buffers of a group (join, pair sort, PIP).
+void plan_alloc(PlanGroup& g)
+{
+  PlanCaps const& c = g.cap;
+  g.out_node.reset(c.pairs);
+  g.out_bbox.reset(c.pairs);
+  g.fa_node.reset(c.frontier);
+  g.fa_bbox.reset(c.frontier);
+  g.fb_node.reset(c.frontier);
+  g.fb_bbox.reset(c.frontier);
+  g.ka.reset(c.pairs);
+  g.kb.reset(c.pairs);
+  g.va.reset(c.pairs);
+  g.vb.reset(c.pairs);
+  g.pb.reset(c.pairs);
+  g.pq.reset(c.pairs);
+  g.pair_counts.reset(sort_counts_elems(c.pairs));
+  g.pair_sums.reset(sort_sums_elems(c.pairs));
+  g.chunk_start.reset(c.pairs + 1);
+  g.masks.reset(c.chunks);
+  g.hit_counts.reset(c.chunks);
+  g.chunk_info.reset(c.chunks);
+  g.hit_start.reset(c.chunks);
+  g.hit_poly.reset(c.hits);
+  g.hit_pos.reset(c.hits);
+}
+
+// Enqueues group g's launch sequence: launches only (no allocation, no host
+// synchronisation, no memset node), so that it can be captured.
+void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
+{
+  DeviceCounts* cnt        = P.counts.get() + gi;
+  DeviceCounts const* prev = gi > 0 ? P.counts.get() + gi - 1 : nullptr;
+  int64_t* job_first       = P.job_first.get() + g.j0;
+  if (!g.tree) {
+    plan_empty_kernel<<<grid_for(g.J, 256), 256, 0, s>>>(cnt, prev, nullptr, job_first, g.J);
+    QPIP_LAUNCH_CHECK();
+    return;
+  }
+  double const* xy = P.points_xy + 2 * g.P0;
+  // 1-2. keys and the sort
+  uint64_t const tiles = sort_tiles(g.n);
+  {
+    KernelTimer kt("batch_keygen", s);
+    batch_keygen_kernel<<<static_cast<unsigned>(tiles), 256, 0, s>>>(
+      xy, g.n, g.jobs.get(), g.J, g.key_bits, g.keys_a.get(), g.counts0.get());
+    QPIP_LAUNCH_CHECK();
+  }
+  SortResult sorted = radix_sort_pairs(g.keys_a.get(), g.keys_b.get(), nullptr, g.vals.get(),
+                                       g.vals_tmp.get(), g.n, g.sort_bits, s, g.counts0.get(),
+                                       "batch_sort", nullptr, g.sort_sums.get());
+  // 3. forest levels
+  zero_struct(g.fmeta.get(), s);
+  {
+    KernelTimer kt("batch_levels", s);
+    batch_level0_kernel<<<(g.J + 3) / 4, 256, 0, s>>>(sorted.keys, g.jobs.get(), g.J, g.key_bits,
+                                                      g.staging.get(), g.trees.get());
+    QPIP_LAUNCH_CHECK();
+    Level0Op l0{g.trees.get(), g.staging.get(), g.nkey.get(), g.nstart.get(), g.ncount.get(),
+                g.nlevel.get(), g.njob.get()};
+    count_scan_emit<Level0Op, uint32_t, uint64_t>(l0, nullptr, g.J, g.partials.get(),
+                                                  &g.fmeta.get()->sizes[0], s,
+                                                  scan_blocks_for(g.J), "batch_level0");
+    JobLevels rule{g.njob.get(), g.jobs.get(), g.key_bits};
+    for (int l = 0; l + 2 <= g.maxd; ++l) {
+      LevelArgs la{sorted.keys, g.nkey.get(), g.nstart.get(), g.ncount.get(), g.nlevel.get(),
+                   g.cfirst.get(), g.ccount.get(), g.split.get(), g.partials.get(),
+                   g.fmeta.get()->sizes, &g.fmeta.get()->overflow, g.node_cap, l};
+      level_split_kernel<<<kLevelBlocks, 256, 0, s>>>(la, rule);
+      QPIP_LAUNCH_CHECK();
+      level_emit_kernel<<<kLevelBlocks, 256, 0, s>>>(la, rule);
+      QPIP_LAUNCH_CHECK();
+    }
+  }
+  int const levels = std::max(g.maxd, 1);
+  batch_marked_kernel<<<(g.J + 255) / 256, 256, 0, s>>>(g.nkey.get(), g.nstart.get(),
+                                                        g.ncount.get(), g.njob.get(),
+                                                        g.jobs.get(), g.J, g.fmeta.get(),
+                                                        g.trees.get());
+  QPIP_LAUNCH_CHECK();
+  batch_finalize_kernel<<<grid_for(g.node_cap, 256), 256, 0, s>>>(
+    g.nkey.get(), g.nstart.get(), g.ncount.get(), g.nlevel.get(), g.njob.get(), g.cfirst.get(),
+    g.ccount.get(), g.jobs.get(), g.trees.get(), 0, g.fmeta.get(), levels, g.f_internal.get(),
+    g.f_length.get(), g.f_offset.get());
+  QPIP_LAUNCH_CHECK();
+  local_point_indices_kernel<<<grid_for(g.n, 256), 256, 0, s>>>(g.vals.get(), g.n, g.jobs.get(),
+                                                               g.J, P.out_pi.get() + g.P0);
+  QPIP_LAUNCH_CHECK();
+  if (!g.join) {
+    plan_empty_kernel<<<grid_for(g.J, 256), 256, 0, s>>>(cnt, prev, g.fmeta.get(), job_first,
+                                                         g.J);
+    QPIP_LAUNCH_CHECK();
+    return;
+  }
+  // 5. join over the forest (node arrays read up to the node capacity)
+  uint64_t const npg = g.npg;
+  double* const bminx = g.boxes.get();
+  TreeView tv{g.nkey.get(), g.nlevel.get(), g.f_internal.get(), g.f_length.get(),
+              g.f_offset.get(), g.node_cap};
+  BoxView bv{bminx, bminx + npg, bminx + 2 * npg, bminx + 3 * npg, npg};
+  JobGeometry geom{g.njob.get(), g.jobs.get()};
+  zero_struct(g.jmeta.get(), s);
+  JoinMeta* dm = g.jmeta.get();
+  TopSizeOp ts{g.trees.get(), g.jobs.get(), g.top_off.get()};
+  count_scan_emit<TopSizeOp, uint64_t, uint64_t>(ts, nullptr, g.J, g.partials64.get(),
+                                                 &dm->top_pairs, s, scan_blocks_for(g.J),
+                                                 "batch_join_setup");
+  BatchTopOp top{tv, bv, geom, g.trees.get(), g.jobs.get(), g.top_off.get(), g.J, dm,
+                 g.out_node.get(), g.out_bbox.get(), g.fa_node.get(), g.fa_bbox.get(),
+                 g.cap.pairs, g.cap.frontier};
+  count_scan_emit<BatchTopOp, uint64_t, SplitTotal>(top, &dm->top_pairs, 0, g.partials64.get(),
+                                                    &dm->level[0], s, kScanBlocks,
+                                                    "batch_join_level");
+  uint32_t* cur_node = g.fa_node.get();
+  uint32_t* cur_bbox = g.fa_bbox.get();
+  uint32_t* nxt_node = g.fb_node.get();
+  uint32_t* nxt_bbox = g.fb_bbox.get();
+  for (int lvl = 1; lvl < g.maxd; ++lvl) {
+    DescendOp<JobGeometry> op{tv, bv, geom, dm, cur_node, cur_bbox, g.out_node.get(),
+                              g.out_bbox.get(), nxt_node, nxt_bbox, g.cap.pairs, g.cap.frontier,
+                              lvl};
+    count_scan_emit<DescendOp<JobGeometry>, uint64_t, SplitTotal>(
+      op, &dm->level[lvl - 1].quad, 0, g.partials64.get(), &dm->level[lvl], s, kScanBlocks,
+      "batch_join_level");
+    std::swap(cur_node, nxt_node);
+    std::swap(cur_bbox, nxt_bbox);
+  }
+  plan_pairs_kernel<<<1, 64, 0, s>>>(dm, levels, g.cap.pairs, g.fmeta.get(), cnt, prev);
+  QPIP_LAUNCH_CHECK();
+  plan_gather_kernel<<<grid_for(g.cap.pairs, 256, 1u << 30), 256, 0, s>>>(
+    g.out_node.get(), g.f_offset.get(), cnt, g.cap.pairs, g.ka.get());
+  QPIP_LAUNCH_CHECK();
+  SortResult ps = radix_sort_pairs(g.ka.get(), g.kb.get(), nullptr, g.va.get(), g.vb.get(),
+                                   g.cap.pairs, std::max(1, bits_for(g.n)), s, nullptr,
+                                   "batch_join_sort", g.pair_counts.get(), g.pair_sums.get());
+  plan_permute_kernel<<<grid_for(g.cap.pairs, 256, 1u << 30), 256, 0, s>>>(
+    ps.values, g.out_node.get(), g.out_bbox.get(), cnt, g.pb.get(), g.pq.get());
+  QPIP_LAUNCH_CHECK();
+  // 6. point-in-polygon on the forest
+  PipDevice d{};
+  d.pair_poly      = g.pb.get();
+  d.pair_quad      = g.pq.get();
+  d.pair_cap       = g.cap.pairs;
+  d.length         = g.f_length.get();
+  d.offset         = g.f_offset.get();
+  d.num_nodes      = g.node_cap;
+  d.point_indices  = g.vals.get();
+  d.num_points     = g.n;
+  d.xy             = xy;
+  d.poly_offsets   = reinterpret_cast<uint32_t const*>(P.poly_offsets + g.PO0);
+  d.num_polys      = npg;
+  d.ring_offsets   = reinterpret_cast<uint32_t const*>(P.ring_offsets);
+  d.num_rings      = static_cast<uint64_t>(P.num_ring_offsets - 1);
+  d.ring_edges     = P.ring_edges.get();
+  d.edges          = P.edges.get();
+  d.edges_b        = P.edges_b.get();
+  d.pairs_per_wave = g.pairs_per_wave;
+  d.chunk_start    = g.chunk_start.get();
+  d.partials       = g.pip_partials.get();
+  d.totals         = g.pip_totals.get();
+  d.masks          = g.masks.get();
+  d.hit_counts     = g.hit_counts.get();
+  d.chunk_info     = g.chunk_info.get();
+  d.hit_start      = g.hit_start.get();
+  d.chunk_cap      = g.cap.chunks;
+  d.out_poly       = g.hit_poly.get();
+  d.out_pos        = g.hit_pos.get();
+  d.hit_cap        = g.cap.hits;
+  d.counts         = cnt;
+  launch_pip_device(d, s);
+  plan_local_hits_kernel<<<grid_for(g.cap.hits, 256, 1u << 30), 256, 0, s>>>(
+    g.hit_poly.get(), g.hit_pos.get(), cnt, g.jobs.get(), g.J, P.hit_cap, P.out_poly.get(),
+    P.out_point.get());
+  QPIP_LAUNCH_CHECK();
+  plan_job_first_kernel<<<grid_for(g.J, 256), 256, 0, s>>>(g.hit_pos.get(), cnt, g.jobs.get(),
+                                                          g.J, job_first);
+  QPIP_LAUNCH_CHECK();
+}
+
+// Builds a group's fixed buffers and its polygon boxes (run once, not captured).
+void plan_group_setup(BatchPlan& P, PlanGroup& g, hipStream_t s)
+{
+  std::vector<HostJob> const& hj = P.hj;
+  int64_t const* jpo             = P.jpo.data();
+  int64_t const* jpoly           = P.jpoly.data();
+  g.J   = static_cast<uint32_t>(g.j1 - g.j0);
+  g.P0  = jpo[g.j0];
+  g.PO0 = jpoly[g.j0];
+  g.n   = static_cast<uint64_t>(jpo[g.j1] - g.P0);
+  g.npg = static_cast<uint64_t>(jpoly[g.j1] - g.PO0);
+  g.key_bits = 1;
+  g.maxd     = 0;
+  for (int64_t j = g.j0; j < g.j1; ++j) {
+    g.key_bits = std::max(g.key_bits, hj[j].key_bits);
+    g.maxd     = std::max(g.maxd, hj[j].d);
+  }
+  g.sort_bits = std::min(32, g.key_bits + bits_for(g.J - 1));
+  g.tree      = g.n > 0;
+  g.join      = g.tree && g.npg > 0;
+  std::vector<JobDev> hd = job_descs(hj, g.j0, g.j1, jpo, jpoly);
+  g.jobs.reset(g.J);
+  QPIP_HIP(hipMemcpyAsync(g.jobs.get(), hd.data(), g.J * sizeof(JobDev), hipMemcpyHostToDevice,
+                          s));
+  QPIP_HIP(hipStreamSynchronize(s));  // hd is pageable host memory
+  if (!g.tree) return;
+  uint64_t const tiles = sort_tiles(g.n);
+  g.keys_a.reset(g.n);
+  g.keys_b.reset(g.n);
+  g.vals.reset(g.n);
+  g.vals_tmp.reset(g.n);
+  g.counts0.reset(tiles * kRadix);
+  g.sort_sums.reset(sort_sums_elems(g.n));
+  g.node_cap = forest_capacity(hj, g.j0, g.j1, jpo);
+  uint64_t const cap = g.node_cap;
+  g.nkey.reset(cap);
+  g.nstart.reset(cap);
+  g.ncount.reset(cap);
+  g.cfirst.reset(cap);
+  g.ccount.reset(cap);
+  g.njob.reset(cap);
+  g.nlevel.reset(cap);
+  g.f_internal.reset(cap);
+  g.f_length.reset(cap);
+  g.f_offset.reset(cap);
+  g.split.reset(cap);
+  g.staging.reset(static_cast<size_t>(g.J) * 64);
+  g.trees.reset(g.J);
+  g.fmeta.reset(1);
+  g.partials.reset(std::max(kLevelBlocks, kScanBlocks) + 1);
+  if (!g.join) return;
+  uint64_t const npg = g.npg;
+  g.boxes.reset(4 * npg);
+  double* bminx = g.boxes.get();
+  fill_nan_kernel<<<grid_for(4 * npg, 256), 256, 0, s>>>(bminx, 4 * npg);
+  QPIP_LAUNCH_CHECK();
+  dbuf<uint32_t> prefix(npg + 1);
+  NonEmptyOp ne{P.poly_offsets + g.PO0, P.ring_offsets, P.num_ring_offsets, P.num_vertices,
+                prefix.get()};
+  count_scan_emit<NonEmptyOp, uint32_t, uint32_t>(ne, nullptr, npg, g.partials.get(),
+                                                  prefix.get() + npg, s, scan_blocks_for(npg),
+                                                  "batch_bbox_scan");
+  batch_bbox_kernel<<<grid_for(npg, 256 / kWave), 256, 0, s>>>(
+    P.poly_offsets + g.PO0, P.ring_offsets, P.num_ring_offsets, P.poly_x, P.poly_y,
+    P.num_vertices, prefix.get(), npg, g.jobs.get(), g.J, bminx, bminx + npg, bminx + 2 * npg,
+    bminx + 3 * npg);
+  QPIP_LAUNCH_CHECK();
+  QPIP_HIP(hipStreamSynchronize(s));  // prefix is freed on return
+  g.top_off.reset(g.J);
+  g.partials64.reset(kScanBlocks + 1);
+  g.jmeta.reset(1);
+  g.pip_partials.reset(kScanBlocks + 1);
+  g.pip_totals.reset(2);
+}
+
+// True sizes of a group's last run (valid where no earlier stage overflowed).
+PlanCaps plan_measure(PlanGroup const& g, hipStream_t s)
+{
+  PlanCaps m;
+  if (!g.join) return m;
+  JoinMeta jm;
+  uint64_t tot[2];
+  QPIP_HIP(hipMemcpyAsync(&jm, g.jmeta.get(), sizeof(JoinMeta), hipMemcpyDeviceToHost, s));
+  QPIP_HIP(hipMemcpyAsync(tot, g.pip_totals.get(), sizeof(tot), hipMemcpyDeviceToHost, s));
+  QPIP_HIP(hipStreamSynchronize(s));
+  m.frontier = jm.top_pairs > 0 ? 1 : 0;
+  for (int l = 0; l < std::max(g.maxd, 1); ++l) {
+    m.pairs += jm.level[l].leaf;
+    m.frontier = std::max<uint64_t>(m.frontier, jm.level[l].quad);
+  }
+  m.chunks = tot[0];
+  m.hits   = tot[1];
+  return m;
+}
+
 }  // namespace
 
 }  // namespace qpip
@@ -867,70 +1428,21 @@ extern "C" int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
       job_hit_offsets[0] = 0;
       return QPIP_SUCCESS;
     }
-    QPIP_EXPECTS(job_point_offsets && job_polygon_offsets && job_bbox && job_scale &&
-                   job_max_depth && job_max_size,
-                 "job descriptions must not be null");
-    QPIP_EXPECTS(num_poly_x == num_poly_y,
-                 "numbers of vertices must be the same for both x and y columns");
-    QPIP_EXPECTS(job_point_offsets[0] == 0 && job_point_offsets[num_jobs] == num_points,
-                 "job point offsets must start at 0 and end at the number of points");
-    int64_t const num_polys = num_poly_offsets > 0 ? num_poly_offsets - 1 : 0;
-    QPIP_EXPECTS(job_polygon_offsets[0] == 0 && job_polygon_offsets[num_jobs] == num_polys,
-                 "job polygon offsets must start at 0 and end at the number of polygons");
-    std::vector<HostJob> hj(num_jobs);
-    for (int64_t j = 0; j < num_jobs; ++j) {
-      QPIP_EXPECTS(job_point_offsets[j + 1] >= job_point_offsets[j] &&
-                     job_polygon_offsets[j + 1] >= job_polygon_offsets[j],
-                   "job offsets must not decrease");
-      HostJob& h  = hj[j];
-      double const x0 = job_bbox[4 * j], x1 = job_bbox[4 * j + 1];
-      double const y0 = job_bbox[4 * j + 2], y1 = job_bbox[4 * j + 3];
-      // indexing.py:165-189 / detail/point_quadtree.cuh:259-268
-      h.min_x    = std::min(x0, x1);
-      h.max_x    = std::max(x0, x1);
-      h.min_y    = std::min(y0, y1);
-      h.max_y    = std::max(y0, y1);
-      h.max_size = std::max(1, job_max_size[j]);
-      h.d        = std::max(0, std::min(15, static_cast<int>(job_max_depth[j])));
-      h.scale    = std::max(job_scale[j],
-                         std::max(h.max_x - h.min_x, h.max_y - h.min_y) / ((1 << h.d) + 2));
-      // quadtree_bbox_filtering.cu:102-106, for jobs that reach the join
-      if (job_point_offsets[j + 1] > job_point_offsets[j] &&
-          job_polygon_offsets[j + 1] > job_polygon_offsets[j]) {
-        QPIP_EXPECTS(h.scale > 0, "scale must be positive");
-        QPIP_EXPECTS(h.min_x < h.max_x && h.min_y < h.max_y,
-                     "invalid bounding box (x_min, x_max, y_min, y_max)");
-        QPIP_EXPECTS(h.d > 0 && h.d < 16, "maximum depth must be positive and less than 16");
-      }
-      int cbits = 0;
-      while ((uint64_t{1} << cbits) <= (uint64_t{1} << h.d) + 2) ++cbits;
-      h.key_bits = std::min(32, 2 * cbits);
-    }
+    std::vector<HostJob> hj =
+      prepare_jobs(num_jobs, job_point_offsets, job_polygon_offsets, job_bbox, job_scale,
+                   job_max_depth, job_max_size, num_points, num_poly_offsets, num_poly_x,
+                   num_poly_y);
     if (num_points > 0) QPIP_EXPECTS(points_xy, "points must not be null");
 
     OutputAllocator oa(mr, stream);
     uint32_t* o_pi = num_points > 0 ? oa.make<uint32_t>(num_points) : nullptr;
     std::vector<uint64_t> job_hits(num_jobs, 0);
     std::vector<GroupResult> groups;
-    // groups of consecutive jobs whose job number and key fit 32 bits, < 2^31 points
-    int64_t j0 = 0;
-    while (j0 < num_jobs) {
-      int64_t j1  = j0 + 1;
-      int kb      = hj[j0].key_bits;
-      while (j1 < num_jobs) {
-        int const kb2 = std::max(kb, hj[j1].key_bits);
-        if (kb2 + bits_for(static_cast<uint64_t>(j1 - j0)) > 32) break;
-        if (job_point_offsets[j1 + 1] - job_point_offsets[j0] >= (int64_t{1} << 31)) break;
-        kb = kb2;
-        ++j1;
-      }
-      QPIP_EXPECTS(job_point_offsets[j1] - job_point_offsets[j0] < (int64_t{1} << 31),
-                   "a job must have fewer than 2^31 points");
+    for (auto const& [j0, j1] : job_groups(hj, job_point_offsets)) {
       groups.emplace_back();
       run_group(hj, j0, j1, job_point_offsets, job_polygon_offsets, points_xy, poly_offsets,
                 ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
                 o_pi ? o_pi + job_point_offsets[j0] : nullptr, groups.back(), job_hits, stream);
-      j0 = j1;
     }
     uint64_t H = 0;
     job_hit_offsets[0] = 0;
@@ -963,4 +1475,291 @@ extern "C" int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
     set_last_error(e.what());
     return QPIP_ERROR_DEVICE;
   }
+}
+
+// ---------------------------------------------------------------------------------
+// Captured plan entry points
+// ---------------------------------------------------------------------------------
+namespace {
+
+template <class F>
+int plan_guarded(F&& f)
+{
+  using namespace qpip;
+  try {
+    f();
+    return QPIP_SUCCESS;
+  } catch (logic_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_LOGIC;
+  } catch (oom_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_OOM;
+  } catch (capacity_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_CAPACITY;
+  } catch (std::exception const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_DEVICE;
+  }
+}
+
+qpip::BatchPlan* plan_of(qpip_batch_plan* p) { return reinterpret_cast<qpip::BatchPlan*>(p); }
+
+void check_device(qpip::BatchPlan const& P)
+{
+  int dev = 0;
+  QPIP_HIP(hipGetDevice(&dev));
+  QPIP_EXPECTS(dev == P.device, "the plan belongs to another device");
+}
+
+}  // namespace
+
+extern "C" int qpip_batch_plan_create(int64_t num_jobs,
+                                      const int64_t* job_point_offsets,
+                                      const int64_t* job_polygon_offsets,
+                                      const double* job_bbox,
+                                      const double* job_scale,
+                                      const int8_t* job_max_depth,
+                                      const int32_t* job_max_size,
+                                      const double* points_xy,
+                                      int64_t num_points,
+                                      const int32_t* poly_offsets,
+                                      int64_t num_poly_offsets,
+                                      const int32_t* ring_offsets,
+                                      int64_t num_ring_offsets,
+                                      const double* poly_x,
+                                      const double* poly_y,
+                                      int64_t num_poly_x,
+                                      int64_t num_poly_y,
+                                      double headroom,
+                                      void* stream_,
+                                      qpip_batch_plan** plan)
+{
+  using namespace qpip;
+  return plan_guarded([&] {
+    QPIP_EXPECTS(plan != nullptr, "output pointers must not be null");
+    *plan = nullptr;
+    QPIP_EXPECTS(num_jobs >= 1, "a plan needs at least one job");
+    QPIP_EXPECTS(headroom >= 1.0 && headroom <= 64.0, "headroom must be in [1, 64]");
+    std::vector<HostJob> hj =
+      prepare_jobs(num_jobs, job_point_offsets, job_polygon_offsets, job_bbox, job_scale,
+                   job_max_depth, job_max_size, num_points, num_poly_offsets, num_poly_x,
+                   num_poly_y);
+    if (num_points > 0) QPIP_EXPECTS(points_xy, "points must not be null");
+    QPIP_EXPECTS(num_ring_offsets >= 1 || num_poly_offsets <= 1,
+                 "ring offsets need at least one entry");
+    // the caller's pending writes of the points and polygons come first
+    QPIP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream_)));
+
+    auto P = std::make_unique<BatchPlan>();
+    QPIP_HIP(hipGetDevice(&P->device));
+    QPIP_HIP(hipStreamCreateWithFlags(&P->cstream, hipStreamNonBlocking));
+    hipStream_t const s = P->cstream;
+    P->hj               = hj;
+    P->jpo.assign(job_point_offsets, job_point_offsets + num_jobs + 1);
+    P->jpoly.assign(job_polygon_offsets, job_polygon_offsets + num_jobs + 1);
+    P->num_jobs         = num_jobs;
+    P->num_points       = num_points;
+    P->points_xy        = points_xy;
+    P->poly_offsets     = poly_offsets;
+    P->ring_offsets     = ring_offsets;
+    P->num_ring_offsets = num_ring_offsets;
+    P->num_vertices     = num_poly_x;
+    P->poly_x           = poly_x;
+    P->poly_y           = poly_y;
+    int64_t const rings = num_ring_offsets > 0 ? num_ring_offsets - 1 : 0;
+    P->edges.reset(std::max<int64_t>(num_poly_x, 1));
+    P->edges_b.reset(std::max<int64_t>(num_poly_x, 1));
+    P->ring_edges.reset(std::max<int64_t>(rings, 1));
+    if (rings > 0) {
+      build_edges_kernel<<<grid_for(static_cast<uint64_t>(rings), 256 / kWave), 256, 0, s>>>(
+        reinterpret_cast<uint32_t const*>(ring_offsets), rings, poly_x, poly_y, num_poly_x,
+        P->edges.get(), P->edges_b.get(), P->ring_edges.get());
+      QPIP_LAUNCH_CHECK();
+    }
+    for (auto const& [j0, j1] : job_groups(P->hj, P->jpo.data())) {
+      auto g = std::make_unique<PlanGroup>();
+      g->j0  = j0;
+      g->j1  = j1;
+      plan_group_setup(*P, *g, s);
+      if (g->join) {
+        uint64_t const c = std::max<uint64_t>(uint64_t{1} << 16, 8 * g->npg);
+        g->cap           = PlanCaps{c, c, 2 * c, std::max<uint64_t>(uint64_t{1} << 16, g->n)};
+      }
+      P->groups.push_back(std::move(g));
+    }
+    size_t const G = P->groups.size();
+    P->counts.reset(G);
+    P->job_first.reset(static_cast<size_t>(num_jobs));
+    P->out_pi.reset(static_cast<size_t>(num_points));
+
+    // sizing runs: grow each exceeded capacity until one run fits
+    auto alloc_all = [&] {
+      uint64_t H = 0;
+      for (auto& g : P->groups) {
+        if (!g->join) continue;
+        plan_alloc(*g);
+        g->pairs_per_wave = pip_pairs_per_wave(g->cap.pairs);
+        H += g->cap.hits;
+      }
+      P->hit_cap = H;
+      P->out_poly.reset(std::max<uint64_t>(H, 1));
+      P->out_point.reset(std::max<uint64_t>(H, 1));
+    };
+    std::vector<DeviceCounts> hc(G);
+    std::vector<PlanCaps> measured(G);
+    bool fits = false;
+    for (int attempt = 0; attempt < 16 && !fits; ++attempt) {
+      alloc_all();
+      for (size_t gi = 0; gi < G; ++gi) plan_enqueue(*P, *P->groups[gi], gi, s);
+      QPIP_HIP(hipMemcpyAsync(hc.data(), P->counts.get(), G * sizeof(DeviceCounts),
+                              hipMemcpyDeviceToHost, s));
+      QPIP_HIP(hipStreamSynchronize(s));
+      fits = true;
+      for (size_t gi = 0; gi < G; ++gi) {
+        if (hc[gi].flags & 1u)
+          throw logic_error("cuSpatial failure: forest child range or index out of range");
+        PlanGroup& g = *P->groups[gi];
+        measured[gi] = plan_measure(g, s);
+        if (!(hc[gi].flags & 2u)) continue;
+        fits          = false;
+        PlanCaps& c   = g.cap;
+        PlanCaps const& m = measured[gi];
+        bool grown    = false;
+        auto grow     = [&](uint64_t& cap, uint64_t need) {
+          if (need <= cap) return;
+          cap   = need + need / 4 + 1024;
+          grown = true;
+        };
+        grow(c.pairs, m.pairs);
+        grow(c.frontier, m.frontier);
+        grow(c.chunks, m.chunks);
+        grow(c.hits, m.hits);
+        if (!grown) {  // a later stage saw clamped input: widen everything
+          c.pairs *= 2;
+          c.frontier *= 2;
+          c.chunks *= 2;
+          c.hits *= 2;
+        }
+      }
+    }
+    if (!fits) throw device_error("batch plan sizing did not converge");
+    // final capacities: the sizing run's counts times the headroom
+    for (size_t gi = 0; gi < G; ++gi) {
+      PlanGroup& g = *P->groups[gi];
+      if (!g.join) continue;
+      PlanCaps const& m = measured[gi];
+      auto room         = [&](uint64_t v) {
+        return static_cast<uint64_t>(static_cast<double>(v) * headroom) + 1024;
+      };
+      g.cap = PlanCaps{room(m.pairs), room(m.frontier), room(m.chunks), room(m.hits)};
+      QPIP_EXPECTS(g.cap.pairs < (uint64_t{1} << 32) && g.cap.chunks < (uint64_t{1} << 31) &&
+                     g.cap.hits < (uint64_t{1} << 32),
+                   "batch plan capacities exceed 32-bit row counts");
+    }
+    alloc_all();
+    for (size_t gi = 0; gi < G; ++gi)
+      if (P->groups[gi]->join) P->groups[gi]->pairs_per_wave = pip_pairs_per_wave(measured[gi].pairs);
+    QPIP_HIP(hipStreamSynchronize(s));
+
+    // capture the whole sequence once
+    QPIP_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    set_capturing(true);
+    try {
+      for (size_t gi = 0; gi < G; ++gi) plan_enqueue(*P, *P->groups[gi], gi, s);
+    } catch (...) {
+      set_capturing(false);
+      hipGraph_t broken = nullptr;
+      (void)hipStreamEndCapture(s, &broken);
+      if (broken) (void)hipGraphDestroy(broken);
+      throw;
+    }
+    set_capturing(false);
+    QPIP_HIP(hipStreamEndCapture(s, &P->graph));
+    QPIP_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    *plan = reinterpret_cast<qpip_batch_plan*>(P.release());
+  });
+}
+
+extern "C" int qpip_batch_plan_launch(qpip_batch_plan* plan, void* stream_)
+{
+  using namespace qpip;
+  return plan_guarded([&] {
+    QPIP_EXPECTS(plan != nullptr, "plan must not be null");
+    BatchPlan& P = *plan_of(plan);
+    check_device(P);
+    hipStream_t const s = static_cast<hipStream_t>(stream_);
+    if (P.launched && P.last_stream != s) QPIP_HIP(hipStreamSynchronize(P.last_stream));
+    QPIP_HIP(hipGraphLaunch(P.exec, s));
+    P.last_stream = s;
+    P.launched    = true;
+  });
+}
+
+extern "C" int qpip_batch_plan_result(qpip_batch_plan* plan,
+                                      const qpip_resource* mr,
+                                      void* stream_,
+                                      uint32_t** point_indices,
+                                      uint32_t** polygon_index,
+                                      uint32_t** point_index,
+                                      int64_t* job_hit_offsets,
+                                      int64_t* num_hits)
+{
+  using namespace qpip;
+  return plan_guarded([&] {
+    QPIP_EXPECTS(plan != nullptr, "plan must not be null");
+    QPIP_EXPECTS(point_indices && polygon_index && point_index && job_hit_offsets && num_hits,
+                 "output pointers must not be null");
+    BatchPlan& P = *plan_of(plan);
+    check_device(P);
+    QPIP_EXPECTS(P.launched, "the plan has not been launched");
+    QPIP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream_)));
+    QPIP_HIP(hipStreamSynchronize(P.last_stream));
+    size_t const G = P.groups.size();
+    std::vector<DeviceCounts> hc(G);
+    std::vector<int64_t> first(static_cast<size_t>(P.num_jobs));
+    QPIP_HIP(hipMemcpy(hc.data(), P.counts.get(), G * sizeof(DeviceCounts),
+                       hipMemcpyDeviceToHost));
+    QPIP_HIP(hipMemcpy(first.data(), P.job_first.get(), first.size() * sizeof(int64_t),
+                       hipMemcpyDeviceToHost));
+    uint32_t flags = 0;
+    for (auto const& c : hc) flags |= c.flags;
+    if (flags & 1u) throw logic_error("cuSpatial failure: forest child range or index out of range");
+    if (flags & 2u)
+      throw capacity_error(
+        "batch plan capacity exceeded: this launch produced more pairs, chunks or hits than "
+        "the plan was sized for; create the plan again (or with more headroom)");
+    uint64_t const H = G ? hc[G - 1].hit_base + hc[G - 1].hits : 0;
+    for (int64_t j = 0; j < P.num_jobs; ++j) job_hit_offsets[j] = first[j];
+    job_hit_offsets[P.num_jobs] = static_cast<int64_t>(H);
+    *num_hits                   = static_cast<int64_t>(H);
+    if (!mr) {  // the plan's own buffers
+      *point_indices = P.out_pi.get();
+      *polygon_index = P.out_poly.get();
+      *point_index   = P.out_point.get();
+      return;
+    }
+    hipStream_t const s = static_cast<hipStream_t>(stream_);
+    OutputAllocator oa(mr, s);
+    uint64_t const n = static_cast<uint64_t>(P.num_points);
+    uint32_t* o_pi   = oa.make<uint32_t>(n);
+    uint32_t* o_poly = oa.make<uint32_t>(H);
+    uint32_t* o_pt   = oa.make<uint32_t>(H);
+    if (n) QPIP_HIP(hipMemcpyAsync(o_pi, P.out_pi.get(), n * 4, hipMemcpyDeviceToDevice, s));
+    if (H) {
+      QPIP_HIP(hipMemcpyAsync(o_poly, P.out_poly.get(), H * 4, hipMemcpyDeviceToDevice, s));
+      QPIP_HIP(hipMemcpyAsync(o_pt, P.out_point.get(), H * 4, hipMemcpyDeviceToDevice, s));
+    }
+    QPIP_HIP(hipStreamSynchronize(s));
+    *point_indices = o_pi;
+    *polygon_index = o_poly;
+    *point_index   = o_pt;
+  });
+}
+
+extern "C" int qpip_batch_plan_destroy(qpip_batch_plan* plan)
+{
+  using namespace qpip;
+  return plan_guarded([&] { delete plan_of(plan); });
 }

@@ -28,6 +28,10 @@ struct device_error : std::runtime_error {
 struct oom_error : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
+// A launch produced more rows than a caller-sized capacity holds.
+struct capacity_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 
 #define QPIP_STR2(x) #x
 #define QPIP_STR(x)  QPIP_STR2(x)
@@ -157,6 +161,7 @@ class OutputAllocator {
 // of HIP events on its stream; the report sums elapsed time per kernel name.
 // ----------------------------------------------------------------------------
 bool profiling_enabled();
+void set_capturing(bool on);  // this thread is capturing a launch sequence
 void profile_begin(char const* name, hipStream_t stream, void** token);
 void profile_end(void* token, hipStream_t stream);
 
@@ -292,7 +297,14 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
                             int bits,
                             hipStream_t stream,
                             uint32_t* first_counts       = nullptr,
-                            char const* tag              = "sort");
+                            char const* tag              = "sort",
+                            uint32_t* counts_ws          = nullptr,
+                            uint32_t* sums_ws            = nullptr);
+// Workspace of radix_sort_pairs for n pairs when the caller provides it (no
+// allocation inside the call, as a captured launch sequence needs): counts_ws
+// (unused when first_counts is given) and sums_ws.
+inline uint64_t sort_counts_elems(uint64_t n) { return sort_tiles(n) * kRadix; }
+inline uint64_t sort_sums_elems(uint64_t n) { return (sort_tiles(n) + 63) / 64 * kRadix; }
 
 inline unsigned grid_for(uint64_t n, int threads, unsigned cap = 1u << 20)
 {

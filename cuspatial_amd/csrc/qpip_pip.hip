@@ -32,7 +32,7 @@
 //   fallback (count, then allocate; :220-234) is the only path here.
 #include <algorithm>
 
-#include "qpip_edges.hpp"
+#include "qpip_pip.hpp"
 #include "qpip_scan.hpp"
 
 namespace qpip {
@@ -61,7 +61,11 @@ struct ChunkOp {
     }
     return (length[q] + kWave - 1) / kWave;
   }
-  __device__ void emit(uint64_t k, uint32_t prefix) const { chunk_start[k] = prefix; }
+  // Starts past 2^32 - 1 saturate: the test kernel's capacity check rejects them.
+  __device__ void emit(uint64_t k, uint64_t prefix) const
+  {
+    chunk_start[k] = prefix < 0xFFFFFFFFull ? static_cast<uint32_t>(prefix) : 0xFFFFFFFFu;
+  }
 };
 
 struct PipArgs {
@@ -87,6 +91,8 @@ struct PipArgs {
   double const* xy;  // interleaved x, y pairs instead of x / y (null: two arrays)
   bool xy_vec;       // xy is 16-byte aligned
   uint32_t pairs_per_wave;  // longest segment of a leaf's pairs one wave takes
+  uint64_t const* num_pairs_dev;  // non-null: the pair count lives on the device
+  uint64_t chunk_cap;             // entries of masks / hit_counts / chunk_info
 };
 
 // Segments of at most max(m, pairs >> s) pairs (m = 16, s = 16 by default; both
@@ -200,7 +206,8 @@ __global__ void __launch_bounds__(256)
                   uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
 {
   uint32_t const k0 = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-  if (k0 >= a.num_pairs) return;
+  uint64_t const K  = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
+  if (k0 >= K) return;
   uint32_t const quad = uniform(a.pair_quad[k0]);
   // A wave takes the pairs of one leaf from k0 up to the run's end or the next
   // multiple of pairs_per_wave, whichever comes first: a leaf that overlaps many
@@ -209,7 +216,7 @@ __global__ void __launch_bounds__(256)
   if (k0 > 0 && k0 % G != 0 && uniform(a.pair_quad[k0 - 1]) == quad) return;
   // Errors are recorded at the end: a global store before the edge loads would
   // keep them off the scalar unit (the compiler must assume it may alias).
-  bool err = false;
+  bool err = false, cap_err = false;
   if (quad >= a.num_nodes) {
     if (lane_id() == 0) atomicOr(bad, 1u);
     return;
@@ -254,7 +261,7 @@ __global__ void __launch_bounds__(256)
       }
       valid_m[u] = ballot(valid);
     }
-    for (uint32_t k = k0; k < a.num_pairs; ++k) {
+    for (uint32_t k = k0; k < K; ++k) {
       if (k > k0 && (k % G == 0 || uniform(a.pair_quad[k]) != quad)) break;
       uint32_t const poly = uniform(a.pair_poly[k]);
       uint64_t hit_m[4]   = {0, 0, 0, 0};
@@ -265,7 +272,9 @@ __global__ void __launch_bounds__(256)
         default: test_polygon<4>(a, poly, px, py, valid_m, hit_m, err); break;
       }
       uint64_t const slot = slot0 + static_cast<uint64_t>(k - k0) * chunks + j0;
-      if (lane < static_cast<int>(nu)) {
+      if (slot + nu > a.chunk_cap) {
+        cap_err = true;  // only with a caller-sized chunk capacity
+      } else if (lane < static_cast<int>(nu)) {
         uint64_t const m = lane == 0 ? hit_m[0] : lane == 1 ? hit_m[1] : lane == 2 ? hit_m[2] : hit_m[3];
         masks[slot + lane]      = m;
         hit_counts[slot + lane] = static_cast<uint32_t>(__popcll(m));
@@ -275,6 +284,7 @@ __global__ void __launch_bounds__(256)
     }
   }
   if (err) atomicOr(bad, 1u);
+  if (cap_err && lane == 0) atomicOr(bad, 2u);
 }
 
 // --- 4. scan of hit counts ----------------------------------------------------------
@@ -286,22 +296,40 @@ struct HitScanOp {
 };
 
 // --- 5. write -------------------------------------------------------------------------
+// With num_chunks_dev the chunk count lives on the device, and hits at or past
+// hit_cap are dropped and flagged (bit 2 of *bad) instead of written.
 __global__ void __launch_bounds__(256)
   pip_write_kernel(uint64_t num_chunks, uint64_t const* __restrict__ masks,
                    uint2 const* __restrict__ chunk_info, uint64_t const* __restrict__ hit_start,
-                   uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point)
+                   uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point,
+                   uint64_t const* num_chunks_dev, uint64_t hit_cap, uint32_t* bad)
 {
   uint32_t const c = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-  if (c >= num_chunks) return;
+  uint64_t const T = num_chunks_dev ? *num_chunks_dev : num_chunks;
+  if (c >= T) return;
   uint64_t const mask = masks[c];
   if (mask == 0) return;
   uint2 const info = chunk_info[c];
   int const l      = lane_id();
   if ((mask >> l) & 1ull) {
     uint64_t const o = hit_start[c] + static_cast<uint64_t>(__popcll(mask & lanemask_lt()));
-    out_poly[o]      = info.x;
-    out_point[o]     = info.y + static_cast<uint32_t>(l);
+    if (o < hit_cap) {
+      out_poly[o]  = info.x;
+      out_point[o] = info.y + static_cast<uint32_t>(l);
+    } else {
+      atomicOr(bad, 2u);
+    }
   }
+}
+
+// *out = min(*in, cap); bit 2 of *flags when *in exceeds cap.
+__global__ void clamp_count_kernel(uint64_t const* in, uint64_t cap, uint64_t* out,
+                                   uint32_t* flags)
+{
+  if (threadIdx.x != 0) return;
+  uint64_t const v = *in;
+  if (v > cap) atomicOr(flags, 2u);
+  *out = v < cap ? v : cap;
 }
 
 // --- brute force: every point against every polygon ------------------------------
@@ -401,6 +429,42 @@ __global__ void __launch_bounds__(256)
 
 }  // namespace
 
+uint32_t pip_pairs_per_wave(uint64_t num_pairs) { return pairs_per_wave(num_pairs); }
+
+void launch_pip_device(PipDevice const& d, hipStream_t stream)
+{
+  uint64_t* const n_pairs  = &d.counts->pairs;
+  uint64_t* const n_chunks = &d.counts->chunks;
+  uint32_t* const flags    = &d.counts->flags;
+  ChunkOp cop{d.pair_quad, d.length, d.num_nodes, d.chunk_start, flags};
+  count_scan_emit<ChunkOp, uint64_t, uint64_t>(cop, n_pairs, 0, d.partials, &d.totals[0], stream,
+                                               kScanBlocks, "pip_chunks");
+  clamp_count_kernel<<<1, 64, 0, stream>>>(&d.totals[0], d.chunk_cap, n_chunks, flags);
+  QPIP_LAUNCH_CHECK();
+  PipArgs args{d.pair_poly, d.pair_quad, d.pair_cap, d.length, d.offset, d.point_indices,
+               d.num_points, nullptr, nullptr, d.poly_offsets, d.num_polys, d.num_rings,
+               d.ring_offsets, d.ring_edges, d.edges, d.edges_b, d.chunk_start, d.chunk_cap,
+               d.num_nodes, d.xy, reinterpret_cast<uintptr_t>(d.xy) % 16 == 0, d.pairs_per_wave,
+               n_pairs, d.chunk_cap};
+  {
+    KernelTimer kt("pip_test", stream);
+    pip_test_kernel<<<grid_for(d.pair_cap, 4, 1u << 30), 256, 0, stream>>>(
+      args, d.masks, d.hit_counts, d.chunk_info, flags);
+    QPIP_LAUNCH_CHECK();
+  }
+  HitScanOp hop{d.hit_counts, d.hit_start};
+  count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, n_chunks, 0, d.partials, &d.totals[1],
+                                                 stream, kScanBlocks, "pip_scan");
+  clamp_count_kernel<<<1, 64, 0, stream>>>(&d.totals[1], d.hit_cap, &d.counts->hits, flags);
+  QPIP_LAUNCH_CHECK();
+  {
+    KernelTimer kt("pip_write", stream);
+    pip_write_kernel<<<grid_for(d.chunk_cap, 4, 1u << 30), 256, 0, stream>>>(
+      0, d.masks, d.chunk_info, d.hit_start, d.out_poly, d.out_pos, n_chunks, d.hit_cap, flags);
+    QPIP_LAUNCH_CHECK();
+  }
+}
+
 }  // namespace qpip
 
 namespace {
@@ -474,9 +538,9 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     }
     // 2. chunks
     dbuf<uint32_t> chunk_start(K + 1);
-    dbuf<uint32_t> partials32(kScanBlocks + 1);
+    dbuf<uint64_t> partials(kScanBlocks + 1);
     ChunkOp cop{quad_index, length, static_cast<uint64_t>(num_nodes), chunk_start.get(), bad.get()};
-    count_scan_emit<ChunkOp, uint32_t, uint64_t>(cop, nullptr, K, partials32.get(),
+    count_scan_emit<ChunkOp, uint64_t, uint64_t>(cop, nullptr, K, partials.get(),
                                                  &meta.get()->num_chunks, stream, scan_blocks_for(K),
                                                  "pip_chunks");
     PipMeta h_meta;
@@ -495,7 +559,7 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                  ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), T,
                  static_cast<uint64_t>(num_nodes), xy,
                  xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
-                 pairs_per_wave(K)};
+                 pairs_per_wave(K), nullptr, T};
     // 3. test
     dbuf<uint64_t> masks(T);
     dbuf<uint32_t> hit_counts(T);
@@ -509,9 +573,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     }
     // 4. scan
     dbuf<uint64_t> hit_start(T);
-    dbuf<uint64_t> partials64(kScanBlocks + 1);
     HitScanOp hop{hit_counts.get(), hit_start.get()};
-    count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, nullptr, T, partials64.get(),
+    count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, nullptr, T, partials.get(),
                                                    &meta.get()->num_hits, stream, scan_blocks_for(T),
                                                    "pip_scan");
     QPIP_HIP(hipMemcpyAsync(&h_meta, meta.get(), sizeof(PipMeta), hipMemcpyDeviceToHost, stream));
@@ -528,7 +591,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     {
       KernelTimer kt("pip_write", stream);
       pip_write_kernel<<<blocks, 256, 0, stream>>>(T, masks.get(), chunk_info.get(),
-                                                   hit_start.get(), o_poly, o_point);
+                                                   hit_start.get(), o_poly, o_point, nullptr, H,
+                                                   bad.get());
       QPIP_LAUNCH_CHECK();
     }
     QPIP_HIP(hipStreamSynchronize(stream));

@@ -1,0 +1,65 @@
+// The point-in-polygon stage as a fixed launch sequence whose sizes live on the
+// device (for captured launch sequences: no host synchronisation, no allocation).
+#pragma once
+
+#include "qpip_edges.hpp"
+
+namespace qpip {
+
+// Per launch, on the device. `pairs` is written by the producer of the pairs
+// before the stage runs; `chunks` and `hits` by the stage (each clamped to its
+// capacity); `flags` bit 1: an index out of range, bit 2: a capacity exceeded.
+struct DeviceCounts {
+  uint64_t pairs;
+  uint64_t chunks;
+  uint64_t hits;
+  uint64_t hit_base;  // first output row of this part of a larger result
+  uint32_t flags;
+  uint32_t pad;
+};
+
+struct PipDevice {
+  // pairs (polygon, leaf) in leaf-offset order, at most pair_cap of them
+  uint32_t const* pair_poly;
+  uint32_t const* pair_quad;
+  uint64_t pair_cap;
+  // the quadtree (num_nodes entries may be a capacity)
+  uint32_t const* length;
+  uint32_t const* offset;
+  uint64_t num_nodes;
+  uint32_t const* point_indices;
+  uint64_t num_points;
+  double const* xy;  // interleaved points
+  // polygons and their edge table
+  uint32_t const* poly_offsets;
+  uint64_t num_polys;
+  uint32_t const* ring_offsets;
+  uint64_t num_rings;
+  uint32_t const* ring_edges;
+  Edge const* edges;
+  EdgeB const* edges_b;
+  uint32_t pairs_per_wave;
+  // workspace
+  uint32_t* chunk_start;  // pair_cap + 1
+  uint64_t* partials;     // kScanBlocks + 1
+  uint64_t* totals;       // 2: chunk and hit totals before clamping
+  uint64_t* masks;        // chunk_cap
+  uint32_t* hit_counts;   // chunk_cap
+  uint2* chunk_info;      // chunk_cap
+  uint64_t* hit_start;    // chunk_cap
+  uint64_t chunk_cap;
+  // output: (polygon, point position) hits, at most hit_cap
+  uint32_t* out_poly;
+  uint32_t* out_pos;
+  uint64_t hit_cap;
+  DeviceCounts* counts;
+};
+
+// Enqueues the stage on `stream`: chunks, test, scan, write.
+void launch_pip_device(PipDevice const& d, hipStream_t stream);
+
+// The test kernel's pairs-per-wave segment length for a join of num_pairs pairs
+// (the runtime tuning parameters pip_min_pairs_per_wave, pip_pairs_shift).
+uint32_t pip_pairs_per_wave(uint64_t num_pairs);
+
+}  // namespace qpip

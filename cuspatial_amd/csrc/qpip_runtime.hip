@@ -238,7 +238,13 @@ void prof_drain_locked()
 }
 }  // namespace
 
-bool profiling_enabled() { return g_prof_on; }
+// Launches enqueued while this thread captures a launch sequence are not timed
+// (an event pair per launch would become part of the captured graph).
+thread_local bool t_capturing = false;
+
+bool profiling_enabled() { return g_prof_on && !t_capturing; }
+
+void set_capturing(bool on) { t_capturing = on; }
 
 void profile_begin(char const* name, hipStream_t stream, void** token)
 {

@@ -319,7 +319,9 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
                             int bits,
                             hipStream_t stream,
                             uint32_t* first_counts,
-                            char const* tag)
+                            char const* tag,
+                            uint32_t* counts_ws,
+                            uint32_t* sums_ws)
 {
   static_assert(kSortThreads == kRadix, "one thread per digit in the scatter prologue");
   if (bits < 1) bits = 1;
@@ -342,9 +344,11 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
   bool const vec = (reinterpret_cast<uintptr_t>(keys_a) % 16 == 0) &&
                    (reinterpret_cast<uintptr_t>(keys_b) % 16 == 0);
 
-  dbuf<uint32_t> own_counts(first_counts ? 0 : static_cast<size_t>(tiles) * kRadix);
-  uint32_t* counts = first_counts ? first_counts : own_counts.get();
-  dbuf<uint32_t> sums(static_cast<size_t>(groups) * kRadix);
+  static_assert(kScanRows == 64, "sort_sums_elems assumes 64 tiles per column-scan group");
+  dbuf<uint32_t> own_counts(first_counts || counts_ws ? 0 : static_cast<size_t>(tiles) * kRadix);
+  uint32_t* counts = first_counts ? first_counts : counts_ws ? counts_ws : own_counts.get();
+  dbuf<uint32_t> own_sums(sums_ws ? 0 : static_cast<size_t>(groups) * kRadix);
+  uint32_t* const sums = sums_ws ? sums_ws : own_sums.get();
   for (int p = 0; p < passes; ++p) {
     int const shift     = p * kRadixBits;
     int const nb        = (bits - shift) < kRadixBits ? (bits - shift) : kRadixBits;
@@ -359,12 +363,12 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
     {
       KernelTimer kt(t_scan.c_str(), stream);
       colscan_reduce_kernel<<<static_cast<unsigned>(groups), kRadix, 0, stream>>>(counts, tiles,
-                                                                               sums.get());
+                                                                               sums);
       QPIP_LAUNCH_CHECK();
-      colscan_top_kernel<<<1, 1024, 0, stream>>>(sums.get(), groups);
+      colscan_top_kernel<<<1, 1024, 0, stream>>>(sums, groups);
       QPIP_LAUNCH_CHECK();
       colscan_down_kernel<<<static_cast<unsigned>(groups), kRadix, 0, stream>>>(counts, tiles,
-                                                                             sums.get());
+                                                                             sums);
       QPIP_LAUNCH_CHECK();
     }
     {

@@ -468,6 +468,79 @@ def batched_quadtree_point_in_polygon(points: Points, job_point_offsets, polygon
                        res.take(pt.value, h.value, torch.uint32), jpo, hits)
 
 
+class BatchPlan:
+    """The batched join captured once as a HIP graph and replayed per launch.
+
+    The plan fixes the jobs, the polygons and the point buffer (``points.xy``,
+    interleaved on the device): ``launch()`` replays the whole pipeline over the
+    points that buffer holds at that moment, with no host synchronisation and no
+    allocation, and ``result()`` returns what ``batched_quadtree_point_in_polygon``
+    returns for those points. Rewrite the buffer in place between launches to
+    join new points with the same job layout. Capacities (leaf pairs, PIP chunks,
+    hits) are sized from the points present at creation times ``headroom``; a
+    launch that needs more raises ``CapacityError`` from ``result()`` and writes
+    nothing out of bounds.
+    """
+
+    def __init__(self, points: Points, job_point_offsets, polygons: Polygons,
+                 job_polygon_offsets, bboxes, scales, max_depths, max_sizes,
+                 headroom: float = 1.25):
+        if not isinstance(points, Points) or points.xy is None:
+            raise ValueError("a plan needs interleaved points (Points.from_points_xy)")
+        if not isinstance(polygons, Polygons):
+            raise ValueError("`polygons` Geoseries must contains only polygons geometries.")
+        self.jpo = np.ascontiguousarray(np.asarray(job_point_offsets, dtype=np.int64))
+        jpy = np.ascontiguousarray(np.asarray(job_polygon_offsets, dtype=np.int64))
+        J = len(self.jpo) - 1
+        if len(jpy) != J + 1:
+            raise ValueError("job_point_offsets and job_polygon_offsets must have the same length")
+        bb = np.ascontiguousarray(np.asarray(bboxes, dtype=np.float64).reshape(J, 4))
+        sc = np.ascontiguousarray(np.asarray(scales, dtype=np.float64).reshape(J))
+        md = np.ascontiguousarray(np.asarray(max_depths, dtype=np.int8).reshape(J))
+        ms = np.ascontiguousarray(np.asarray(max_sizes, dtype=np.int32).reshape(J))
+        self.points, self.polygons = points, polygons  # kept alive: the graph reads them
+        self.device = points.device
+        self.num_jobs, self.num_points = J, len(points)
+        h = ctypes.c_void_p()
+        ptr = lambda a: a.ctypes.data_as(_vp)  # noqa: E731
+        _capi.check(_capi.lib().qpip_batch_plan_create(
+            J, ptr(self.jpo), ptr(jpy), ptr(bb), ptr(sc), ptr(md), ptr(ms), _p(points.xy),
+            len(points), _p(polygons.part_offset), polygons.part_offset.numel(),
+            _p(polygons.ring_offset), polygons.ring_offset.numel(), _p(polygons.x),
+            _p(polygons.y), polygons.x.numel(), polygons.y.numel(), float(headroom),
+            _capi.stream_handle(self.device), ctypes.byref(h)))
+        self._h = h
+
+    def launch(self) -> None:
+        _capi.check(_capi.lib().qpip_batch_plan_launch(self._h,
+                                                       _capi.stream_handle(self.device)))
+
+    def result(self) -> BatchResult:
+        """Waits for the last launch; its outputs, copied into torch tensors."""
+        res = _capi.TorchResource(self.device)
+        pi, po, pt = _outs(3)
+        hits = np.zeros(self.num_jobs + 1, dtype=np.int64)
+        h = ctypes.c_int64()
+        _capi.check(_capi.lib().qpip_batch_plan_result(
+            self._h, ctypes.byref(res.struct), _capi.stream_handle(self.device),
+            ctypes.byref(pi), ctypes.byref(po), ctypes.byref(pt), hits.ctypes.data_as(_vp),
+            ctypes.byref(h)))
+        return BatchResult(res.take(pi.value, self.num_points, torch.uint32),
+                           res.take(po.value, h.value, torch.uint32),
+                           res.take(pt.value, h.value, torch.uint32), self.jpo, hits)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _capi.check(_capi.lib().qpip_batch_plan_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
 # ---------------------------------------------------------------------------
 # Linestrings
 # ---------------------------------------------------------------------------

@@ -43,6 +43,7 @@ extern "C" {
 #define QPIP_ERROR_LOGIC  1
 #define QPIP_ERROR_DEVICE 2
 #define QPIP_ERROR_OOM    3
+#define QPIP_ERROR_CAPACITY 4 /* a launch outgrew a capacity fixed when a plan was made */
 
 /* Caller-supplied device memory resource for outputs. */
 typedef struct qpip_resource {
@@ -393,6 +394,63 @@ QPIP_API int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
                                                     uint32_t** point_index,
                                                     int64_t* job_hit_offsets,
                                                     int64_t* num_hits);
+
+/*
+ * Batched quadtree point-in-polygon as one captured HIP graph (no counterpart in
+ * the reference). A plan fixes the jobs (the arguments of
+ * qpip_batched_quadtree_point_in_polygon), the polygons and the ADDRESS of the
+ * point buffer; each qpip_batch_plan_launch replays the whole pipeline as one
+ * graph launch over the points the buffer then holds (rewrite them between
+ * launches, keeping the job layout), with no host synchronisation and no
+ * allocation. Per job the results equal qpip_batched_quadtree_point_in_polygon's
+ * on the same points.
+ *   create:  sizes the intermediate buffers by running the pipeline over the
+ *            points in the buffer now (capacities = those counts x headroom,
+ *            headroom in [1, 64]), then captures it. The polygons and the point
+ *            buffer must outlive the plan.
+ *   launch:  enqueues one replay on `stream`. Launches of one plan are
+ *            serialised (a launch on another stream waits for the previous one).
+ *   result:  waits for the last launch and returns its outputs:
+ *            point_indices (num_points), polygon_index / point_index (*num_hits),
+ *            job_hit_offsets (HOST, num_jobs + 1), as in the batched call. With
+ *            a resource `mr` the columns are copied into memory allocated from
+ *            it; with mr == NULL they are the plan's own buffers (valid until its
+ *            next launch or destroy; do not qpip_free them).
+ *            QPIP_ERROR_CAPACITY when the launch produced more leaf pairs, PIP
+ *            chunks or hits than the plan holds (no output was written out of
+ *            bounds; create the plan again from these points).
+ */
+typedef struct qpip_batch_plan qpip_batch_plan;
+QPIP_API int qpip_batch_plan_create(int64_t num_jobs,
+                                    const int64_t* job_point_offsets,
+                                    const int64_t* job_polygon_offsets,
+                                    const double* job_bbox,
+                                    const double* job_scale,
+                                    const int8_t* job_max_depth,
+                                    const int32_t* job_max_size,
+                                    const double* points_xy,
+                                    int64_t num_points,
+                                    const int32_t* poly_offsets,
+                                    int64_t num_poly_offsets,
+                                    const int32_t* ring_offsets,
+                                    int64_t num_ring_offsets,
+                                    const double* poly_x,
+                                    const double* poly_y,
+                                    int64_t num_poly_x,
+                                    int64_t num_poly_y,
+                                    double headroom,
+                                    void* stream,
+                                    qpip_batch_plan** plan);
+QPIP_API int qpip_batch_plan_launch(qpip_batch_plan* plan, void* stream);
+QPIP_API int qpip_batch_plan_result(qpip_batch_plan* plan,
+                                    const qpip_resource* mr,
+                                    void* stream,
+                                    uint32_t** point_indices,
+                                    uint32_t** polygon_index,
+                                    uint32_t** point_index,
+                                    int64_t* job_hit_offsets,
+                                    int64_t* num_hits);
+QPIP_API int qpip_batch_plan_destroy(qpip_batch_plan* plan);
 
 /*
  * linestring_bounding_boxes

@@ -33,7 +33,7 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 27
+    assert len(names) == 31
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)
@@ -335,3 +335,32 @@ def test_tuning_parameters(capi):
     with pytest.raises(RuntimeError, match="unknown tuning parameter"):
         autotune.set("no_such_knob", 1)
     assert autotune.get("pip_pairs_shift") == 16
+
+
+def test_batch_plan_preconditions(capi):
+    """Plan creation checks its arguments before any device work; a capacity
+    overflow has its own status code and exception."""
+    L = capi.lib()
+    plan = ctypes.c_void_p()
+    jpo = (ctypes.c_int64 * 2)(0, 0)
+    jpy = (ctypes.c_int64 * 2)(0, 0)
+    bb = (ctypes.c_double * 4)(0.0, 1.0, 0.0, 1.0)
+    sc = (ctypes.c_double * 1)(0.1)
+    md = (ctypes.c_int8 * 1)(3)
+    ms = (ctypes.c_int32 * 1)(8)
+
+    def create(num_jobs, headroom):
+        return L.qpip_batch_plan_create(num_jobs, jpo, jpy, bb, sc, md, ms, None, 0, None, 1,
+                                        None, 1, None, None, 0, 0, headroom, None,
+                                        ctypes.byref(plan))
+
+    assert create(0, 1.25) == capi.QPIP_ERROR_LOGIC
+    assert "a plan needs at least one job" in capi.lib().qpip_last_error_message().decode()
+    assert create(1, 0.5) == capi.QPIP_ERROR_LOGIC
+    assert "headroom must be in [1, 64]" in capi.lib().qpip_last_error_message().decode()
+    assert plan.value is None
+    assert L.qpip_batch_plan_launch(None, None) == capi.QPIP_ERROR_LOGIC
+    assert "plan must not be null" in capi.lib().qpip_last_error_message().decode()
+    assert L.qpip_batch_plan_destroy(None) == capi.QPIP_SUCCESS
+    with pytest.raises(capi.CapacityError):
+        capi.check(capi.QPIP_ERROR_CAPACITY)
