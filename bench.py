@@ -98,6 +98,11 @@ def main():
                     help="labelled points the ranker is trained on (voronoi_learned)")
     ap.add_argument("--epochs", type=int, default=3, help="ranker training epochs")
     ap.add_argument("--jobs", type=int, default=4096, help="jobs of the batched workload")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batched_jobs: time only the eager batched call, not the captured plan")
+    ap.add_argument("--graph-only", action="store_true",
+                    help="batched_jobs: one eager call (the reference), then only the captured "
+                         "plan's replays (no per-job baseline)")
     ap.add_argument("--autotune", action="store_true",
                     help="batched_jobs: pick the PIP segment parameters with the autotuner "
                          "(cuspatial_amd/autotune.py) on this workload before timing")
@@ -289,59 +294,17 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def bench_batched(args, dev):
-    """Batched join: every job of the batched_jobs workload in one call per step;
-    the plain path (four single-job calls per job) on the same inputs is timed
-    once beside it, and every job's hits are compared between the two."""
+def plain_per_job(w, polys, dev, out):
+    """The plain path on the same inputs, job by job: each job's points and its own
+    polygons (offsets rebased to the job, vertex arrays sliced), as a caller that
+    owns one job's data passes them; built before the timed loop. Returns
+    (seconds, points/s, per-job hit-count equality with the batched result)."""
     import numpy as np
     import torch
 
     import cuspatial_amd as cs
-    from cuspatial_amd import _capi, datasets
 
-    log(f"[bench] generating batched_jobs ({args.jobs} jobs) on {dev}")
-    w = datasets.batched_jobs(num_jobs=args.jobs, device=dev)
     n = w.num_points
-    pts = cs.Points.from_points_xy(w.xy)
-    polys = cs.Polygons(w.part_offset, w.ring_offset, w.px, w.py)
-    torch.cuda.synchronize()
-    log(f"[bench] {n} points, {len(polys)} polygons, {len(w.px)} vertices")
-
-    def step():
-        return cs.batched_quadtree_point_in_polygon(pts, w.job_point_offsets, polys,
-                                                    w.job_polygon_offsets, w.bboxes, w.scales,
-                                                    w.max_depths, w.max_sizes)
-
-    tuned = None
-    if args.autotune:
-        from cuspatial_amd import autotune
-
-        tuned = autotune.tune(step, repeats=1, log=lambda s: log("[bench] " + s))
-        log(f"[bench] autotune chose {tuned['config']}: pip_test {tuned['baseline_ms']:.2f} -> "
-            f"{tuned['best_ms']:.2f} ms")
-    for i in range(args.warmup):
-        log(f"[bench] warmup {i}")
-        out = step()
-    del out
-    torch.cuda.synchronize()
-    _capi.profile_reset()
-    _capi.profile_enable(True)
-    times = []
-    for i in range(args.steps):
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        out = step()
-        torch.cuda.synchronize()
-        times.append(time.perf_counter() - t)
-    _capi.profile_enable(False)
-    prof = _capi.profile_report()
-    ms = 1e3 * sum(times) / len(times)
-    value = n / (ms / 1e3)
-    log(f"[bench] batched: {ms:.2f} ms per step, {value:.4g} points/s, hits={out.num_hits}")
-
-    # the plain path on the same inputs, job by job: each job's points and its own
-    # polygons (offsets rebased to the job, vertex arrays sliced), as a caller that
-    # owns one job's data passes them; built before the timed loop
     log("[bench] plain per-job calls")
     xy = w.xy
     part_h = np.asarray(w.part_offset, np.int64)
@@ -374,8 +337,102 @@ def bench_batched(args, dev):
             for j in range(w.num_jobs)]
     log(f"[bench] plain: {plain_s * 1e3:.1f} ms, {plain_value:.4g} points/s; hit counts equal "
         f"for {sum(same)}/{len(same)} jobs")
+    return plain_s, plain_value, same
+
+
+def bench_batched(args, dev):
+    """Batched join: every job of the batched_jobs workload in one call per step;
+    the plain path (four single-job calls per job) on the same inputs is timed
+    once beside it, and every job's hits are compared between the two."""
+    import numpy as np
+    import torch
+
+    import cuspatial_amd as cs
+    from cuspatial_amd import _capi, datasets
+
+    log(f"[bench] generating batched_jobs ({args.jobs} jobs) on {dev}")
+    w = datasets.batched_jobs(num_jobs=args.jobs, device=dev)
+    n = w.num_points
+    pts = cs.Points.from_points_xy(w.xy)
+    polys = cs.Polygons(w.part_offset, w.ring_offset, w.px, w.py)
+    torch.cuda.synchronize()
+    log(f"[bench] {n} points, {len(polys)} polygons, {len(w.px)} vertices")
+
+    def step():
+        return cs.batched_quadtree_point_in_polygon(pts, w.job_point_offsets, polys,
+                                                    w.job_polygon_offsets, w.bboxes, w.scales,
+                                                    w.max_depths, w.max_sizes)
+
+    tuned = None
+    if args.autotune:
+        from cuspatial_amd import autotune
+
+        tuned = autotune.tune(step, repeats=1, log=lambda s: log("[bench] " + s))
+        log(f"[bench] autotune chose {tuned['config']}: pip_test {tuned['baseline_ms']:.2f} -> "
+            f"{tuned['best_ms']:.2f} ms")
+    if args.graph_only:  # one eager call: the reference for the replays
+        out = step()
+        times, prof, ms, value = [float("nan")], {}, float("nan"), float("nan")
+        plain_s = plain_value = float("nan")
+        same = []
+    else:
+        for i in range(args.warmup):
+            log(f"[bench] warmup {i}")
+            out = step()
+        del out
+        torch.cuda.synchronize()
+        _capi.profile_reset()
+        _capi.profile_enable(True)
+        times = []
+        for i in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            out = step()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t)
+        _capi.profile_enable(False)
+        prof = _capi.profile_report()
+        ms = 1e3 * sum(times) / len(times)
+        value = n / (ms / 1e3)
+        log(f"[bench] batched: {ms:.2f} ms per step, {value:.4g} points/s, hits={out.num_hits}")
+        plain_s, plain_value, same = plain_per_job(w, polys, dev, out)
+
+    # the same call captured once as a HIP graph and replayed per step
+    graph = None
+    if not args.no_graph:
+        t = time.perf_counter()
+        plan = cs.BatchPlan(pts, w.job_point_offsets, polys, w.job_polygon_offsets, w.bboxes,
+                            w.scales, w.max_depths, w.max_sizes, headroom=1.25)
+        create_s = time.perf_counter() - t
+        log(f"[bench] graph plan created in {create_s:.2f} s")
+        for _ in range(args.warmup):
+            plan.launch()
+        torch.cuda.synchronize()
+        gtimes = []
+        for i in range(args.steps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            plan.launch()
+            torch.cuda.synchronize()
+            gtimes.append(time.perf_counter() - t)
+        got = plan.result()
+        identical = bool(np.array_equal(got.job_hit_offsets, out.job_hit_offsets) and all(
+            torch.equal(getattr(got, k).view(torch.int32), getattr(out, k).view(torch.int32))
+            for k in ("point_indices", "polygon_index", "point_index")))
+        plan.close()
+        gms = 1e3 * sum(gtimes) / len(gtimes)
+        graph = {"ms_per_step": gms, "value": n / (gms / 1e3), "create_s": create_s,
+                 "identical_to_eager": identical,
+                 "p_ms": {"min": 1e3 * min(gtimes), "max": 1e3 * max(gtimes)}}
+        log(f"[bench] graph: {gms:.2f} ms per step ({n / (gms / 1e3):.4g} points/s), "
+            f"identical to the eager call: {identical}")
+        if not identical:
+            raise SystemExit("graph replay differs from the eager batched call")
     kernels = {k: {"launches_per_step": c / args.steps, "ms_per_step": round(v / args.steps, 6)}
                for k, (c, v) in prof.items()}
+    eager = None if args.graph_only else {"ms_per_step": ms, "value": value}
+    if graph is not None:
+        value, ms = graph["value"], graph["ms_per_step"]
     line = {
         "metric": METRIC,
         "value": value,
@@ -389,17 +446,21 @@ def bench_batched(args, dev):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"batched_jobs: {w.num_jobs} jobs in one batched call",
+        "config": {"workload": f"batched_jobs: {w.num_jobs} jobs in "
+                               f"{'one captured HIP graph launch' if graph else 'one batched call'}",
                    "points": n, "polygons": len(polys), "vertices": int(len(w.px)),
                    "hits": out.num_hits, "seed": w.seed},
-        "plain_per_job": {"value": plain_value, "unit": "points/s", "seconds": plain_s,
-                          "jobs_with_equal_hit_counts": sum(same)},
-        "speedup_vs_plain": value / plain_value,
+        "plain_per_job": None if args.graph_only else {
+            "value": plain_value, "unit": "points/s", "seconds": plain_s,
+            "jobs_with_equal_hit_counts": sum(same)},
+        "speedup_vs_plain": None if args.graph_only else value / plain_value,
+        "eager_batched_call": eager,
+        "graph": graph,
         "autotune": None if tuned is None else {
             "config": tuned["config"], "pip_test_ms_before": tuned["baseline_ms"],
             "pip_test_ms_after": tuned["best_ms"], "trials": len(tuned["trials"])},
         "kernels": kernels,
-        "p_ms": {"min": 1e3 * min(times), "max": 1e3 * max(times)},
+        "p_ms": None if args.graph_only else {"min": 1e3 * min(times), "max": 1e3 * max(times)},
     }
     print(json.dumps(line), flush=True)
 

@@ -273,22 +273,14 @@ __global__ void __launch_bounds__(256)
 // The forest's is_internal / length / offset (detail/point_quadtree.cuh:88-133): an
 // internal node's offset is its first child's forest index, a leaf's its first
 // point position in the group's sorted order (minus the quirk's skipped points).
-__global__ void __launch_bounds__(256)
-  batch_finalize_kernel(uint32_t const* __restrict__ nkey, uint32_t const* __restrict__ nstart,
-                        uint32_t const* __restrict__ ncount, uint8_t const* __restrict__ nlevel,
-                        uint32_t const* __restrict__ njob, uint32_t const* __restrict__ cfirst,
-                        uint32_t const* __restrict__ ccount, JobDev const* __restrict__ jobs,
-                        JobTree const* __restrict__ trees, uint64_t m,
-                        ForestMeta const* __restrict__ meta, int levels,
-                        uint8_t* __restrict__ internal_out, uint32_t* __restrict__ length_out,
-                        uint32_t* __restrict__ offset_out)
+__device__ __forceinline__ void finalize_node(
+  uint32_t const* __restrict__ nkey, uint32_t const* __restrict__ nstart,
+  uint32_t const* __restrict__ ncount, uint8_t const* __restrict__ nlevel,
+  uint32_t const* __restrict__ njob, uint32_t const* __restrict__ cfirst,
+  uint32_t const* __restrict__ ccount, JobDev const* __restrict__ jobs,
+  JobTree const* __restrict__ trees, uint64_t i, uint8_t* __restrict__ internal_out,
+  uint32_t* __restrict__ length_out, uint32_t* __restrict__ offset_out)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (meta) {  // node count on the device: the sum of the level sizes
-    m = 0;
-    for (int l = 0; l < levels; ++l) m += meta->sizes[l];
-  }
-  if (i >= m) return;
   uint32_t const j     = njob[i];
   JobDev const& J      = jobs[j];
   int const level      = nlevel[i];
@@ -317,6 +309,26 @@ __global__ void __launch_bounds__(256)
   internal_out[i] = internal ? 1 : 0;
   length_out[i]   = length;
   offset_out[i]   = offset;
+}
+
+__global__ void __launch_bounds__(256)
+  batch_finalize_kernel(uint32_t const* __restrict__ nkey, uint32_t const* __restrict__ nstart,
+                        uint32_t const* __restrict__ ncount, uint8_t const* __restrict__ nlevel,
+                        uint32_t const* __restrict__ njob, uint32_t const* __restrict__ cfirst,
+                        uint32_t const* __restrict__ ccount, JobDev const* __restrict__ jobs,
+                        JobTree const* __restrict__ trees, uint64_t m,
+                        ForestMeta const* __restrict__ meta, int levels,
+                        uint8_t* __restrict__ internal_out, uint32_t* __restrict__ length_out,
+                        uint32_t* __restrict__ offset_out)
+{
+  if (meta) {  // node count on the device: the sum of the level sizes
+    m = 0;
+    for (int l = 0; l < levels; ++l) m += meta->sizes[l];
+  }
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    finalize_node(nkey, nstart, ncount, nlevel, njob, cfirst, ccount, jobs, trees, i,
+                  internal_out, length_out, offset_out);
 }
 
 // --- join ----------------------------------------------------------------------------
@@ -414,8 +426,9 @@ __global__ void __launch_bounds__(256)
   gather_offsets_kernel(uint32_t const* __restrict__ out_node, uint32_t const* __restrict__ offset,
                         uint64_t k, uint32_t* __restrict__ keys)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < k) keys[i] = offset[out_node[i]];
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    keys[i] = offset[out_node[i]];
 }
 
 __global__ void __launch_bounds__(256)
@@ -423,8 +436,8 @@ __global__ void __launch_bounds__(256)
                  uint32_t const* __restrict__ out_bbox, uint64_t k, uint32_t* __restrict__ bbox_o,
                  uint32_t* __restrict__ quad_o)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < k) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     uint32_t const p = perm[i];
     bbox_o[i]        = out_bbox[p];
     quad_o[i]        = out_node[p];
@@ -432,15 +445,30 @@ __global__ void __launch_bounds__(256)
 }
 
 // --- results back to job-local indices ---------------------------------------------
+// Tiles of 1024 sorted positions per block iteration: the tile's first and last
+// jobs are found once, each position's job by a search between them.
 __global__ void __launch_bounds__(256)
   local_point_indices_kernel(uint32_t const* __restrict__ sorted_vals, uint64_t n,
                              JobDev const* __restrict__ jobs, uint32_t num_jobs,
                              uint32_t* __restrict__ out)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t const j = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(i));
-  out[i]           = sorted_vals[i] - jobs[j].point_begin;
+  constexpr uint64_t kTile = 1024;
+  __shared__ uint32_t s_jobs[2];
+  for (uint64_t lo = static_cast<uint64_t>(blockIdx.x) * kTile; lo < n;
+       lo += static_cast<uint64_t>(gridDim.x) * kTile) {
+    uint64_t const hi = lo + kTile < n ? lo + kTile : n;
+    __syncthreads();  // the previous tile's readers of s_jobs are done
+    if (threadIdx.x == 0) {
+      s_jobs[0] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(lo));
+      s_jobs[1] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(hi - 1));
+    }
+    __syncthreads();
+    uint32_t const j0 = s_jobs[0], j1 = s_jobs[1];
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      uint32_t const j = j0 == j1 ? j0 : find_job(jobs, j0, j1, static_cast<uint32_t>(i));
+      out[i]           = sorted_vals[i] - jobs[j].point_begin;
+    }
+  }
 }
 
 __global__ void __launch_bounds__(256)
@@ -448,12 +476,13 @@ __global__ void __launch_bounds__(256)
                     uint64_t h, JobDev const* __restrict__ jobs, uint32_t num_jobs,
                     uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= h) return;
-  uint32_t const p = pos[i];
-  uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
-  out_poly[i]      = poly[i] - jobs[j].poly_begin;
-  out_point[i]     = p - jobs[j].point_begin;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t const p = pos[i];
+    uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
+    out_poly[i]      = poly[i] - jobs[j].poly_begin;
+    out_point[i]     = p - jobs[j].point_begin;
+  }
 }
 
 // First hit of every job (hits are in job order): first h with pos[h] >= point_begin.
@@ -505,16 +534,16 @@ struct NonEmptyOp {
   __device__ void emit(uint64_t p, uint32_t pre) const { prefix[p] = pre; }
 };
 
-__global__ void __launch_bounds__(256)
-  batch_bbox_kernel(int32_t const* __restrict__ part, int32_t const* __restrict__ ring,
-                    int64_t num_ring_offsets, double const* __restrict__ vx,
-                    double const* __restrict__ vy, int64_t nv, uint32_t const* __restrict__ prefix,
-                    uint64_t npg, JobDev const* __restrict__ jobs, uint32_t num_jobs,
-                    double* __restrict__ minx, double* __restrict__ miny, double* __restrict__ maxx,
-                    double* __restrict__ maxy)
+// The box of polygon p, by one wave.
+__device__ __forceinline__ void bbox_one(int32_t const* __restrict__ part,
+                                         int32_t const* __restrict__ ring,
+                                         int64_t num_ring_offsets, double const* __restrict__ vx,
+                                         double const* __restrict__ vy, int64_t nv,
+                                         uint32_t const* __restrict__ prefix, uint64_t p,
+                                         JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                                         double* __restrict__ minx, double* __restrict__ miny,
+                                         double* __restrict__ maxx, double* __restrict__ maxy)
 {
-  uint64_t const p = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (p >= npg) return;
   int64_t const v0 = poly_first_vertex(part, ring, num_ring_offsets, nv, p);
   int64_t const v1 = poly_first_vertex(part, ring, num_ring_offsets, nv, p + 1);
   if (v1 <= v0) return;
@@ -552,10 +581,26 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+__global__ void __launch_bounds__(256)
+  batch_bbox_kernel(int32_t const* __restrict__ part, int32_t const* __restrict__ ring,
+                    int64_t num_ring_offsets, double const* __restrict__ vx,
+                    double const* __restrict__ vy, int64_t nv, uint32_t const* __restrict__ prefix,
+                    uint64_t npg, JobDev const* __restrict__ jobs, uint32_t num_jobs,
+                    double* __restrict__ minx, double* __restrict__ miny, double* __restrict__ maxx,
+                    double* __restrict__ maxy)
+{
+  uint64_t const waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  for (uint64_t p = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+       p < npg; p += waves)
+    bbox_one(part, ring, num_ring_offsets, vx, vy, nv, prefix, p, jobs, num_jobs, minx, miny, maxx,
+             maxy);
+}
+
 __global__ void fill_nan_kernel(double* __restrict__ p, uint64_t n)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = __longlong_as_double(0x7FF8000000000000ll);
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    p[i] = __longlong_as_double(0x7FF8000000000000ll);
 }
 
 // --- host -------------------------------------------------------------------------
@@ -972,9 +1017,10 @@ __global__ void __launch_bounds__(256)
   plan_gather_kernel(uint32_t const* __restrict__ out_node, uint32_t const* __restrict__ offset,
                      DeviceCounts const* cnt, uint64_t cap, uint32_t* __restrict__ keys)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= cap) return;
-  keys[i] = i < cnt->pairs ? offset[out_node[i]] : 0xFFFFFFFFu;
+  uint64_t const k = cnt->pairs;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < cap;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    keys[i] = i < k ? offset[out_node[i]] : 0xFFFFFFFFu;
 }
 
 __global__ void __launch_bounds__(256)
@@ -982,11 +1028,13 @@ __global__ void __launch_bounds__(256)
                       uint32_t const* __restrict__ out_bbox, DeviceCounts const* cnt,
                       uint32_t* __restrict__ bbox_o, uint32_t* __restrict__ quad_o)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= cnt->pairs) return;
-  uint32_t const p = perm[i];
-  bbox_o[i]        = out_bbox[p];
-  quad_o[i]        = out_node[p];
+  uint64_t const k = cnt->pairs;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t const p = perm[i];
+    bbox_o[i]        = out_bbox[p];
+    quad_o[i]        = out_node[p];
+  }
 }
 
 // Job-local hits, written at the group's first output row.
@@ -996,17 +1044,19 @@ __global__ void __launch_bounds__(256)
                          uint64_t out_cap, uint32_t* __restrict__ out_poly,
                          uint32_t* __restrict__ out_point)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= cnt->hits) return;
-  uint64_t const o = cnt->hit_base + i;
-  if (o >= out_cap) {
-    atomicOr(&cnt->flags, 2u);
-    return;
+  uint64_t const h = cnt->hits, base = cnt->hit_base;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint64_t const o = base + i;
+    if (o >= out_cap) {
+      atomicOr(&cnt->flags, 2u);
+      return;
+    }
+    uint32_t const p = pos[i];
+    uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
+    out_poly[o]      = poly[i] - jobs[j].poly_begin;
+    out_point[o]     = p - jobs[j].point_begin;
   }
-  uint32_t const p = pos[i];
-  uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
-  out_poly[o]      = poly[i] - jobs[j].poly_begin;
-  out_point[o]     = p - jobs[j].point_begin;
 }
 
 // First output row of every job of the group.
@@ -1242,13 +1292,13 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   }
   plan_pairs_kernel<<<1, 64, 0, s>>>(dm, levels, g.cap.pairs, g.fmeta.get(), cnt, prev);
   QPIP_LAUNCH_CHECK();
-  plan_gather_kernel<<<grid_for(g.cap.pairs, 256, 1u << 30), 256, 0, s>>>(
+  plan_gather_kernel<<<grid_for(g.cap.pairs, 256), 256, 0, s>>>(
     g.out_node.get(), g.f_offset.get(), cnt, g.cap.pairs, g.ka.get());
   QPIP_LAUNCH_CHECK();
   SortResult ps = radix_sort_pairs(g.ka.get(), g.kb.get(), nullptr, g.va.get(), g.vb.get(),
                                    g.cap.pairs, std::max(1, bits_for(g.n)), s, nullptr,
                                    "batch_join_sort", g.pair_counts.get(), g.pair_sums.get());
-  plan_permute_kernel<<<grid_for(g.cap.pairs, 256, 1u << 30), 256, 0, s>>>(
+  plan_permute_kernel<<<grid_for(g.cap.pairs, 256), 256, 0, s>>>(
     ps.values, g.out_node.get(), g.out_bbox.get(), cnt, g.pb.get(), g.pq.get());
   QPIP_LAUNCH_CHECK();
   // 6. point-in-polygon on the forest
@@ -1283,7 +1333,7 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.hit_cap        = g.cap.hits;
   d.counts         = cnt;
   launch_pip_device(d, s);
-  plan_local_hits_kernel<<<grid_for(g.cap.hits, 256, 1u << 30), 256, 0, s>>>(
+  plan_local_hits_kernel<<<grid_for(g.cap.hits, 256), 256, 0, s>>>(
     g.hit_poly.get(), g.hit_pos.get(), cnt, g.jobs.get(), g.J, P.hit_cap, P.out_poly.get(),
     P.out_point.get());
   QPIP_LAUNCH_CHECK();

@@ -16,6 +16,8 @@
 // reference).
 #include <cmath>
 
+#include <algorithm>
+
 #include "qpip_box.hpp"
 #include "qpip_scan.hpp"
 
@@ -79,9 +81,11 @@ __global__ void __launch_bounds__(256)
                      double* __restrict__ minx,
                      double* __restrict__ miny,
                      double* __restrict__ maxx,
-                     double* __restrict__ maxy)
+                     double* __restrict__ maxy,
+                     int64_t base)
 {
-  int64_t const row = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  int64_t const row =
+    base + static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   if (row >= num_polys) return;
   double const nan = __builtin_nan("");
   if (row >= static_cast<int64_t>(*num_runs)) {
@@ -164,12 +168,16 @@ extern "C" int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
                                                       flags.get(), stream, scan_blocks_for(n_int),
                                                       "bbox_intervals");
     }
-    unsigned const blocks = static_cast<unsigned>((num_polys + 3) / 4);
     {
+      // one wave per box, in slices of 2^22 boxes (grids within 2^28 work-items)
       KernelTimer kt("bbox_reduce", stream);
-      bbox_reduce_kernel<<<blocks, 256, 0, stream>>>(x, y, lo.get(), hi.get(), flags.get(),
-                                                     num_polys, expansion_radius, o_minx, o_miny,
-                                                     o_maxx, o_maxy);
+      for (int64_t b = 0; b < num_polys; b += int64_t{1} << 22) {
+        int64_t const m = std::min<int64_t>(num_polys - b, int64_t{1} << 22);
+        bbox_reduce_kernel<<<static_cast<unsigned>((m + 3) / 4), 256, 0, stream>>>(
+          x, y, lo.get(), hi.get(), flags.get(), num_polys, expansion_radius, o_minx, o_miny,
+          o_maxx, o_maxy, b);
+        QPIP_LAUNCH_CHECK();
+      }
       QPIP_LAUNCH_CHECK();
     }
     uint32_t h_flags[2];

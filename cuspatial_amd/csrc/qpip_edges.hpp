@@ -41,18 +41,16 @@ __device__ __forceinline__ bool float_equal(double a, double b)
 // bitwise equal to the last one (a closed ring). Then the lanes build the edges in
 // parallel (ballot + popcount give each kept vertex its slot). Any other ring is
 // walked sequentially by lane 0, exactly as the reference does.
-__global__ void __launch_bounds__(256)
-  build_edges_kernel(uint32_t const* __restrict__ ring_offsets,
-                     int64_t num_rings,
-                     double const* __restrict__ vx,
-                     double const* __restrict__ vy,
-                     int64_t nv,
-                     Edge* __restrict__ edges,
-                     EdgeB* __restrict__ edges_b,
-                     uint32_t* __restrict__ ring_edges)
+// One ring, by one wave (build_edges_kernel below strides its waves over rings).
+__device__ __forceinline__ void build_ring_edges(uint32_t const* __restrict__ ring_offsets,
+                                                 int64_t r,
+                                                 double const* __restrict__ vx,
+                                                 double const* __restrict__ vy,
+                                                 int64_t nv,
+                                                 Edge* __restrict__ edges,
+                                                 EdgeB* __restrict__ edges_b,
+                                                 uint32_t* __restrict__ ring_edges)
 {
-  int64_t const r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-  if (r >= num_rings) return;
   int const lane = lane_id();
   int64_t vb     = ring_offsets[r];
   int64_t ve     = ring_offsets[r + 1];
@@ -117,6 +115,22 @@ __global__ void __launch_bounds__(256)
     by = ay;
   }
   ring_edges[r] = ne;
+}
+
+__global__ void __launch_bounds__(256)
+  build_edges_kernel(uint32_t const* __restrict__ ring_offsets,
+                     int64_t num_rings,
+                     double const* __restrict__ vx,
+                     double const* __restrict__ vy,
+                     int64_t nv,
+                     Edge* __restrict__ edges,
+                     EdgeB* __restrict__ edges_b,
+                     uint32_t* __restrict__ ring_edges)
+{
+  int64_t const waves = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+       r < num_rings; r += waves)
+    build_ring_edges(ring_offsets, r, vx, vy, nv, edges, edges_b, ring_edges);
 }
 
 }  // namespace

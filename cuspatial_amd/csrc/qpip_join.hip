@@ -99,8 +99,9 @@ __global__ void __launch_bounds__(256)
   gather_keys_kernel(uint32_t const* __restrict__ out_node, uint32_t const* __restrict__ offset,
                      uint64_t k, uint32_t* __restrict__ keys)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < k) keys[i] = offset[out_node[i]];
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    keys[i] = offset[out_node[i]];
 }
 
 __global__ void __launch_bounds__(256)
@@ -108,8 +109,8 @@ __global__ void __launch_bounds__(256)
                        uint32_t const* __restrict__ out_bbox, uint64_t k,
                        uint32_t* __restrict__ bbox_offset, uint32_t* __restrict__ quad_offset)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < k) {
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < k;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     uint32_t const j = perm[i];
     bbox_offset[i]   = out_bbox[j];
     quad_offset[i]   = out_node[j];

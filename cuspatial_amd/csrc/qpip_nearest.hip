@@ -109,9 +109,11 @@ __global__ void __launch_bounds__(256) nearest_fill_kernel(NearestArgs a)
   a.out_dist[p]  = 0.0;
 }
 
-__global__ void __launch_bounds__(256) nearest_kernel(NearestArgs a)
+// Pairs [base, base + waves of the grid): the host launches slices of 2^22 pairs so
+// that a grid stays within 2^28 work-items.
+__global__ void __launch_bounds__(256) nearest_kernel(NearestArgs a, uint64_t base)
 {
-  uint64_t const k0 = __builtin_amdgcn_readfirstlane(
+  uint64_t const k0 = base + __builtin_amdgcn_readfirstlane(
     static_cast<uint32_t>(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
   if (k0 >= a.num_pairs) return;
   uint32_t const quad = uniform_load(a.pair_quad + k0);
@@ -273,8 +275,12 @@ extern "C" int qpip_quadtree_point_to_nearest_linestring(const uint32_t* linestr
     }
     {
       KernelTimer kt("nearest_linestring", stream);
-      nearest_kernel<<<static_cast<unsigned>((num_pairs + 3) / 4), 256, 0, stream>>>(a);
-      QPIP_LAUNCH_CHECK();
+      uint64_t const K = static_cast<uint64_t>(num_pairs);
+      for (uint64_t b = 0; b < K; b += uint64_t{1} << 22) {
+        uint64_t const m = std::min<uint64_t>(K - b, uint64_t{1} << 22);
+        nearest_kernel<<<static_cast<unsigned>((m + 3) / 4), 256, 0, stream>>>(a, b);
+        QPIP_LAUNCH_CHECK();
+      }
     }
     uint32_t h_bad = 0;
     QPIP_HIP(hipMemcpyAsync(&h_bad, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

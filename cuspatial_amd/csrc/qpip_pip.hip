@@ -201,13 +201,19 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
   }
 }
 
+// Pairs [base, base + waves of the grid) per launch: the host launches slices of
+// kPipPairsPerLaunch pairs, so a grid never exceeds 2^28 work-items.
+constexpr uint64_t kPipPairsPerLaunch = uint64_t{1} << 22;
+
 __global__ void __launch_bounds__(256)
-  pip_test_kernel(PipArgs a, uint64_t* __restrict__ masks, uint32_t* __restrict__ hit_counts,
-                  uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
+  pip_test_kernel(PipArgs a, uint64_t base, uint64_t* __restrict__ masks,
+                  uint32_t* __restrict__ hit_counts, uint2* __restrict__ chunk_info,
+                  uint32_t* __restrict__ bad)
 {
-  uint32_t const k0 = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  uint64_t const kk = base + (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
   uint64_t const K  = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
-  if (k0 >= K) return;
+  if (kk >= K) return;
+  uint32_t const k0 = uniform(static_cast<uint32_t>(kk));
   uint32_t const quad = uniform(a.pair_quad[k0]);
   // A wave takes the pairs of one leaf from k0 up to the run's end or the next
   // multiple of pairs_per_wave, whichever comes first: a leaf that overlaps many
@@ -304,21 +310,35 @@ __global__ void __launch_bounds__(256)
                    uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point,
                    uint64_t const* num_chunks_dev, uint64_t hit_cap, uint32_t* bad)
 {
-  uint32_t const c = uniform(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
-  uint64_t const T = num_chunks_dev ? *num_chunks_dev : num_chunks;
-  if (c >= T) return;
-  uint64_t const mask = masks[c];
-  if (mask == 0) return;
-  uint2 const info = chunk_info[c];
-  int const l      = lane_id();
-  if ((mask >> l) & 1ull) {
-    uint64_t const o = hit_start[c] + static_cast<uint64_t>(__popcll(mask & lanemask_lt()));
-    if (o < hit_cap) {
-      out_poly[o]  = info.x;
-      out_point[o] = info.y + static_cast<uint32_t>(l);
-    } else {
-      atomicOr(bad, 2u);
+  uint64_t const T     = num_chunks_dev ? *num_chunks_dev : num_chunks;
+  uint64_t const waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+  int const l          = lane_id();
+  for (uint64_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < T; c += waves) {
+    uint64_t const mask = masks[c];
+    if (mask == 0) continue;
+    uint2 const info = chunk_info[c];
+    if ((mask >> l) & 1ull) {
+      uint64_t const o = hit_start[c] + static_cast<uint64_t>(__popcll(mask & lanemask_lt()));
+      if (o < hit_cap) {
+        out_poly[o]  = info.x;
+        out_point[o] = info.y + static_cast<uint32_t>(l);
+      } else {
+        atomicOr(bad, 2u);
+      }
     }
+  }
+}
+
+// Launches pip_test_kernel over pairs [0, pairs) in slices.
+void launch_pip_test(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
+                     uint2* chunk_info, uint32_t* bad, hipStream_t stream)
+{
+  for (uint64_t b = 0; b < pairs; b += kPipPairsPerLaunch) {
+    uint64_t const n = std::min<uint64_t>(pairs - b, kPipPairsPerLaunch);
+    pip_test_kernel<<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(a, b, masks,
+                                                                            hit_counts, chunk_info,
+                                                                            bad);
+    QPIP_LAUNCH_CHECK();
   }
 }
 
@@ -393,11 +413,8 @@ __global__ void __launch_bounds__(256)
 
 // pairwise_point_in_polygon (detail/point_in_polygon.cuh:104-160): point i against
 // polygon i. Each lane has its own polygon, so the edges are read per lane.
-__global__ void __launch_bounds__(256)
-  pip_pairwise_kernel(PipArgs a, uint64_t n, uint8_t* __restrict__ out)
+__device__ __forceinline__ void pairwise_one(PipArgs const& a, uint64_t i, uint8_t* out)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   double px, py;
   load_point(a, i, px, py);
   uint32_t r0 = a.poly_offsets[i], r1 = a.poly_offsets[i + 1];
@@ -427,6 +444,14 @@ __global__ void __launch_bounds__(256)
   out[i] = (inside && !on_edge) ? 1 : 0;
 }
 
+__global__ void __launch_bounds__(256)
+  pip_pairwise_kernel(PipArgs a, uint64_t n, uint8_t* __restrict__ out)
+{
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    pairwise_one(a, i, out);
+}
+
 }  // namespace
 
 uint32_t pip_pairs_per_wave(uint64_t num_pairs) { return pairs_per_wave(num_pairs); }
@@ -448,9 +473,7 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
                n_pairs, d.chunk_cap};
   {
     KernelTimer kt("pip_test", stream);
-    pip_test_kernel<<<grid_for(d.pair_cap, 4, 1u << 30), 256, 0, stream>>>(
-      args, d.masks, d.hit_counts, d.chunk_info, flags);
-    QPIP_LAUNCH_CHECK();
+    launch_pip_test(args, d.pair_cap, d.masks, d.hit_counts, d.chunk_info, flags, stream);
   }
   HitScanOp hop{d.hit_counts, d.hit_start};
   count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, n_chunks, 0, d.partials, &d.totals[1],
@@ -459,7 +482,7 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
   QPIP_LAUNCH_CHECK();
   {
     KernelTimer kt("pip_write", stream);
-    pip_write_kernel<<<grid_for(d.chunk_cap, 4, 1u << 30), 256, 0, stream>>>(
+    pip_write_kernel<<<grid_for(d.chunk_cap, 4), 256, 0, stream>>>(
       0, d.masks, d.chunk_info, d.hit_start, d.out_poly, d.out_pos, n_chunks, d.hit_cap, flags);
     QPIP_LAUNCH_CHECK();
   }
@@ -564,12 +587,11 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     dbuf<uint64_t> masks(T);
     dbuf<uint32_t> hit_counts(T);
     dbuf<uint2> chunk_info(T);
-    unsigned const blocks = static_cast<unsigned>((T + 3) / 4);
+
     {
       KernelTimer kt("pip_test", stream);
-      pip_test_kernel<<<static_cast<unsigned>((K + 3) / 4), 256, 0, stream>>>(
-        args, masks.get(), hit_counts.get(), chunk_info.get(), bad.get());
-      QPIP_LAUNCH_CHECK();
+      launch_pip_test(args, K, masks.get(), hit_counts.get(), chunk_info.get(), bad.get(),
+                      stream);
     }
     // 4. scan
     dbuf<uint64_t> hit_start(T);
@@ -590,7 +612,7 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     uint32_t* o_point = oa.make<uint32_t>(H);
     {
       KernelTimer kt("pip_write", stream);
-      pip_write_kernel<<<blocks, 256, 0, stream>>>(T, masks.get(), chunk_info.get(),
+      pip_write_kernel<<<grid_for(T, 4), 256, 0, stream>>>(T, masks.get(), chunk_info.get(),
                                                    hit_start.get(), o_poly, o_point, nullptr, H,
                                                    bad.get());
       QPIP_LAUNCH_CHECK();

@@ -262,37 +262,39 @@ __global__ void __launch_bounds__(256)
                   uint32_t* __restrict__ out_length,
                   uint32_t* __restrict__ out_offset)
 {
-  uint64_t const i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  uint32_t const key   = nkey[i];
-  int const level      = nlevel[i];
-  uint32_t const count = ncount[i];
-  uint32_t const start = nstart[i];
-  bool internal        = level <= d - 2 && count > max_size;
-  uint32_t length      = internal ? ccount[i] : count;
-  uint32_t offset      = internal ? cfirst[i] : start;
-  uint32_t const E     = meta->num_marked;
-  if (E > 0 && !internal) {
-    bool marked = false;
-    uint32_t skipped = 0;
-    for (uint32_t e = 0; e < E; ++e) {
-      if (!meta->marked_internal[e]) continue;
-      if (level == d - 1 && meta->marked_key[e] == key) marked = true;
-      if (meta->marked_start[e] < start) skipped += meta->marked_count[e];
-    }
-    if (marked) {
-      internal = true;
-      length   = 0;
-      offset   = static_cast<uint32_t>(m);
-    } else {
-      offset = start - skipped;
-    }
-  }
-  out_key[i]      = key;
-  out_level[i]    = static_cast<uint8_t>(level);
-  out_internal[i] = internal ? 1 : 0;
-  out_length[i]   = length;
-  out_offset[i]   = offset;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+    [&] {
+      uint32_t const key   = nkey[i];
+      int const level      = nlevel[i];
+      uint32_t const count = ncount[i];
+      uint32_t const start = nstart[i];
+      bool internal        = level <= d - 2 && count > max_size;
+      uint32_t length      = internal ? ccount[i] : count;
+      uint32_t offset      = internal ? cfirst[i] : start;
+      uint32_t const E     = meta->num_marked;
+      if (E > 0 && !internal) {
+        bool marked = false;
+        uint32_t skipped = 0;
+        for (uint32_t e = 0; e < E; ++e) {
+          if (!meta->marked_internal[e]) continue;
+          if (level == d - 1 && meta->marked_key[e] == key) marked = true;
+          if (meta->marked_start[e] < start) skipped += meta->marked_count[e];
+        }
+        if (marked) {
+          internal = true;
+          length   = 0;
+          offset   = static_cast<uint32_t>(m);
+        } else {
+          offset = start - skipped;
+        }
+      }
+      out_key[i]      = key;
+      out_level[i]    = static_cast<uint8_t>(level);
+      out_internal[i] = internal ? 1 : 0;
+      out_length[i]   = length;
+      out_offset[i]   = offset;
+    }();
 }
 
 }  // namespace

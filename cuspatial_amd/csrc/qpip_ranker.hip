@@ -331,20 +331,22 @@ struct alignas(16) Record {
 
 __global__ void __launch_bounds__(256) records_kernel(PolyView P, Record* __restrict__ out)
 {
-  int64_t const i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= P.num_polys) return;
-  uint32_t const r0 = P.poly_offsets[i];
-  uint32_t r1       = P.poly_offsets[i + 1];
-  if (r1 > P.num_rings) r1 = static_cast<uint32_t>(P.num_rings);
-  Record rec{0u, kMultiRing, 0.0};
-  if (r1 == r0 + 1) {
-    rec.e0 = P.ring_offsets[r0];
-    rec.ne = P.ring_edges[r0];
-    rec.y0 = rec.ne ? P.edges_b[rec.e0].by : 0.0;
-  } else if (r1 <= r0) {
-    rec.ne = 0;  // no rings: never contains a point
-  }
-  out[i] = rec;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < P.num_polys;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    [&] {
+      uint32_t const r0 = P.poly_offsets[i];
+      uint32_t r1       = P.poly_offsets[i + 1];
+      if (r1 > P.num_rings) r1 = static_cast<uint32_t>(P.num_rings);
+      Record rec{0u, kMultiRing, 0.0};
+      if (r1 == r0 + 1) {
+        rec.e0 = P.ring_offsets[r0];
+        rec.ne = P.ring_edges[r0];
+        rec.y0 = rec.ne ? P.edges_b[rec.e0].by : 0.0;
+      } else if (r1 <= r0) {
+        rec.ne = 0;  // no rings: never contains a point
+      }
+      out[i] = rec;
+    }();
 }
 
 struct LocateArgs {
