@@ -194,7 +194,10 @@ def main():
         "point_sort_count": ("hbm", n * 4 / 1e9, "GB"),
         # pass 0 reads keys and writes keys + indices; later passes move both
         "point_sort_scatter": ("hbm", n * (12 + 16 * (passes - 1)) / passes / 1e9, "GB"),
-        "pip_test": ("fp64", 6.0 * edge_tests / 1e12, "TFLOP"),
+        # each candidate point's coordinates and index, gathered in leaf order from
+        # randomly ordered input: the bound is the gather (one 128-byte line per
+        # 16-byte point), not the FP64 edge arithmetic (6 flop per edge test)
+        "pip_test": ("hbm", candidates * (16 + 4) / 1e9, "GB"),
     }
     kernels = {}
     for name, (cnt, tot_ms) in prof.items():
@@ -208,6 +211,10 @@ def main():
             entry.update(bound=bound, achieved=round(rate, 3),
                          unit="GB/s" if bound == "hbm" else "TFLOP/s",
                          frac=round(rate / peak, 4))
+        if name == "pip_test":
+            t = avg_ms / 1e3
+            entry.update(line_granular_gbs=round(candidates * 128 / 1e9 / t, 3),
+                         fp64_tflops=round(6.0 * edge_tests / 1e12 / t, 3))
         kernels[name] = entry
     dominant = max(prof.items(), key=lambda kv: kv[1][1])[0]
     dom = kernels[dominant]

@@ -1,0 +1,127 @@
+"""The quadtree point-in-polygon test at the predicate's boundaries.
+
+Points a few ULPs above, below and beside polygon vertices; on the lines of
+horizontal edges beyond their ends; on the vertical lines through vertical edges
+far above and below them (the reference's on-edge test holds there for any y:
+run == 0 makes both cross products zero); tight clusters hugging a vertex's y
+from one side; thin slivers with nearly horizontal and nearly vertical edges.
+These are where the collinearity filter and the crossing rule decide, and where
+any shortcut that passes over edges would go wrong.
+
+Bar: the hits equal the oracle's bit for bit, and the per-point polygon sets
+equal the brute-force bitmask's (GPU and oracle).
+"""
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import assert_same, run_gpu, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def cs():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a ROCm GPU"
+    import cuspatial_amd
+
+    return cuspatial_amd
+
+
+def ulps(v, k):
+    """v moved k ULPs (k may be negative)."""
+    out = np.float64(v)
+    step = np.inf if k > 0 else -np.inf
+    for _ in range(abs(k)):
+        out = np.nextafter(out, step)
+    return out
+
+
+def scene(seed, ext=64.0):
+    rng = np.random.default_rng(seed)
+    part, ring, vx, vy = [0], [0], [], []
+
+    def add_ring(xs, ys):
+        xs, ys = list(xs) + [xs[0]], list(ys) + [ys[0]]
+        vx.extend(xs)
+        vy.extend(ys)
+        ring.append(ring[-1] + len(xs))
+        part.append(part[-1] + 1)
+
+    # axis-aligned rectangles: vertical and horizontal edges
+    for _ in range(12):
+        x0, y0 = rng.uniform(2, ext - 12, 2)
+        w, h = rng.uniform(0.5, 8, 2)
+        add_ring([x0, x0 + w, x0 + w, x0], [y0, y0, y0 + h, y0 + h])
+    # stars and thin slivers (nearly horizontal / nearly vertical edges)
+    for _ in range(12):
+        cx, cy = rng.uniform(8, ext - 8, 2)
+        k = int(rng.integers(3, 14))
+        a = np.sort(rng.uniform(0, 2 * np.pi, k))
+        r = rng.uniform(1, 6) * rng.uniform(0.6, 1.4, k)
+        add_ring(cx + r * np.cos(a), cy + r * np.sin(a))
+    for _ in range(6):
+        x0, y0 = rng.uniform(4, ext - 20, 2)
+        add_ring([x0, x0 + 15, x0 + 7.5], [y0, ulps(y0, 3), y0 + 1e-9])
+        add_ring([x0, ulps(x0, 2), x0 + 1e-9], [y0, y0 + 15, y0 + 7.5])
+    vx, vy = np.array(vx), np.array(vy)
+    # points: around every vertex (a few ULPs in y and x), on vertical lines above
+    # and below, on horizontal lines beyond the ends, tight clusters, background
+    xs, ys = [], []
+    for px_, py_ in zip(vx, vy):
+        for k in (-8, -4, -2, -1, 0, 1, 2, 4, 8):
+            xs.append(px_)
+            ys.append(ulps(py_, k))
+        for k in (-2, 2):
+            xs.append(ulps(px_, k))
+            ys.append(py_)
+        for dy in (-20.0, -3.0, 3.0, 20.0):  # the vertical line through the vertex
+            xs.append(px_)
+            ys.append(py_ + dy)
+        for dx in (-5.0, 5.0):  # the horizontal line through the vertex
+            xs.append(px_ + dx)
+            ys.append(py_)
+    for _ in range(40):  # clusters hugging a vertex's y from one side
+        i = int(rng.integers(len(vx)))
+        side = 1 if rng.uniform() < 0.5 else -1
+        m = 64
+        xs.extend(vx[i] + rng.uniform(-0.5, 0.5, m))
+        ys.extend([ulps(vy[i], side * int(rng.integers(1, 6))) for _ in range(m)])
+    xs.extend(rng.uniform(0, ext, 3000))
+    ys.extend(rng.uniform(0, ext, 3000))
+    x = np.clip(np.array(xs), 0.0, np.nextafter(ext, 0))
+    y = np.clip(np.array(ys), 0.0, np.nextafter(ext, 0))
+    return x, y, (np.array(part, np.int32), np.array(ring, np.int32), vx, vy)
+
+
+@pytest.mark.parametrize("seed,d,max_size", [(1, 6, 16), (2, 8, 64), (3, 4, 256), (4, 10, 4),
+                                              (5, 7, 1)])
+def test_boundaries_match_oracle(cs, oracle_mod, seed, d, max_size):
+    x, y, polys = scene(seed)
+    bbox = (0.0, 64.0, 0.0, 64.0)
+    scale = 64.0 / (1 << d)
+    g = run_gpu(cs, x, y, bbox, scale, d, max_size, polys)
+    o = run_oracle(oracle_mod, x, y, bbox, scale, d, max_size, polys)
+    assert_same(g, o)
+    assert len(o["polygon_index"]) > 0
+
+
+def test_boundaries_match_bitmask(cs, oracle_mod):
+    """Per point, the polygons the quadtree path finds equal the brute-force
+    bitmask's (31 polygons), on the GPU and in the oracle."""
+    x, y, (part, ring, vx, vy) = scene(9)
+    part = part[:32]
+    ring = ring[:int(part[-1]) + 1]
+    vx, vy = vx[:int(ring[-1])], vy[:int(ring[-1])]
+    polys = (part, ring, vx, vy)
+    g = run_gpu(cs, x, y, (0.0, 64.0, 0.0, 64.0), 64.0 / 256, 8, 32, polys)
+    bits = np.zeros(len(x), np.int64)
+    for poly, pos in zip(g["polygon_index"], g["point_index"]):
+        bits[g["pi"][pos]] |= 1 << int(poly)
+    P = cs.Polygons(part, ring, vx, vy, device="cuda")
+    bm = cs.point_in_polygon_bitmask(cs.Points(x, y, device="cuda"), P)
+    assert np.array_equal(bits, bm.cpu().numpy().astype(np.int64))
+    ob = oracle_mod.point_in_polygon_bitmask(x, y, part, ring, vx, vy)
+    assert np.array_equal(bits, np.asarray(ob).astype(np.int64))
