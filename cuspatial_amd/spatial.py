@@ -259,6 +259,8 @@ def _u32(t: torch.Tensor, name) -> torch.Tensor:
 def _u8(t: torch.Tensor, name) -> torch.Tensor:
     if t.dtype.is_floating_point:
         raise TypeError(f"{name}: expected an integer column (got {t.dtype})")
+    if t.dtype == torch.bool:  # one byte holding 0 or 1: the BOOL8 column as it is
+        return t.contiguous().view(torch.uint8)
     return t.to(torch.uint8).contiguous()
 
 
