@@ -361,7 +361,7 @@ struct LocateArgs {
   int32_t* out_poly;
   uint8_t* out_tests;
   uint8_t* out_flags;
-  uint64_t* order;  // per point: candidate slots by rank, 4 bits each
+  uint16_t* order;  // per point: the best 3 slots (4 bits each), then their number
 };
 
 // The exact test of the quadtree path for one point and one polygon, plus whether
@@ -417,6 +417,10 @@ __device__ __forceinline__ void exact_test(PolyView const& P, uint32_t poly, dou
 }
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // D = A B + C on one 16x16x32 bf16 tile (gfx950). Lane l supplies
 // A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16] (j = 0..7) and holds
@@ -524,14 +528,18 @@ __global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
         for (int t = 0; t < kLocTiles; ++t) H[t][mt] = mfma32(w, X[t][c], H[t][mt]);
       }
     }
+    // ReLU after the conversion: a bf16 value is negative exactly when its 16 bits
+    // are as a signed integer, so a packed int16 max with 0 clamps two values per
+    // instruction (bf16(max(x, 0)) and max(bf16(x), +0) agree for every non-NaN x)
+    auto relu2 = [](float x, float y) {  // two values -> one word of bf16 pairs
+      s16x2 v = __builtin_bit_cast(s16x2, __builtin_convertvector(f32x2{x, y}, bf16x2));
+      return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(v, s16x2{0, 0}));
+    };
     auto chain = [&](f32x4 const (&h)[kHid / 16], int c) {
-      bf16x8 b;
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        b[j]     = bf(fmaxf(h[2 * c][j], 0.f));
-        b[j + 4] = bf(fmaxf(h[2 * c + 1][j], 0.f));
-      }
-      return b;
+      u32x4 const w{relu2(h[2 * c][0], h[2 * c][1]), relu2(h[2 * c][2], h[2 * c][3]),
+                    relu2(h[2 * c + 1][0], h[2 * c + 1][1]),
+                    relu2(h[2 * c + 1][2], h[2 * c + 1][3])};
+      return __builtin_bit_cast(bf16x8, w);
     };
     // layer 2
     f32x4 H2[kLocTiles][kHid / 16];
@@ -592,23 +600,29 @@ __global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
       if (g == t) c = cell[t];
     int const cnt   = a.g.count[c];
     int const limit = cnt > kCand ? 0 : (cnt < a.max_tests ? cnt : a.max_tests);
-    uint64_t order  = 0;
-    uint32_t used   = 0;
-    for (int k = 0; k < limit; ++k) {  // slots by decreasing logit, ties to the lower slot
-      int s   = -1;
-      float b = 0.f;
-      for (int q = 0; q < cnt; ++q) {
-        if ((used >> q) & 1u) continue;
-        float const zq = z[q] == z[q] ? z[q] : -INFINITY;
-        if (s < 0 || zq > b) {
-          s = q;
-          b = zq;
-        }
-      }
-      used |= 1u << s;
-      order |= static_cast<uint64_t>(s) << (4 * k);
+    // The three best slots by decreasing logit (NaN lowest, near-ties to the lower
+    // slot): each slot's key is the logit's order-preserving bits with the low 4
+    // bits replaced by 15 - slot, so keys are distinct and three max passes find
+    // them. The other slots are tested in list order after these; the order only
+    // decides how many exact tests a point needs, never the answer.
+    uint32_t key[kCand];
+#pragma unroll
+    for (int q = 0; q < kCand; ++q) {
+      float const zq   = z[q] == z[q] ? z[q] : -INFINITY;
+      uint32_t const b = __float_as_uint(zq);
+      uint32_t const o = (b >> 31) ? ~b : (b | 0x80000000u);
+      key[q]           = q < cnt ? ((o & 0xFFFFFFF0u) | static_cast<uint32_t>(15 - q)) : 0u;
     }
-    a.order[p] = order;
+    uint32_t m0 = 0, m1 = 0, m2 = 0;
+#pragma unroll
+    for (int q = 0; q < kCand; ++q) m0 = max(m0, key[q]);
+#pragma unroll
+    for (int q = 0; q < kCand; ++q) m1 = max(m1, key[q] == m0 ? 0u : key[q]);
+#pragma unroll
+    for (int q = 0; q < kCand; ++q) m2 = max(m2, (key[q] == m0 || key[q] == m1) ? 0u : key[q]);
+    uint32_t const ranked = static_cast<uint32_t>(limit < 3 ? limit : 3);
+    a.order[p] = static_cast<uint16_t>((15u - (m0 & 15u)) | ((15u - (m1 & 15u)) << 4) |
+                                       ((15u - (m2 & 15u)) << 8) | (ranked << 12));
   }
 }
 
@@ -627,23 +641,35 @@ __global__ void __launch_bounds__(256) verify_kernel(LocateArgs a)
   if (cnt > kCand) {
     flags = 1;  // overflowed cell: quadtree path
   } else {
-    uint64_t const order = a.order[p];
-    int const limit      = cnt < a.max_tests ? cnt : a.max_tests;
-    while (tests < limit) {
-      int const s = static_cast<int>((order >> (4 * tests)) & 15u);
+    uint32_t const order  = a.order[p];
+    uint32_t const ranked = order >> 12;
+    int const limit       = cnt < a.max_tests ? cnt : a.max_tests;
+    // one candidate: true when the search is over (a hit, or an error)
+    auto test = [&](int s) {
       ++tests;
       int32_t const poly = a.g.cand[static_cast<int64_t>(c) * kCand + s];
       if (poly < 0 || poly >= a.P.num_polys) {
         flags |= 1;
-        break;
+        return true;
       }
       bool inside, near;
       exact_test(a.P, static_cast<uint32_t>(poly), x, y, inside, near);
-      if (inside) {
-        if (near) flags |= 1;
-        else result = poly;
-        break;
-      }
+      if (!inside) return false;
+      if (near)
+        flags |= 1;
+      else
+        result = poly;
+      return true;
+    };
+    bool done = false;
+    // the ranked slots, then the others in list order
+    for (uint32_t i = 0; i < ranked && !done && tests < limit; ++i)
+      done = test(static_cast<int>((order >> (4 * i)) & 15u));
+    for (int q = 0; q < cnt && !done && tests < limit; ++q) {
+      bool ranked_q = false;
+      for (uint32_t i = 0; i < ranked; ++i)
+        ranked_q |= ((order >> (4 * i)) & 15u) == static_cast<uint32_t>(q);
+      if (!ranked_q) done = test(q);
     }
     if (result < 0 && !(flags & 1) && tests < cnt) flags |= 1;  // stopped at max_tests
     if (!(result >= 0 && tests == 1)) flags |= 2;  // first prediction was not the answer
@@ -959,7 +985,7 @@ extern "C" int qpip_ranker_locate(const qpip_ranker* model,
         a.P, records.get());
       QPIP_LAUNCH_CHECK();
     }
-    dbuf<uint64_t> order(num_points);
+    dbuf<uint16_t> order(num_points);
     a.order = order.get();
     int dev = 0, cus = 0;
     QPIP_HIP(hipGetDevice(&dev));
