@@ -19,7 +19,7 @@ from . import _capi
 
 # parameter -> candidates tried by default, and the timer(s) it is judged by
 PARAMETERS = {
-    "pip_min_pairs_per_wave": ([4, 8, 16, 32, 64, 256, 1024], ("pip_test",)),
+    "pip_min_pairs_per_wave": ([1, 2, 4, 8, 16, 32, 64, 256], ("pip_test",)),
     "pip_pairs_shift": ([10, 12, 14, 16, 18, 20, 24], ("pip_test",)),
 }
 

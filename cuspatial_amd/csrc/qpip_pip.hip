@@ -95,10 +95,11 @@ struct PipArgs {
   uint64_t chunk_cap;             // entries of masks / hit_counts / chunk_info
 };
 
-// Segments of at most max(m, pairs >> s) pairs (m = 16, s = 16 by default; both
+// Segments of at most max(m, pairs >> s) pairs (m = 2, s = 16 by default; both
 // are tuning parameters): few pairs (a small join, or leaves that overlap
-// thousands of polygons) are spread over many waves; with many pairs the GPU is
-// full anyway and whole runs save re-gathering a leaf's points per segment.
+// thousands of polygons) are spread over many waves, so that enough waves hide
+// the latency of the edge loads; with many pairs the GPU is full anyway and
+// long segments save re-gathering a leaf's points per segment.
 inline uint32_t pairs_per_wave(uint64_t num_pairs)
 {
   uint64_t const m = static_cast<uint64_t>(tuning(Tuning::PipMinPairsPerWave));

@@ -293,7 +293,7 @@ struct TuningParam {
   int64_t lo, hi;
 };
 TuningParam g_tuning[kNumTuning] = {
-  {"pip_min_pairs_per_wave", {16}, 1, int64_t{1} << 30},
+  {"pip_min_pairs_per_wave", {2}, 1, int64_t{1} << 30},
   {"pip_pairs_shift", {16}, 0, 40},
 };
 }  // namespace

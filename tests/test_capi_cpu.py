@@ -325,11 +325,11 @@ def test_tuning_parameters(capi):
     """The autotuner's knobs: defaults, range checks, unknown names (host only)."""
     from cuspatial_amd import autotune
 
-    assert autotune.get("pip_min_pairs_per_wave") == 16
+    assert autotune.get("pip_min_pairs_per_wave") == 2
     assert autotune.get("pip_pairs_shift") == 16
     autotune.set("pip_min_pairs_per_wave", 64)
     assert autotune.current()["pip_min_pairs_per_wave"] == 64
-    autotune.apply({"pip_min_pairs_per_wave": 16})
+    autotune.apply({"pip_min_pairs_per_wave": 2})
     with pytest.raises(RuntimeError, match="out of range"):
         autotune.set("pip_pairs_shift", 99)
     with pytest.raises(RuntimeError, match="unknown tuning parameter"):
