@@ -125,10 +125,18 @@ inline uint32_t pairs_per_wave(uint64_t num_pairs)
 //
 // Lane state is kept as wave-uniform 64-bit masks (one bit per lane and point
 // slot), so the loop's bookkeeping is a handful of scalar ops per edge.
+// Ring staging: a wave copies up to kStageEdges edges of the ring into its own LDS
+// slot with one coalesced vector load per lane (one memory latency per batch),
+// then walks them with broadcast LDS reads. Walking them from global memory took
+// a dependent scalar load (and its latency) every two edges, which dominated
+// when the edge table is not in L2: a small join right after the table is built,
+// or the batched join's 6.8 GB table.
+constexpr int kStageEdges = kWave;
+
 template <int U>
 __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, double const (&px)[4],
                                              double const (&py)[4], uint64_t const (&valid_m)[4],
-                                             uint64_t (&hit_m)[4], bool& err)
+                                             uint64_t (&hit_m)[4], bool& err, Edge* stage)
 {
   uint64_t inside_m[U], suspect_m[U];
 #pragma unroll
@@ -165,14 +173,27 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
         y0[u] = y1;
       }
     };
-    uint32_t e = 0;
-    if (U <= 2) {  // unrolled so that the loop's scalar bookkeeping is amortised
-      for (; e + 2 <= ne; e += 2) {
-        edge(E[e]);
-        edge(E[e + 1]);
+    int const lane = lane_id();
+    for (uint32_t base = 0; base < ne; base += kStageEdges) {
+      uint32_t const n = ne - base < kStageEdges ? ne - base : kStageEdges;
+      // the previous batch has been read by every lane before it is overwritten,
+      // and the new one is written before any lane reads it
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      if (static_cast<uint32_t>(lane) < n) stage[lane] = E[base + lane];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      uint32_t e = 0;
+      if (U <= 2) {  // unrolled so that the loop's scalar bookkeeping is amortised
+        for (; e + 2 <= n; e += 2) {
+          edge(stage[e]);
+          edge(stage[e + 1]);
+        }
       }
+      for (; e < n; ++e) edge(stage[e]);
     }
-    for (; e < ne; ++e) edge(E[e]);
   }
   int const lane = lane_id();
 #pragma unroll
@@ -211,6 +232,8 @@ __global__ void __launch_bounds__(256)
                   uint32_t* __restrict__ hit_counts, uint2* __restrict__ chunk_info,
                   uint32_t* __restrict__ bad)
 {
+  __shared__ Edge s_stage[256 / kWave][kStageEdges];
+  Edge* const stage = s_stage[threadIdx.x / kWave];
   uint64_t const kk = base + (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
   uint64_t const K  = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
   if (kk >= K) return;
@@ -273,10 +296,10 @@ __global__ void __launch_bounds__(256)
       uint32_t const poly = uniform(a.pair_poly[k]);
       uint64_t hit_m[4]   = {0, 0, 0, 0};
       switch (nu) {
-        case 1: test_polygon<1>(a, poly, px, py, valid_m, hit_m, err); break;
-        case 2: test_polygon<2>(a, poly, px, py, valid_m, hit_m, err); break;
-        case 3: test_polygon<3>(a, poly, px, py, valid_m, hit_m, err); break;
-        default: test_polygon<4>(a, poly, px, py, valid_m, hit_m, err); break;
+        case 1: test_polygon<1>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        case 2: test_polygon<2>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        case 3: test_polygon<3>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        default: test_polygon<4>(a, poly, px, py, valid_m, hit_m, err, stage); break;
       }
       uint64_t const slot = slot0 + static_cast<uint64_t>(k - k0) * chunks + j0;
       if (slot + nu > a.chunk_cap) {
@@ -377,6 +400,8 @@ __device__ __forceinline__ void load_point(PipArgs const& a, uint64_t i, double&
 __global__ void __launch_bounds__(256)
   pip_bitmask_kernel(PipArgs a, uint64_t n, int32_t* __restrict__ out)
 {
+  __shared__ Edge s_stage[256 / kWave][kStageEdges];
+  Edge* const stage = s_stage[threadIdx.x / kWave];
   uint64_t const w  = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   uint64_t const p0 = w * 4 * kWave;
   if (p0 >= n) return;
@@ -397,10 +422,10 @@ __global__ void __launch_bounds__(256)
   for (uint32_t j = 0; j < a.num_polys; ++j) {
     uint64_t hit_m[4] = {0, 0, 0, 0};
     switch (nu) {
-      case 1: test_polygon<1>(a, j, px, py, valid_m, hit_m, err); break;
-      case 2: test_polygon<2>(a, j, px, py, valid_m, hit_m, err); break;
-      case 3: test_polygon<3>(a, j, px, py, valid_m, hit_m, err); break;
-      default: test_polygon<4>(a, j, px, py, valid_m, hit_m, err); break;
+      case 1: test_polygon<1>(a, j, px, py, valid_m, hit_m, err, stage); break;
+      case 2: test_polygon<2>(a, j, px, py, valid_m, hit_m, err, stage); break;
+      case 3: test_polygon<3>(a, j, px, py, valid_m, hit_m, err, stage); break;
+      default: test_polygon<4>(a, j, px, py, valid_m, hit_m, err, stage); break;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) bits[u] |= static_cast<uint32_t>((hit_m[u] >> lane) & 1ull) << j;
