@@ -21,6 +21,8 @@ from . import _capi
 PARAMETERS = {
     "pip_min_pairs_per_wave": ([1, 2, 4, 8, 16, 32, 64, 256], ("pip_test",)),
     "pip_pairs_shift": ([10, 12, 14, 16, 18, 20, 24], ("pip_test",)),
+    "pip_waves_per_block": ([2, 4, 8], ("pip_test",)),
+    "pip_stage_edges": ([64, 128], ("pip_test",)),
 }
 
 

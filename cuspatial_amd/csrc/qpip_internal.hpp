@@ -63,8 +63,13 @@ struct capacity_error : std::runtime_error {
 void set_last_error(std::string const& msg);
 
 // Runtime tuning parameters (qpip_tuning_set / qpip_tuning_get), read at launch.
-enum class Tuning : int { PipMinPairsPerWave = 0, PipPairsShift = 1 };
-constexpr int kNumTuning = 2;
+enum class Tuning : int {
+  PipMinPairsPerWave = 0,
+  PipPairsShift      = 1,
+  PipWavesPerBlock   = 2,
+  PipStageEdges      = 3
+};
+constexpr int kNumTuning = 4;
 int64_t tuning(Tuning t);
 
 // ----------------------------------------------------------------------------

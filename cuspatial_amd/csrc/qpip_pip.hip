@@ -95,11 +95,11 @@ struct PipArgs {
   uint64_t chunk_cap;             // entries of masks / hit_counts / chunk_info
 };
 
-// Segments of at most max(m, pairs >> s) pairs (m = 2, s = 16 by default; both
-// are tuning parameters): few pairs (a small join, or leaves that overlap
-// thousands of polygons) are spread over many waves, so that enough waves hide
-// the latency of the edge loads; with many pairs the GPU is full anyway and
-// long segments save re-gathering a leaf's points per segment.
+// Segments of at most max(m, pairs >> s) pairs (m = 2, s = 24 by default; both
+// are tuning parameters): short segments spread the pairs over many waves, so
+// that enough waves hide the latency of the point gathers and edge loads; a
+// large s keeps segments at m pairs up to 2^25 pairs, and past that bounds the
+// number of waves (each segment re-gathers its leaf's points).
 inline uint32_t pairs_per_wave(uint64_t num_pairs)
 {
   uint64_t const m = static_cast<uint64_t>(tuning(Tuning::PipMinPairsPerWave));
@@ -131,9 +131,11 @@ inline uint32_t pairs_per_wave(uint64_t num_pairs)
 // a dependent scalar load (and its latency) every two edges, which dominated
 // when the edge table is not in L2: a small join right after the table is built,
 // or the batched join's 6.8 GB table.
-constexpr int kStageEdges = kWave;
+// The depth S (edges per batch, 64 or 128) and the waves per workgroup are runtime
+// tuning parameters (pip_stage_edges, pip_waves_per_block).
+constexpr int kStageEdges = kWave;  // the default depth, and the bitmask kernel's
 
-template <int U>
+template <int U, int S = kStageEdges>
 __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, double const (&px)[4],
                                              double const (&py)[4], uint64_t const (&valid_m)[4],
                                              uint64_t (&hit_m)[4], bool& err, Edge* stage)
@@ -174,14 +176,16 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
       }
     };
     int const lane = lane_id();
-    for (uint32_t base = 0; base < ne; base += kStageEdges) {
-      uint32_t const n = ne - base < kStageEdges ? ne - base : kStageEdges;
+    for (uint32_t base = 0; base < ne; base += S) {
+      uint32_t const n = ne - base < S ? ne - base : S;
       // the previous batch has been read by every lane before it is overwritten,
       // and the new one is written before any lane reads it
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      if (static_cast<uint32_t>(lane) < n) stage[lane] = E[base + lane];
+#pragma unroll
+      for (int i = 0; i < S; i += kWave)
+        if (static_cast<uint32_t>(i + lane) < n) stage[i + lane] = E[base + i + lane];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -227,14 +231,15 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
 // kPipPairsPerLaunch pairs, so a grid never exceeds 2^28 work-items.
 constexpr uint64_t kPipPairsPerLaunch = uint64_t{1} << 22;
 
-__global__ void __launch_bounds__(256)
+template <int W, int S>  // W waves per workgroup, S edges per staged batch
+__global__ void __launch_bounds__(W * kWave)
   pip_test_kernel(PipArgs a, uint64_t base, uint64_t* __restrict__ masks,
                   uint32_t* __restrict__ hit_counts, uint2* __restrict__ chunk_info,
                   uint32_t* __restrict__ bad)
 {
-  __shared__ Edge s_stage[256 / kWave][kStageEdges];
+  __shared__ Edge s_stage[W][S];
   Edge* const stage = s_stage[threadIdx.x / kWave];
-  uint64_t const kk = base + (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
+  uint64_t const kk = base + (blockIdx.x * static_cast<uint64_t>(W) + threadIdx.x / kWave);
   uint64_t const K  = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
   if (kk >= K) return;
   uint32_t const k0 = uniform(static_cast<uint32_t>(kk));
@@ -296,10 +301,10 @@ __global__ void __launch_bounds__(256)
       uint32_t const poly = uniform(a.pair_poly[k]);
       uint64_t hit_m[4]   = {0, 0, 0, 0};
       switch (nu) {
-        case 1: test_polygon<1>(a, poly, px, py, valid_m, hit_m, err, stage); break;
-        case 2: test_polygon<2>(a, poly, px, py, valid_m, hit_m, err, stage); break;
-        case 3: test_polygon<3>(a, poly, px, py, valid_m, hit_m, err, stage); break;
-        default: test_polygon<4>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        case 1: test_polygon<1, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        case 2: test_polygon<2, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        case 3: test_polygon<3, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        default: test_polygon<4, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
       }
       uint64_t const slot = slot0 + static_cast<uint64_t>(k - k0) * chunks + j0;
       if (slot + nu > a.chunk_cap) {
@@ -353,17 +358,35 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Launches pip_test_kernel over pairs [0, pairs) in slices.
-void launch_pip_test(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
-                     uint2* chunk_info, uint32_t* bad, hipStream_t stream)
+// Launches pip_test_kernel over pairs [0, pairs) in slices, with the tuned
+// workgroup size (2, 4 or 8 waves) and staging depth (64 or 128 edges).
+template <int W, int S>
+void launch_pip_test_ws(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
+                        uint2* chunk_info, uint32_t* bad, hipStream_t stream)
 {
   for (uint64_t b = 0; b < pairs; b += kPipPairsPerLaunch) {
     uint64_t const n = std::min<uint64_t>(pairs - b, kPipPairsPerLaunch);
-    pip_test_kernel<<<static_cast<unsigned>((n + 3) / 4), 256, 0, stream>>>(a, b, masks,
-                                                                            hit_counts, chunk_info,
-                                                                            bad);
+    pip_test_kernel<W, S><<<static_cast<unsigned>((n + W - 1) / W), W * kWave, 0, stream>>>(
+      a, b, masks, hit_counts, chunk_info, bad);
     QPIP_LAUNCH_CHECK();
   }
+}
+
+void launch_pip_test(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
+                     uint2* chunk_info, uint32_t* bad, hipStream_t stream)
+{
+  int64_t const w = tuning(Tuning::PipWavesPerBlock);
+  bool const deep = tuning(Tuning::PipStageEdges) > 64;
+#define QPIP_PIP_TEST(W)                                                                       \
+  (deep ? launch_pip_test_ws<W, 2 * kWave>(a, pairs, masks, hit_counts, chunk_info, bad, stream) \
+        : launch_pip_test_ws<W, kWave>(a, pairs, masks, hit_counts, chunk_info, bad, stream))
+  if (w <= 2)
+    QPIP_PIP_TEST(2);
+  else if (w <= 4)
+    QPIP_PIP_TEST(4);
+  else
+    QPIP_PIP_TEST(8);
+#undef QPIP_PIP_TEST
 }
 
 // *out = min(*in, cap); bit 2 of *flags when *in exceeds cap.

@@ -294,7 +294,9 @@ struct TuningParam {
 };
 TuningParam g_tuning[kNumTuning] = {
   {"pip_min_pairs_per_wave", {2}, 1, int64_t{1} << 30},
-  {"pip_pairs_shift", {16}, 0, 40},
+  {"pip_pairs_shift", {24}, 0, 40},
+  {"pip_waves_per_block", {2}, 1, 16},  // PIP test workgroup: 2, 4 or 8 waves (rounded up)
+  {"pip_stage_edges", {64}, 1, 128},    // PIP ring staging depth: 64 or 128 edges
 };
 }  // namespace
 

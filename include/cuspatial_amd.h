@@ -71,9 +71,13 @@ QPIP_API void qpip_trim(void);
 /* Tuning parameters (no counterpart in the reference; set by the autotuner,
  * cuspatial_amd/autotune.py). Process-wide, read when a kernel is launched:
  *   pip_min_pairs_per_wave  shortest segment of a leaf's pairs that one wave of
- *                           the point-in-polygon test takes (default 16)
+ *                           the point-in-polygon test takes (default 2)
  *   pip_pairs_shift         segments are at most max(min, pairs >> shift) pairs
- *                           long (default 16)
+ *                           long (default 24)
+ *   pip_waves_per_block     waves per workgroup of that test: 2, 4 or 8 (values
+ *                           round up to one of these; default 2)
+ *   pip_stage_edges         ring edges a wave stages in LDS per batch: 64 or 128
+ *                           (values above 64 select 128; default 64)
  * Results do not depend on them, only the time. Errors: unknown name, value out
  * of range. */
 QPIP_API int qpip_tuning_set(const char* name, int64_t value);

@@ -326,7 +326,11 @@ def test_tuning_parameters(capi):
     from cuspatial_amd import autotune
 
     assert autotune.get("pip_min_pairs_per_wave") == 2
-    assert autotune.get("pip_pairs_shift") == 16
+    assert autotune.get("pip_pairs_shift") == 24
+    assert autotune.get("pip_waves_per_block") == 2
+    assert autotune.get("pip_stage_edges") == 64
+    with pytest.raises(RuntimeError, match="out of range"):
+        autotune.set("pip_stage_edges", 129)
     autotune.set("pip_min_pairs_per_wave", 64)
     assert autotune.current()["pip_min_pairs_per_wave"] == 64
     autotune.apply({"pip_min_pairs_per_wave": 2})
@@ -334,7 +338,7 @@ def test_tuning_parameters(capi):
         autotune.set("pip_pairs_shift", 99)
     with pytest.raises(RuntimeError, match="unknown tuning parameter"):
         autotune.set("no_such_knob", 1)
-    assert autotune.get("pip_pairs_shift") == 16
+    assert autotune.get("pip_pairs_shift") == 24
 
 
 def test_batch_plan_preconditions(capi):

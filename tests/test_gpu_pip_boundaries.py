@@ -125,3 +125,23 @@ def test_boundaries_match_bitmask(cs, oracle_mod):
     assert np.array_equal(bits, bm.cpu().numpy().astype(np.int64))
     ob = oracle_mod.point_in_polygon_bitmask(x, y, part, ring, vx, vy)
     assert np.array_equal(bits, np.asarray(ob).astype(np.int64))
+
+
+@pytest.mark.parametrize("waves,stage,min_pairs", [(2, 64, 2), (8, 64, 2), (4, 128, 2),
+                                                   (8, 128, 1), (2, 128, 16)])
+def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs):
+    """The PIP test's workgroup size, staging depth and segment length change only
+    the time: every setting gives the oracle's pairs on the boundary scene."""
+    from cuspatial_amd import autotune
+
+    x, y, polys = scene(21)
+    bbox = (0.0, 64.0, 0.0, 64.0)
+    saved = autotune.current()
+    try:
+        autotune.apply({"pip_waves_per_block": waves, "pip_stage_edges": stage,
+                        "pip_min_pairs_per_wave": min_pairs})
+        g = run_gpu(cs, x, y, bbox, 64.0 / 64, 6, 8, polys)
+    finally:
+        autotune.apply(saved)
+    o = run_oracle(oracle_mod, x, y, bbox, 64.0 / 64, 6, 8, polys)
+    assert_same(g, o)
