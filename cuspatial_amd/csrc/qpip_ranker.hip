@@ -394,8 +394,19 @@ __device__ __forceinline__ void exact_test(PolyView const& P, uint32_t poly, dou
   if (rec.ne != kMultiRing) {
     bool inside = false, on_edge = false, near = false;
     bool y0 = rec.y0 > py;
-    for (uint32_t e = 0; e < rec.ne; ++e)
-      edge_step(P.edges[rec.e0 + e], P.edges_b + rec.e0 + e, px, py, y0, inside, on_edge, near);
+    // four edges' loads in flight per wait (a lane's edges are its own, so each
+    // load is a round trip to L2; vector loads complete in order)
+    Edge const* E = P.edges + rec.e0;
+    EdgeB const* B = P.edges_b + rec.e0;
+    uint32_t e = 0;
+    for (; e + 4 <= rec.ne; e += 4) {
+      Edge const e0_ = E[e], e1_ = E[e + 1], e2_ = E[e + 2], e3_ = E[e + 3];
+      edge_step(e0_, B + e, px, py, y0, inside, on_edge, near);
+      edge_step(e1_, B + e + 1, px, py, y0, inside, on_edge, near);
+      edge_step(e2_, B + e + 2, px, py, y0, inside, on_edge, near);
+      edge_step(e3_, B + e + 3, px, py, y0, inside, on_edge, near);
+    }
+    for (; e < rec.ne; ++e) edge_step(E[e], B + e, px, py, y0, inside, on_edge, near);
     inside_out = inside && !on_edge;
     near_out   = near;
     return;
@@ -633,6 +644,7 @@ __global__ void __launch_bounds__(256) verify_kernel(LocateArgs a)
   if (p >= a.n) return;
   double const x  = a.xy[2 * p];
   double const y  = a.xy[2 * p + 1];
+  uint32_t const order = a.order[p];  // issued with the point, before the cell lookup
   uint32_t const c = locate_cell(a.g, x, y).cell;
   int const cnt    = a.g.count[c];
   int32_t result   = -1;
@@ -641,7 +653,6 @@ __global__ void __launch_bounds__(256) verify_kernel(LocateArgs a)
   if (cnt > kCand) {
     flags = 1;  // overflowed cell: quadtree path
   } else {
-    uint32_t const order  = a.order[p];
     uint32_t const ranked = order >> 12;
     int const limit       = cnt < a.max_tests ? cnt : a.max_tests;
     // one candidate: true when the search is over (a hit, or an error)
