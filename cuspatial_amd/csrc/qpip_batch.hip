@@ -1515,15 +1515,8 @@ extern "C" int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
     *point_index   = o_point;
     *num_hits      = static_cast<int64_t>(H);
     return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }
 
@@ -1531,28 +1524,6 @@ extern "C" int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
 // Captured plan entry points
 // ---------------------------------------------------------------------------------
 namespace {
-
-template <class F>
-int plan_guarded(F&& f)
-{
-  using namespace qpip;
-  try {
-    f();
-    return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (capacity_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_CAPACITY;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
-  }
-}
 
 qpip::BatchPlan* plan_of(qpip_batch_plan* p) { return reinterpret_cast<qpip::BatchPlan*>(p); }
 
@@ -1587,7 +1558,7 @@ extern "C" int qpip_batch_plan_create(int64_t num_jobs,
                                       qpip_batch_plan** plan)
 {
   using namespace qpip;
-  return plan_guarded([&] {
+  return qpip::guarded([&] {
     QPIP_EXPECTS(plan != nullptr, "output pointers must not be null");
     *plan = nullptr;
     QPIP_EXPECTS(num_jobs >= 1, "a plan needs at least one job");
@@ -1735,7 +1706,7 @@ extern "C" int qpip_batch_plan_create(int64_t num_jobs,
 extern "C" int qpip_batch_plan_launch(qpip_batch_plan* plan, void* stream_)
 {
   using namespace qpip;
-  return plan_guarded([&] {
+  return qpip::guarded([&] {
     QPIP_EXPECTS(plan != nullptr, "plan must not be null");
     BatchPlan& P = *plan_of(plan);
     check_device(P);
@@ -1757,7 +1728,7 @@ extern "C" int qpip_batch_plan_result(qpip_batch_plan* plan,
                                       int64_t* num_hits)
 {
   using namespace qpip;
-  return plan_guarded([&] {
+  return qpip::guarded([&] {
     QPIP_EXPECTS(plan != nullptr, "plan must not be null");
     QPIP_EXPECTS(point_indices && polygon_index && point_index && job_hit_offsets && num_hits,
                  "output pointers must not be null");
@@ -1811,5 +1782,5 @@ extern "C" int qpip_batch_plan_result(qpip_batch_plan* plan,
 extern "C" int qpip_batch_plan_destroy(qpip_batch_plan* plan)
 {
   using namespace qpip;
-  return plan_guarded([&] { delete plan_of(plan); });
+  return qpip::guarded([&] { delete plan_of(plan); });
 }

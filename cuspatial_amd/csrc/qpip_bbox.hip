@@ -196,14 +196,7 @@ extern "C" int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
     *maxy      = o_maxy;
     *num_boxes = num_polys;
     return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }

@@ -62,6 +62,43 @@ struct capacity_error : std::runtime_error {
 
 void set_last_error(std::string const& msg);
 
+// Records the message of the exception in flight and returns its status code.
+// Call only inside a catch block: every C entry point ends with
+// `catch (...) { return qpip::error_status(); }`.
+inline int error_status()
+{
+  try {
+    throw;
+  } catch (logic_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_LOGIC;
+  } catch (oom_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_OOM;
+  } catch (capacity_error const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_CAPACITY;
+  } catch (std::exception const& e) {
+    set_last_error(e.what());
+    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    set_last_error("cuSpatial failure: unknown exception");
+    return QPIP_ERROR_DEVICE;
+  }
+}
+
+// Runs f and returns QPIP_SUCCESS, or the status code of what it threw.
+template <class F>
+int guarded(F&& f)
+{
+  try {
+    f();
+    return QPIP_SUCCESS;
+  } catch (...) {
+    return error_status();
+  }
+}
+
 // Runtime tuning parameters (qpip_tuning_set / qpip_tuning_get), read at launch.
 enum class Tuning : int {
   PipMinPairsPerWave = 0,

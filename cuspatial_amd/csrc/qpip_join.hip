@@ -248,14 +248,7 @@ extern "C" int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
     *quad_offset = o_quad;
     *num_pairs   = static_cast<int64_t>(K);
     return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }

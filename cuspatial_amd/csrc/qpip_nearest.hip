@@ -297,15 +297,8 @@ extern "C" int qpip_quadtree_point_to_nearest_linestring(const uint32_t* linestr
     *distance                 = o_dist;
     *num_rows                 = n;
     return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }
 
@@ -346,14 +339,7 @@ extern "C" int qpip_linestring_bounding_boxes(const int32_t* linestring_offsets,
                                                expansion_radius, mr, stream_, minx, miny, maxx,
                                                maxy, num_boxes);
     return rc;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }

@@ -671,15 +671,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     *point_index   = o_point;
     *num_hits      = static_cast<int64_t>(H);
     return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }
 
@@ -840,15 +833,8 @@ int point_in_polygon_entry(const double* x,
     QPIP_HIP(hipStreamSynchronize(stream));
     *num_results = num_x;
     return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }
 

@@ -469,15 +469,8 @@ int quadtree_on_points_entry(const double* x,
                                          static_cast<hipStream_t>(stream), point_indices, key,
                                          level, is_internal_node, length, offset);
     return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
+  } catch (...) {
+    return qpip::error_status();
   }
 }
 

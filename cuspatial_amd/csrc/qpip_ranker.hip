@@ -768,24 +768,6 @@ void forward(qpip_ranker const* m, GridView const& g, double const* xy, int64_t 
   launch_gemm(l3, stream);
 }
 
-template <class F>
-int guarded(F&& f)
-{
-  try {
-    f();
-    return QPIP_SUCCESS;
-  } catch (logic_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_LOGIC;
-  } catch (oom_error const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_OOM;
-  } catch (std::exception const& e) {
-    set_last_error(e.what());
-    return QPIP_ERROR_DEVICE;
-  }
-}
-
 int64_t round16(int64_t n) { return ((n + 15) / 16) * 16; }
 
 }  // namespace
