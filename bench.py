@@ -576,6 +576,7 @@ def bench_learned(args, dev):
     launches = max(prof.get("ranker_rank", (1, 0.0))[0], 1)
     loc_ms = prof.get("ranker_rank", (0, 0.0))[1] / launches
     flops = 2.0 * (96 * 128 + 128 * 128 + 128 * 16) * n
+    by_size = learned_vs_plain_by_batch(polys, loc, bbox, w, xy, n)
     kernels = {k: {"launches": c, "ms_per_launch": round(v / max(c, 1), 6)}
                for k, (c, v) in prof.items()}
     line = {
@@ -610,8 +611,49 @@ def bench_learned(args, dev):
                      "traffic": None},
         "kernels": kernels,
         "p_ms": {"min": 1e3 * min(times), "max": 1e3 * max(times)},
+        "by_batch_points": by_size,
     }
     print(json.dumps(line), flush=True)
+
+
+def learned_vs_plain_by_batch(polys, loc, bbox, w, xy, n):
+    """One request's latency by batch size, as a serving index answers it: the
+    first b points of the workload located by the learned locator and by the
+    plain join with the polygon boxes resident (serving.ResidentIndex.query, both
+    sorted by point, then polygon). Median of 9 timed calls after 3 warm-up
+    calls; the two answers are compared exactly."""
+    import torch
+
+    from cuspatial_amd import serving
+
+    learned = serving.ResidentIndex(polys, bbox, w.scale, w.max_depth, w.max_size, locator=loc)
+    plain = serving.ResidentIndex(polys, bbox, w.scale, w.max_depth, w.max_size)
+
+    def median_ms(index, sub):
+        for _ in range(3):
+            index.query(sub)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(9):
+            t0 = time.perf_counter()
+            index.query(sub)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return 1e3 * sorted(ts)[len(ts) // 2]
+
+    rows = []
+    for b in (1_000, 10_000, 100_000, 1_000_000, 10_000_000, 30_000_000):
+        if b > n:
+            break
+        sub = xy[: 2 * b]
+        lms, pms = median_ms(learned, sub), median_ms(plain, sub)
+        a, c = learned.query(sub), plain.query(sub)
+        same = bool(all(torch.equal(u, v) for u, v in zip(a, c)))
+        rows.append({"points": b, "learned_ms": lms, "plain_ms": pms, "speedup": pms / lms,
+                     "identical": same})
+        log(f"[bench] batch of {b} points: learned {lms:.3f} ms, plain {pms:.3f} ms "
+            f"({pms / lms:.2f}x), identical: {same}")
+    return rows
 
 
 def bench_serving(args, dev):
