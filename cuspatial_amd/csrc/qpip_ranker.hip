@@ -459,7 +459,10 @@ __host__ __device__ __forceinline__ int chained_col(int c, int g, int j)
 
 __device__ __forceinline__ __bf16 bf(float f) { return static_cast<__bf16>(f); }
 
-__global__ void __launch_bounds__(kLocWaves * kWave) rank_kernel(LocateArgs a)
+// Two waves per SIMD (at most 256 registers, which also keeps the accumulators in
+// VGPRs: no AGPR copies before the bf16 conversions), as many as two workgroups'
+// LDS allows: one wave's feature and conversion work overlaps the other's MFMAs.
+__global__ void __launch_bounds__(kLocWaves * kWave, 2) rank_kernel(LocateArgs a)
 {
   __shared__ __attribute__((aligned(16))) uint16_t w_lds[kFrags * kWave * kFragElems];
   __shared__ float b_lds[kHid + kHid + kCand];
