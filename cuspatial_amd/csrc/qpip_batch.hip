@@ -80,6 +80,34 @@ __device__ __forceinline__ uint32_t find_job(JobDev const* jobs, uint32_t lo, ui
   return lo;
 }
 
+// find_job by a whole wave (every lane calls it; all get the answer): each round
+// probes 64 evenly spaced jobs of [lo, hi] at once and keeps the stretch between
+// the last probe whose point_begin <= pos and the next probe, so 4096 jobs take 2
+// or 3 dependent rounds instead of 12. point_begin never decreases with the job
+// index, so the probes' predicate holds for a prefix of the lanes.
+__device__ __forceinline__ uint32_t wave_find_job(JobDev const* jobs, uint32_t lo, uint32_t hi,
+                                                  uint32_t pos)
+{
+  int const lane = lane_id();
+  while (lo < hi) {
+    uint64_t const span = hi - lo;
+    // probe(l) = lo + ceil((l + 1) * span / 64): non-decreasing in l, probe(63) = hi
+    auto probe = [&](int l) {
+      return lo + static_cast<uint32_t>((static_cast<uint64_t>(l + 1) * span + kWave - 1) / kWave);
+    };
+    bool const ok = jobs[probe(lane)].point_begin <= pos;
+    int const c   = __popcll(ballot(ok));  // lanes [0, c) hold
+    if (c == 0) {
+      hi = probe(0) - 1;
+    } else {
+      uint32_t const nlo = probe(c - 1);
+      hi                 = c < kWave ? probe(c) - 1 : hi;
+      lo                 = nlo;
+    }
+  }
+  return lo;
+}
+
 // --- keys ----------------------------------------------------------------------------
 // One block per radix-sort tile; the tile's jobs are found once per block, each
 // point's job by a search among them. Also counts the sort's first digit.
@@ -445,22 +473,25 @@ __global__ void __launch_bounds__(256)
 }
 
 // --- results back to job-local indices ---------------------------------------------
-// Tiles of 1024 sorted positions per block iteration: the tile's first and last
-// jobs are found once, each position's job by a search between them.
+// Tiles of 4096 sorted positions per block iteration: the tile's first and last
+// jobs are found once (waves 0 and 1 search in parallel), each position's job by
+// a search between them.
 __global__ void __launch_bounds__(256)
   local_point_indices_kernel(uint32_t const* __restrict__ sorted_vals, uint64_t n,
                              JobDev const* __restrict__ jobs, uint32_t num_jobs,
                              uint32_t* __restrict__ out)
 {
-  constexpr uint64_t kTile = 1024;
+  constexpr uint64_t kTile = 4096;
   __shared__ uint32_t s_jobs[2];
+  int const w = threadIdx.x / kWave;
   for (uint64_t lo = static_cast<uint64_t>(blockIdx.x) * kTile; lo < n;
        lo += static_cast<uint64_t>(gridDim.x) * kTile) {
     uint64_t const hi = lo + kTile < n ? lo + kTile : n;
     __syncthreads();  // the previous tile's readers of s_jobs are done
-    if (threadIdx.x == 0) {
-      s_jobs[0] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(lo));
-      s_jobs[1] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(hi - 1));
+    if (w < 2) {  // wave-uniform
+      uint32_t const j = wave_find_job(jobs, 0, num_jobs - 1,
+                                       static_cast<uint32_t>(w == 0 ? lo : hi - 1));
+      if (lane_id() == 0) s_jobs[w] = j;
     }
     __syncthreads();
     uint32_t const j0 = s_jobs[0], j1 = s_jobs[1];
@@ -832,8 +863,9 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
     QPIP_LAUNCH_CHECK();
   }
   // point_indices (job-local)
-  local_point_indices_kernel<<<grid_for(n, 256), 256, 0, stream>>>(vals.get(), n, jobs.get(), J,
-                                                                    out_pi);
+  // one block per 4096-position tile
+  local_point_indices_kernel<<<grid_for(n, 4096), 256, 0, stream>>>(vals.get(), n, jobs.get(), J,
+                                                                     out_pi);
   QPIP_LAUNCH_CHECK();
   if (M == 0 || npg == 0) {
     QPIP_HIP(hipStreamSynchronize(stream));
@@ -1248,8 +1280,8 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
     g.ccount.get(), g.jobs.get(), g.trees.get(), 0, g.fmeta.get(), levels, g.f_internal.get(),
     g.f_length.get(), g.f_offset.get());
   QPIP_LAUNCH_CHECK();
-  local_point_indices_kernel<<<grid_for(g.n, 256), 256, 0, s>>>(g.vals.get(), g.n, g.jobs.get(),
-                                                               g.J, P.out_pi.get() + g.P0);
+  local_point_indices_kernel<<<grid_for(g.n, 4096), 256, 0, s>>>(g.vals.get(), g.n, g.jobs.get(),
+                                                                g.J, P.out_pi.get() + g.P0);
   QPIP_LAUNCH_CHECK();
   if (!g.join) {
     plan_empty_kernel<<<grid_for(g.J, 256), 256, 0, s>>>(cnt, prev, g.fmeta.get(), job_first,
