@@ -18,16 +18,17 @@
 //   2. Chunks. Pair k covers ceil(length / 64) chunks of 64 consecutive point
 //      positions; a scan gives each pair its first chunk, and a map gives each
 //      chunk its pair (get_quad_and_local_point_indices.cuh:26-45 searches for it).
-//   3. Test: one wave per leaf, for all the pairs of that leaf (they are adjacent
-//      in pair order), up to 256 points (4 chunks) at a time. The lanes read
+//   3. Test: one wave per segment of a leaf's pairs (they are adjacent in pair
+//      order), up to 256 points (4 chunks) at a time. The lanes read
 //      consecutive point_indices entries (coalesced) and gather the coordinates
-//      once; each polygon's edges are wave-uniform and are read through the
-//      scalar cache. The FP64 arithmetic is
-//      the reference's; the 4-ULP collinearity test is preceded by an exact
+//      once; each polygon's edges are wave-uniform and are staged through the
+//      wave's LDS slot in coalesced batches. The FP64 arithmetic is the
+//      reference's; the 4-ULP collinearity test is preceded by an exact
 //      floating-point filter that rejects pairs of products too far apart to be
 //      within 4 ULP. Each wave stores its 64-bit hit ballot and its popcount.
 //   4. Scan of hit counts -> output positions; one host sync for the total.
-//   5. Write: each wave expands its ballot into (polygon, point position) pairs.
+//   5. Write: each wave expands the ballots of 64 chunks into (polygon, point
+//      position) pairs.
 //   No candidate-sized buffer is ever allocated, so the reference's out-of-memory
 //   fallback (count, then allocate; :220-234) is the only path here.
 #include <algorithm>
@@ -331,8 +332,21 @@ struct HitScanOp {
 };
 
 // --- 5. write -------------------------------------------------------------------------
+// A wave owns 64 consecutive chunks: lane i loads chunk c0 + i's mask, and for a
+// non-empty one its first hit and info, in coalesced loads (one round trip for
+// 64 chunks instead of one wave, and two dependent round trips, per chunk). The
+// wave then writes the non-empty chunks one at a time, each chunk's values
+// broadcast from its lane; a chunk's hits are its set lanes, in lane order.
 // With num_chunks_dev the chunk count lives on the device, and hits at or past
 // hit_cap are dropped and flagged (bit 2 of *bad) instead of written.
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int j)
+{
+  uint32_t const lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), j));
+  uint32_t const hi =
+    static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), j));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 __global__ void __launch_bounds__(256)
   pip_write_kernel(uint64_t num_chunks, uint64_t const* __restrict__ masks,
                    uint2 const* __restrict__ chunk_info, uint64_t const* __restrict__ hit_start,
@@ -342,17 +356,31 @@ __global__ void __launch_bounds__(256)
   uint64_t const T     = num_chunks_dev ? *num_chunks_dev : num_chunks;
   uint64_t const waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
   int const l          = lane_id();
-  for (uint64_t c = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; c < T; c += waves) {
-    uint64_t const mask = masks[c];
-    if (mask == 0) continue;
-    uint2 const info = chunk_info[c];
-    if ((mask >> l) & 1ull) {
-      uint64_t const o = hit_start[c] + static_cast<uint64_t>(__popcll(mask & lanemask_lt()));
-      if (o < hit_cap) {
-        out_poly[o]  = info.x;
-        out_point[o] = info.y + static_cast<uint32_t>(l);
-      } else {
-        atomicOr(bad, 2u);
+  uint64_t const lt    = lanemask_lt();
+  for (uint64_t c0 = (blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave) * kWave; c0 < T;
+       c0 += waves * kWave) {
+    uint64_t const c = c0 + static_cast<uint64_t>(l);
+    uint64_t mask = 0, start = 0;
+    uint2 info    = make_uint2(0u, 0u);
+    if (c < T) mask = masks[c];
+    if (mask != 0) {
+      start = hit_start[c];
+      info  = chunk_info[c];
+    }
+    uint64_t pending = ballot(mask != 0);
+    while (pending) {  // wave-uniform
+      int const j = __builtin_ctzll(pending);
+      pending &= pending - 1;
+      uint64_t const m = readlane64(mask, j);
+      if ((m >> l) & 1ull) {
+        uint64_t const o = readlane64(start, j) + static_cast<uint64_t>(__popcll(m & lt));
+        if (o < hit_cap) {
+          out_poly[o]  = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(info.x), j));
+          out_point[o] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(info.y), j)) +
+                         static_cast<uint32_t>(l);
+        } else {
+          atomicOr(bad, 2u);
+        }
       }
     }
   }
@@ -531,7 +559,8 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
   QPIP_LAUNCH_CHECK();
   {
     KernelTimer kt("pip_write", stream);
-    pip_write_kernel<<<grid_for(d.chunk_cap, 4), 256, 0, stream>>>(
+    // 4 waves per block, 64 chunks per wave
+    pip_write_kernel<<<grid_for(d.chunk_cap, 4 * kWave), 256, 0, stream>>>(
       0, d.masks, d.chunk_info, d.hit_start, d.out_poly, d.out_pos, n_chunks, d.hit_cap, flags);
     QPIP_LAUNCH_CHECK();
   }
@@ -661,7 +690,7 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     uint32_t* o_point = oa.make<uint32_t>(H);
     {
       KernelTimer kt("pip_write", stream);
-      pip_write_kernel<<<grid_for(T, 4), 256, 0, stream>>>(T, masks.get(), chunk_info.get(),
+      pip_write_kernel<<<grid_for(T, 4 * kWave), 256, 0, stream>>>(T, masks.get(), chunk_info.get(),
                                                    hit_start.get(), o_poly, o_point, nullptr, H,
                                                    bad.get());
       QPIP_LAUNCH_CHECK();
