@@ -502,18 +502,45 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Job-local (polygon, point) of hits [0, h), written at rows base + i. Hits are in
+// job order (pairs follow the leaves' sorted positions), so a tile's first and
+// last hits bound its jobs: per 4096-hit tile, waves 0 and 1 find those two jobs
+// in parallel, and each hit's job is searched between them (usually equal). h is
+// block-uniform, so every thread reaches every barrier.
+__device__ __forceinline__ void local_hit_tiles(uint32_t const* __restrict__ poly,
+                                                uint32_t const* __restrict__ pos, uint64_t h,
+                                                uint64_t base, JobDev const* __restrict__ jobs,
+                                                uint32_t num_jobs, uint32_t* __restrict__ out_poly,
+                                                uint32_t* __restrict__ out_point)
+{
+  constexpr uint64_t kTile = 4096;
+  __shared__ uint32_t s_jobs[2];
+  int const w = threadIdx.x / kWave;
+  for (uint64_t lo = static_cast<uint64_t>(blockIdx.x) * kTile; lo < h;
+       lo += static_cast<uint64_t>(gridDim.x) * kTile) {
+    uint64_t const hi = lo + kTile < h ? lo + kTile : h;
+    __syncthreads();  // the previous tile's readers of s_jobs are done
+    if (w < 2) {  // wave-uniform
+      uint32_t const j = wave_find_job(jobs, 0, num_jobs - 1, pos[w == 0 ? lo : hi - 1]);
+      if (lane_id() == 0) s_jobs[w] = j;
+    }
+    __syncthreads();
+    uint32_t const j0 = s_jobs[0], j1 = s_jobs[1];
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+      uint32_t const p = pos[i];
+      uint32_t const j = j0 == j1 ? j0 : find_job(jobs, j0, j1, p);
+      out_poly[base + i]  = poly[i] - jobs[j].poly_begin;
+      out_point[base + i] = p - jobs[j].point_begin;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(256)
   local_hits_kernel(uint32_t const* __restrict__ poly, uint32_t const* __restrict__ pos,
                     uint64_t h, JobDev const* __restrict__ jobs, uint32_t num_jobs,
                     uint32_t* __restrict__ out_poly, uint32_t* __restrict__ out_point)
 {
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint32_t const p = pos[i];
-    uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
-    out_poly[i]      = poly[i] - jobs[j].poly_begin;
-    out_point[i]     = p - jobs[j].point_begin;
-  }
+  local_hit_tiles(poly, pos, h, 0, jobs, num_jobs, out_poly, out_point);
 }
 
 // First hit of every job (hits are in job order): first h with pos[h] >= point_begin.
@@ -986,8 +1013,9 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   res.poly.reset(h);
   res.point.reset(h);
   res.hits = h;
-  local_hits_kernel<<<grid_for(h, 256), 256, 0, stream>>>(hp, ht, h, jobs.get(), J, res.poly.get(),
-                                                          res.point.get());
+  // one block per 4096-hit tile
+  local_hits_kernel<<<grid_for(h, 4096), 256, 0, stream>>>(hp, ht, h, jobs.get(), J,
+                                                           res.poly.get(), res.point.get());
   QPIP_LAUNCH_CHECK();
   dbuf<uint64_t> first(J);
   job_hit_offsets_kernel<<<(J + 255) / 256, 256, 0, stream>>>(ht, h, jobs.get(), J, first.get());
@@ -1076,19 +1104,15 @@ __global__ void __launch_bounds__(256)
                          uint64_t out_cap, uint32_t* __restrict__ out_poly,
                          uint32_t* __restrict__ out_point)
 {
-  uint64_t const h = cnt->hits, base = cnt->hit_base;
-  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < h;
-       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
-    uint64_t const o = base + i;
-    if (o >= out_cap) {
-      atomicOr(&cnt->flags, 2u);
-      return;
-    }
-    uint32_t const p = pos[i];
-    uint32_t const j = find_job(jobs, 0, num_jobs - 1, p);
-    out_poly[o]      = poly[i] - jobs[j].poly_begin;
-    out_point[o]     = p - jobs[j].point_begin;
+  uint64_t const base = cnt->hit_base;
+  uint64_t h          = cnt->hits;
+  // rows past out_cap are dropped and flagged; the count stays block-uniform
+  uint64_t const room = out_cap > base ? out_cap - base : 0;
+  if (h > room) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(&cnt->flags, 2u);
+    h = room;
   }
+  local_hit_tiles(poly, pos, h, base, jobs, num_jobs, out_poly, out_point);
 }
 
 // First output row of every job of the group.
@@ -1365,7 +1389,7 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.hit_cap        = g.cap.hits;
   d.counts         = cnt;
   launch_pip_device(d, s);
-  plan_local_hits_kernel<<<grid_for(g.cap.hits, 256), 256, 0, s>>>(
+  plan_local_hits_kernel<<<grid_for(g.cap.hits, 4096), 256, 0, s>>>(
     g.hit_poly.get(), g.hit_pos.get(), cnt, g.jobs.get(), g.J, P.hit_cap, P.out_poly.get(),
     P.out_point.get());
   QPIP_LAUNCH_CHECK();
