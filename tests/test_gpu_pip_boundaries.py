@@ -145,3 +145,46 @@ def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs)
         autotune.apply(saved)
     o = run_oracle(oracle_mod, x, y, bbox, 64.0 / 64, 6, 8, polys)
     assert_same(g, o)
+
+
+def test_hole_vertical_lines_match_oracle(cs, oracle_mod):
+    """Squares with square holes: points inside an outer ring on the vertical lines
+    through a hole's edges (above and below the hole) are "on edge" for the hole,
+    so the reference reports the whole polygon false there
+    (is_point_in_polygon.cuh:56-65, :93-96); one ULP beside the line they are
+    inside. Any per-ring shortcut would get these wrong. Bar: the oracle's pairs
+    bit for bit, and the hand-picked answers."""
+    part, ring, vx, vy = [0], [0], [], []
+    xs, ys, want_in, want_out = [], [], [], []
+    rng = np.random.default_rng(11)
+    for k in range(16):
+        ox, oy = 4.0 * (k % 4) + 1.0, 4.0 * (k // 4) + 1.0  # outer [ox, ox+3]^2
+        hx0, hy0 = ox + 1.0, oy + 1.0                      # hole [hx0, hx0+1]^2
+        for rx, ry in (([ox, ox + 3, ox + 3, ox, ox], [oy, oy, oy + 3, oy + 3, oy]),
+                       ([hx0, hx0 + 1, hx0 + 1, hx0, hx0], [hy0, hy0, hy0 + 1, hy0 + 1, hy0])):
+            vx.extend(rx)
+            vy.extend(ry)
+            ring.append(ring[-1] + 5)
+        part.append(part[-1] + 2)
+        for hx in (hx0, hx0 + 1.0):
+            for py_ in (oy + 0.5, oy + 2.5):  # below / above the hole, inside the outer ring
+                want_out.append(len(xs))
+                xs.append(hx)
+                ys.append(py_)
+                want_in.append(len(xs))
+                xs.append(np.nextafter(hx, ox) if hx == hx0 else np.nextafter(hx, ox + 3.0))
+                ys.append(py_)
+        xs.extend(rng.uniform(ox, ox + 3.0, 40))
+        ys.extend(rng.uniform(oy, oy + 3.0, 40))
+    x, y = np.array(xs), np.array(ys)
+    polys = (np.array(part, np.int32), np.array(ring, np.int32), np.array(vx), np.array(vy))
+    bbox = (0.0, 17.0, 0.0, 17.0)
+    for d, max_size in ((5, 8), (3, 64)):
+        scale = 17.0 / (1 << d)
+        g = run_gpu(cs, x, y, bbox, scale, d, max_size, polys)
+        o = run_oracle(oracle_mod, x, y, bbox, scale, d, max_size, polys)
+        assert_same(g, o)
+        hit = np.zeros(len(x), bool)
+        hit[g["pi"][g["point_index"]]] = True
+        assert not hit[want_out].any()
+        assert hit[want_in].all()

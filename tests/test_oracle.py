@@ -221,6 +221,17 @@ def test_pip_vertices_and_vertical_extension_are_outside(oracle_mod):
     assert oracle_mod.is_point_in_polygon(3.0, 1.0, ring, 0, 1, lx, ly)
     assert oracle_mod.is_point_in_polygon(1.0, 3.0, ring, 0, 1, lx, ly)
     assert not oracle_mod.is_point_in_polygon(3.0, 3.0, ring, 0, 1, lx, ly)
+    # The same holds across rings: the on-edge hit of any ring makes the whole
+    # polygon false (is_point_in_polygon.cuh:93-96). A point inside the outer
+    # ring, on the vertical line through a hole's edge but above the hole, is
+    # outside; a point beside that line is inside.
+    hr = np.array([0, 5, 10], np.uint32)
+    hx = np.array([0.0, 10.0, 10.0, 0.0, 0.0, 2.0, 4.0, 4.0, 2.0, 2.0])
+    hy = np.array([0.0, 0.0, 10.0, 10.0, 0.0, 2.0, 2.0, 4.0, 4.0, 2.0])
+    for x, y in ((2.0, 7.0), (4.0, 7.0), (2.0, 1.0), (4.0, 0.5), (3.0, 3.0)):
+        assert not oracle_mod.is_point_in_polygon(x, y, hr, 0, 2, hx, hy), (x, y)
+    for x, y in ((3.0, 7.0), (1.0, 3.0), (2.5, 9.0), (np.nextafter(2.0, 3.0), 7.0)):
+        assert oracle_mod.is_point_in_polygon(x, y, hr, 0, 2, hx, hy), (x, y)
 
 
 # ---------------------------------------------------------------------------
