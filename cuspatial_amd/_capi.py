@@ -55,6 +55,7 @@ EXPORTED = (
     "qpip_free",
     "qpip_set_memory_limit",
     "qpip_trim",
+    "qpip_memory_info",
     "qpip_profile_enable",
     "qpip_profile_reset",
     "qpip_profile_report",
@@ -75,6 +76,8 @@ EXPORTED = (
     "qpip_batch_plan_destroy",
     "qpip_linestring_bounding_boxes",
     "qpip_quadtree_point_to_nearest_linestring",
+    "qpip_radix_sort_pairs",
+    "qpip_compact_nonnegative",
     "qpip_ranker_num_params",
     "qpip_ranker_features",
     "qpip_ranker_logits",
@@ -102,6 +105,8 @@ def lib():
         L.qpip_version.restype = ctypes.c_char_p
         L.qpip_free.argtypes = [_vp]
         L.qpip_set_memory_limit.argtypes = [ctypes.c_size_t]
+        L.qpip_memory_info.argtypes = [P(ctypes.c_size_t), P(ctypes.c_size_t)]
+        L.qpip_memory_info.restype = None
         L.qpip_profile_enable.argtypes = [ctypes.c_int]
         L.qpip_profile_report.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
         L.qpip_profile_report.restype = ctypes.c_int64
@@ -150,6 +155,8 @@ def lib():
             _vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
             _vp, _i64, _vp, _vp, _i64, _i64, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp),
             P(_i64)]
+        L.qpip_radix_sort_pairs.argtypes = [_vp, _vp, _i64, ctypes.c_int32, _vp]
+        L.qpip_compact_nonnegative.argtypes = [_vp, _i64, _vp, _vp, P(_i64), _vp]
         L.qpip_ranker_num_params.argtypes = [_i64]
         L.qpip_ranker_num_params.restype = _i64
         L.qpip_ranker_features.argtypes = [_vp, _vp, _vp, _i64, _vp, _vp, _vp]
@@ -169,6 +176,7 @@ def lib():
             "qpip_quadtree_on_points", "qpip_quadtree_on_points_xy", "qpip_polygon_bounding_boxes",
             "qpip_join_quadtree_and_bounding_boxes", "qpip_quadtree_point_in_polygon",
             "qpip_quadtree_point_in_polygon_xy",
+            "qpip_radix_sort_pairs", "qpip_compact_nonnegative",
         ):
             getattr(L, name).restype = ctypes.c_int
         _lib = L
@@ -196,6 +204,44 @@ def profile_report() -> dict:
     buf = ctypes.create_string_buffer(int(n) + 1)
     L.qpip_profile_report(buf, len(buf))
     return {k: (int(v[0]), float(v[1])) for k, v in json.loads(buf.value.decode()).items()}
+
+
+def memory_info() -> tuple:
+    """(live, cached) bytes of the library's pool on the current device."""
+    live, cached = ctypes.c_size_t(), ctypes.c_size_t()
+    lib().qpip_memory_info(ctypes.byref(live), ctypes.byref(cached))
+    return int(live.value), int(cached.value)
+
+
+def sort_pairs(keys: torch.Tensor, values: torch.Tensor, key_bits: int = 32) -> None:
+    """Stable radix sort of (keys, values) in place: uint32/int32 tensors of
+    equal length on one device, ordered by the low key_bits bits of the keys."""
+    if keys.numel() != values.numel():
+        raise ValueError("keys and values must have the same length")
+    if keys.element_size() != 4 or values.element_size() != 4 or not keys.is_contiguous() \
+            or not values.is_contiguous():
+        raise ValueError("keys and values must be contiguous 32-bit tensors")
+    check(lib().qpip_radix_sort_pairs(ctypes.c_void_p(keys.data_ptr()),
+                                      ctypes.c_void_p(values.data_ptr()), keys.numel(),
+                                      int(key_bits), stream_handle(keys.device)))
+
+
+def compact_nonnegative(values: torch.Tensor, with_values: bool = True):
+    """(indices, values) of the entries of an int32 tensor that are >= 0, in
+    order; indices as int64, values as int32 (None without with_values)."""
+    if values.dtype != torch.int32 or not values.is_contiguous():
+        raise ValueError("values must be a contiguous int32 tensor")
+    n = values.numel()
+    idx = torch.empty(max(n, 1), dtype=torch.int32, device=values.device)
+    val = torch.empty(max(n, 1), dtype=torch.int32, device=values.device) if with_values \
+        else None
+    cnt = ctypes.c_int64()
+    check(lib().qpip_compact_nonnegative(
+        ctypes.c_void_p(values.data_ptr()), n, ctypes.c_void_p(idx.data_ptr()),
+        ctypes.c_void_p(val.data_ptr()) if val is not None else None, ctypes.byref(cnt),
+        stream_handle(values.device)))
+    k = int(cnt.value)
+    return idx[:k].to(torch.int64), (val[:k] if val is not None else None)
 
 
 def check(rc: int) -> None:

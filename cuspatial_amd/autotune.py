@@ -7,12 +7,21 @@ under each candidate value, one parameter at a time (coordinate descent from the
 current values), timing the kernels the parameter affects with the library's
 HIP-event timers, and keeps the fastest. The chosen values are applied, and can
 be saved to and loaded from a JSON file next to the workload that chose them.
+
+A ``TuningTable`` keeps the best values per shape bucket (number of jobs,
+points per job, edges per polygon, each rounded to a power of two) in a JSON
+file; ``BatchPlan`` looks its shape up in the package's table (``tuning.json``,
+written by ``python bench.py --workload batched_jobs --autotune --save-tuning``)
+and captures its launch sequence under those values.
 """
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import json
+import math
+import os
 import time
 
 from . import _capi
@@ -100,9 +109,84 @@ def save(path, result: dict, workload: str) -> None:
                   indent=1)
 
 
-def load(path) -> dict:
-    """Applies a saved configuration; returns it."""
+def load(path, bucket: str | None = None) -> dict:
+    """Applies a saved configuration and returns it: the file of ``save``, or
+    the entry of `bucket` in a TuningTable file (nothing applied when the table
+    has no such bucket)."""
     with open(path) as f:
-        cfg = json.load(f)["config"]
+        data = json.load(f)
+    if "buckets" in data:
+        cfg = TuningTable(path).lookup(bucket) if bucket else None
+        if cfg:
+            apply(cfg)
+        return cfg or {}
+    cfg = data["config"]
     apply(cfg)
     return cfg
+
+
+DEFAULT_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+
+
+def shape_bucket(num_jobs: int, num_points: int, num_polygons: int, num_edges: int) -> str:
+    """The bucket of a batch's shape: log2 of the job count, of the mean points
+    per job and of the mean edges per polygon, each rounded."""
+    def lg(v):
+        return int(round(math.log2(max(float(v), 1.0))))
+
+    return (f"jobs{lg(num_jobs)}_points{lg(num_points / max(num_jobs, 1))}"
+            f"_edges{lg(num_edges / max(num_polygons, 1))}")
+
+
+@contextlib.contextmanager
+def applied(config: dict | None):
+    """Runs the block under `config` and restores the previous values after."""
+    if not config:
+        yield
+        return
+    prev = current()
+    apply(config)
+    try:
+        yield
+    finally:
+        apply(prev)
+
+
+class TuningTable:
+    """Best tuning values per shape bucket, kept in a JSON file:
+    {"version": 1, "buckets": {bucket: {"config": {...}, "workload": ...,
+    "baseline_ms": ..., "best_ms": ...}}}."""
+
+    VERSION = 1
+
+    def __init__(self, path: str | None = None):
+        self.path = path
+        self.buckets: dict = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                data = json.load(f)
+            if data.get("version") != self.VERSION:
+                raise ValueError(f"{path}: tuning table version {data.get('version')} is not "
+                                 f"{self.VERSION}")
+            self.buckets = data.get("buckets", {})
+            for b in self.buckets.values():
+                unknown = {*b["config"]} - {*PARAMETERS}  # `set` is qpip_tuning_set here
+                if unknown:
+                    raise ValueError(f"{path}: unknown tuning parameters {sorted(unknown)}")
+
+    def lookup(self, bucket: str) -> dict | None:
+        entry = self.buckets.get(bucket)
+        return dict(entry["config"]) if entry else None
+
+    def record(self, bucket: str, result: dict, workload: str) -> None:
+        self.buckets[bucket] = {"config": dict(result["config"]), "workload": workload,
+                                "baseline_ms": result["baseline_ms"],
+                                "best_ms": result["best_ms"]}
+
+    def save(self, path: str | None = None) -> str:
+        path = path or self.path
+        with open(path + ".tmp", "w") as f:
+            json.dump({"version": self.VERSION, "buckets": self.buckets}, f, indent=1,
+                      sort_keys=True)
+        os.replace(path + ".tmp", path)
+        return path

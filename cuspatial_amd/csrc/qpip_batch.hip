@@ -1214,6 +1214,17 @@ struct BatchPlan {
       (void)hipStreamSynchronize(cstream);
       (void)hipStreamDestroy(cstream);
     }
+    // every stream that used the buffers has finished: they go back idle
+    CompletedScope done;
+    groups.clear();
+    edges.release();
+    edges_b.release();
+    ring_edges.release();
+    counts.release();
+    job_first.release();
+    out_pi.release();
+    out_poly.release();
+    out_point.release();
   }
 };
 
@@ -1540,6 +1551,7 @@ extern "C" int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
                    num_poly_y);
     if (num_points > 0) QPIP_EXPECTS(points_xy, "points must not be null");
 
+    StreamScope scope(stream);
     OutputAllocator oa(mr, stream);
     uint32_t* o_pi = num_points > 0 ? oa.make<uint32_t>(num_points) : nullptr;
     std::vector<uint64_t> job_hits(num_jobs, 0);
@@ -1565,7 +1577,7 @@ extern "C" int qpip_batched_quadtree_point_in_polygon(int64_t num_jobs,
       QPIP_HIP(hipMemcpyAsync(o_point + at, g.point.get(), g.hits * 4, hipMemcpyDeviceToDevice, stream));
       at += g.hits;
     }
-    QPIP_HIP(hipStreamSynchronize(stream));
+    oa.keep();
     *point_indices = o_pi;
     *polygon_index = o_poly;
     *point_index   = o_point;
@@ -1633,6 +1645,7 @@ extern "C" int qpip_batch_plan_create(int64_t num_jobs,
     QPIP_HIP(hipGetDevice(&P->device));
     QPIP_HIP(hipStreamCreateWithFlags(&P->cstream, hipStreamNonBlocking));
     hipStream_t const s = P->cstream;
+    StreamScope scope(s);
     P->hj               = hj;
     P->jpo.assign(job_point_offsets, job_point_offsets + num_jobs + 1);
     P->jpoly.assign(job_polygon_offsets, job_polygon_offsets + num_jobs + 1);
@@ -1818,6 +1831,7 @@ extern "C" int qpip_batch_plan_result(qpip_batch_plan* plan,
       return;
     }
     hipStream_t const s = static_cast<hipStream_t>(stream_);
+    StreamScope scope(s);
     OutputAllocator oa(mr, s);
     uint64_t const n = static_cast<uint64_t>(P.num_points);
     uint32_t* o_pi   = oa.make<uint32_t>(n);
@@ -1828,7 +1842,7 @@ extern "C" int qpip_batch_plan_result(qpip_batch_plan* plan,
       QPIP_HIP(hipMemcpyAsync(o_poly, P.out_poly.get(), H * 4, hipMemcpyDeviceToDevice, s));
       QPIP_HIP(hipMemcpyAsync(o_pt, P.out_point.get(), H * 4, hipMemcpyDeviceToDevice, s));
     }
-    QPIP_HIP(hipStreamSynchronize(s));
+    oa.keep();
     *point_indices = o_pi;
     *polygon_index = o_poly;
     *point_index   = o_pt;

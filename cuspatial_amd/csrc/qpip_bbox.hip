@@ -151,6 +151,7 @@ extern "C" int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
     QPIP_EXPECTS(num_poly_offsets > 0, "Polygon offsets must contain at least one (1) value");
     QPIP_EXPECTS(num_ring_offsets > 0, "Polygon ring offsets must contain at least one (1) value");
     int64_t const num_rings = num_ring_offsets - 1;
+    StreamScope scope(stream);
     OutputAllocator oa(mr, stream);
     double* o_minx = oa.make<double>(num_polys);
     double* o_miny = oa.make<double>(num_polys);
@@ -183,13 +184,9 @@ extern "C" int qpip_polygon_bounding_boxes(const int32_t* poly_offsets,
     uint32_t h_flags[2];
     QPIP_HIP(hipMemcpyAsync(h_flags, flags.get(), sizeof(h_flags), hipMemcpyDeviceToHost, stream));
     QPIP_HIP(hipStreamSynchronize(stream));
-    if (h_flags[1]) {
-      oa.deallocate(o_minx, num_polys * 8);
-      oa.deallocate(o_miny, num_polys * 8);
-      oa.deallocate(o_maxx, num_polys * 8);
-      oa.deallocate(o_maxy, num_polys * 8);
+    if (h_flags[1])  // the outputs are released by `oa`
       throw logic_error("cuSpatial failure: polygon offset out of range of ring offsets");
-    }
+    oa.keep();
     *minx      = o_minx;
     *miny      = o_miny;
     *maxx      = o_maxx;

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/cuspatial_amd.h"
 
@@ -111,12 +112,42 @@ int64_t tuning(Tuning t);
 
 // ----------------------------------------------------------------------------
 // Caching device pool (library-owned temporaries and default-resource outputs).
+//
+// The pool is stream-ordered: a block freed while kernels queued on a stream
+// may still use it goes back to the pool at once, and is handed out again
+//   * on the same stream with no wait (the stream orders the work), or
+//   * on another stream behind a hipStreamWaitEvent on the completion event of
+//     the call that last used it.
+// Each C entry point opens a StreamScope for the stream its work runs on; when
+// the outermost scope ends, one event is recorded on that stream and stamped on
+// every block the call allocated or freed. So entry points need not synchronise
+// their stream before returning, and temporaries of one thread's call can serve
+// another thread's stream safely.
 // ----------------------------------------------------------------------------
 void* pool_alloc(size_t bytes);
 void pool_free(void* p);
 void pool_set_limit(size_t bytes);
 void pool_trim();
 size_t pool_bytes_in_use();
+void pool_stats(size_t* live, size_t* cached);
+
+class StreamScope {
+ public:
+  explicit StreamScope(hipStream_t stream);
+  ~StreamScope();
+  StreamScope(StreamScope const&)            = delete;
+  StreamScope& operator=(StreamScope const&) = delete;
+};
+
+// Inside a CompletedScope, freed blocks are known idle (the caller has
+// synchronised every stream that used them): any stream may reuse them at once.
+class CompletedScope {
+ public:
+  CompletedScope();
+  ~CompletedScope();
+  CompletedScope(CompletedScope const&)            = delete;
+  CompletedScope& operator=(CompletedScope const&) = delete;
+};
 
 // RAII temporary buffer from the pool.
 template <class T>
@@ -171,18 +202,28 @@ class dbuf {
 };
 
 // Output buffers: from the caller's resource when one is given, else from the
-// pool (then freed with qpip_free).
+// pool (then freed with qpip_free). Outputs are owned here until keep() hands
+// them to the caller: an entry point that throws releases what it allocated.
 class OutputAllocator {
  public:
   OutputAllocator(qpip_resource const* mr, hipStream_t stream) : mr_(mr), stream_(stream) {}
+  ~OutputAllocator();
+  OutputAllocator(OutputAllocator const&)            = delete;
+  OutputAllocator& operator=(OutputAllocator const&) = delete;
   void* allocate(size_t bytes);
   void deallocate(void* p, size_t bytes);
+  // The caller owns every output allocated so far.
+  void keep() { owned_.clear(); }
   // Move a pool buffer into an output: no copy for the default resource.
   template <class T>
   T* adopt(dbuf<T>& tmp, size_t n)
   {
     if (n == 0) return nullptr;
-    if (!mr_) return tmp.release_ownership();
+    if (!mr_) {
+      T* p = tmp.release_ownership();
+      owned_.push_back({p, n * sizeof(T)});
+      return p;
+    }
     T* out = static_cast<T*>(allocate(n * sizeof(T)));
     QPIP_HIP(hipMemcpyAsync(out, tmp.get(), n * sizeof(T), hipMemcpyDeviceToDevice, stream_));
     return out;
@@ -194,8 +235,13 @@ class OutputAllocator {
   }
 
  private:
+  struct Owned {
+    void* p;
+    size_t bytes;
+  };
   qpip_resource const* mr_;
   hipStream_t stream_;
+  std::vector<Owned> owned_;
 };
 
 // ----------------------------------------------------------------------------

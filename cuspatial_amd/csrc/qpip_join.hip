@@ -158,6 +158,7 @@ extern "C" int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
     QPIP_EXPECTS(num_nodes < (int64_t{1} << 32) && num_bboxes < (int64_t{1} << 32),
                  "quadtree and bounding box tables must have fewer than 2^32 rows");
 
+    StreamScope scope(stream);
     TreeView tv{key, level, is_internal_node, length, offset, static_cast<uint64_t>(num_nodes)};
     BoxView bv{bbox_minx, bbox_miny, bbox_maxx, bbox_maxy, static_cast<uint64_t>(num_bboxes)};
     Params prm{x_min, y_min, scale, static_cast<int>(max_depth)};
@@ -243,7 +244,7 @@ extern "C" int qpip_join_quadtree_and_bounding_boxes(const uint32_t* key,
     permute_pairs_kernel<<<grid_for(K, 256), 256, 0, stream>>>(s.values, out_node.get(),
                                                                out_bbox.get(), K, o_bbox, o_quad);
     QPIP_LAUNCH_CHECK();
-    QPIP_HIP(hipStreamSynchronize(stream));
+    oa.keep();
     *bbox_offset = o_bbox;
     *quad_offset = o_quad;
     *num_pairs   = static_cast<int64_t>(K);

@@ -257,6 +257,7 @@ extern "C" int qpip_quadtree_point_to_nearest_linestring(const uint32_t* linestr
         num_linestring_offsets == 0)
       return QPIP_SUCCESS;
     int64_t const n = num_point_indices;
+    StreamScope scope(stream);
     OutputAllocator oa(mr, stream);
     uint32_t* o_point = oa.make<uint32_t>(n);
     uint32_t* o_ls    = oa.make<uint32_t>(n);
@@ -285,13 +286,10 @@ extern "C" int qpip_quadtree_point_to_nearest_linestring(const uint32_t* linestr
     uint32_t h_bad = 0;
     QPIP_HIP(hipMemcpyAsync(&h_bad, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     QPIP_HIP(hipStreamSynchronize(stream));
-    if (h_bad) {
-      oa.deallocate(o_point, n * 4);
-      oa.deallocate(o_ls, n * 4);
-      oa.deallocate(o_dist, n * 8);
+    if (h_bad)  // the outputs are released by `oa`
       throw logic_error(
         "cuSpatial failure: a quadrant, point or linestring index is out of range");
-    }
+    oa.keep();
     *point_index              = o_point;
     *nearest_linestring_index = o_ls;
     *distance                 = o_dist;
@@ -327,9 +325,10 @@ extern "C" int qpip_linestring_bounding_boxes(const int32_t* linestring_offsets,
     QPIP_EXPECTS(num_x == num_y, "x and y must be the same size");
     QPIP_EXPECTS(expansion_radius >= 0, "expansion radius must be greater or equal than 0");
     QPIP_EXPECTS(num_x >= 2 * num_ls, "all linestrings must have at least 2 vertices");
-    if (num_linestring_offsets == 0 || num_x == 0) return QPIP_SUCCESS;
+    if (num_ls == 0 || num_x == 0) return QPIP_SUCCESS;
     // a linestring is a polygon with one ring: the same per-geometry reduction
     // (detail/bounding_boxes.cuh:97-131 and :139-184 both reduce by geometry id)
+    StreamScope scope(stream);
     dbuf<int32_t> parts(num_ls + 1);
     iota_i32_kernel<<<grid_for(static_cast<uint64_t>(num_ls + 1), 256, 1u << 30), 256, 0,
                       stream>>>(parts.get(), num_ls + 1);

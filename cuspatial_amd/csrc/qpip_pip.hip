@@ -40,12 +40,6 @@ namespace qpip {
 
 namespace {
 
-struct PipMeta {
-  uint64_t num_chunks;
-  uint64_t num_hits;
-  uint32_t bad;
-};
-
 // --- 2. chunk enumeration ---------------------------------------------------------
 struct ChunkOp {
   uint32_t const* pair_quad;
@@ -559,8 +553,10 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
   QPIP_LAUNCH_CHECK();
   {
     KernelTimer kt("pip_write", stream);
-    // 4 waves per block, 64 chunks per wave
-    pip_write_kernel<<<grid_for(d.chunk_cap, 4 * kWave), 256, 0, stream>>>(
+    // 4 waves per block, 64 chunks per wave; the kernel strides over what a
+    // grid of at most 2048 blocks does not cover (a capacity may be far above
+    // the count)
+    pip_write_kernel<<<grid_for(d.chunk_cap, 4 * kWave, 2048), 256, 0, stream>>>(
       0, d.masks, d.chunk_info, d.hit_start, d.out_poly, d.out_pos, n_chunks, d.hit_cap, flags);
     QPIP_LAUNCH_CHECK();
   }
@@ -620,11 +616,13 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                  "inputs must have fewer than 2^32 rows");
     uint64_t const K         = static_cast<uint64_t>(num_pairs);
     uint64_t const num_rings = num_ring_offsets > 0 ? static_cast<uint64_t>(num_ring_offsets - 1) : 0;
+    StreamScope scope(stream);
 
-    dbuf<PipMeta> meta(1);
-    dbuf<uint32_t> bad(1);
-    QPIP_HIP(hipMemsetAsync(meta.get(), 0, sizeof(PipMeta), stream));
-    QPIP_HIP(hipMemsetAsync(bad.get(), 0, sizeof(uint32_t), stream));
+    // bad[0]: a quadrant index past the tree (chunk count); bad[1]: bit 1 a point or
+    // polygon index out of range (test), bit 2 more chunks than the capacity
+    dbuf<uint32_t> bad(2);
+    dbuf<uint64_t> totals(2);  // chunks, hits
+    QPIP_HIP(hipMemsetAsync(bad.get(), 0, 2 * sizeof(uint32_t), stream));
 
     // 1. edges
     dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
@@ -637,54 +635,87 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
         edges_b.get(), ring_edges.get());
       QPIP_LAUNCH_CHECK();
     }
-    // 2. chunks
+    // 2. chunks: pair k's first chunk
     dbuf<uint32_t> chunk_start(K + 1);
     dbuf<uint64_t> partials(kScanBlocks + 1);
     ChunkOp cop{quad_index, length, static_cast<uint64_t>(num_nodes), chunk_start.get(), bad.get()};
-    count_scan_emit<ChunkOp, uint64_t, uint64_t>(cop, nullptr, K, partials.get(),
-                                                 &meta.get()->num_chunks, stream, scan_blocks_for(K),
-                                                 "pip_chunks");
-    PipMeta h_meta;
-    QPIP_HIP(hipMemcpyAsync(&h_meta, meta.get(), sizeof(PipMeta), hipMemcpyDeviceToHost, stream));
-    uint32_t h_bad = 0;
-    QPIP_HIP(hipMemcpyAsync(&h_bad, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    QPIP_HIP(hipStreamSynchronize(stream));
-    if (h_bad) throw logic_error("cuSpatial failure: quadrant index out of range of the quadtree");
-    uint64_t const T = h_meta.num_chunks;
+    count_scan_emit<ChunkOp, uint64_t, uint64_t>(cop, nullptr, K, partials.get(), totals.get(),
+                                                 stream, scan_blocks_for(K), "pip_chunks");
+
+    // 3-4. test and hit scan into per-chunk arrays of a guessed capacity, so that
+    // one host wait reads the chunk and hit totals together. The guess covers four
+    // candidate points per input point (the headline has 0.4); a join with more
+    // candidates is run again with the exact chunk count, as is a guess the
+    // memory limit refuses.
+    uint64_t const n_pts = static_cast<uint64_t>(num_x);
+    uint64_t chunk_cap   = K + (n_pts >> 4) + 4096;
+    bool exact           = false;
+    dbuf<uint64_t> masks, hit_start;
+    dbuf<uint32_t> hit_counts;
+    dbuf<uint2> chunk_info;
+    uint64_t h_tot[2] = {0, 0};
+    uint32_t h_bad[2] = {0, 0};
+    for (;;) {
+      try {
+        masks.reset(chunk_cap);
+        hit_counts.reset(chunk_cap);
+        chunk_info.reset(chunk_cap);
+        hit_start.reset(chunk_cap);
+      } catch (oom_error const&) {
+        if (exact) throw;
+        masks.release();
+        hit_counts.release();
+        chunk_info.release();
+        hit_start.release();
+        QPIP_HIP(hipMemcpyAsync(h_tot, totals.get(), sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                stream));
+        QPIP_HIP(hipStreamSynchronize(stream));
+        QPIP_EXPECTS(h_tot[0] < (uint64_t{1} << 31), "too many candidate points");
+        chunk_cap = std::max<uint64_t>(h_tot[0], 1);
+        exact     = true;
+        continue;
+      }
+      PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
+                   n_pts, x, y, poly_offsets,
+                   static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
+                   ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), chunk_cap,
+                   static_cast<uint64_t>(num_nodes), xy,
+                   xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
+                   pairs_per_wave(K), nullptr, chunk_cap};
+      {
+        KernelTimer kt("pip_test", stream);
+        launch_pip_test(args, K, masks.get(), hit_counts.get(), chunk_info.get(), bad.get() + 1,
+                        stream);
+      }
+      // chunks past the capacity were not tested: the scan covers those that were
+      dbuf<uint64_t> n_chunks(1);
+      clamp_count_kernel<<<1, 64, 0, stream>>>(totals.get(), chunk_cap, n_chunks.get(),
+                                               bad.get() + 1);
+      QPIP_LAUNCH_CHECK();
+      HitScanOp hop{hit_counts.get(), hit_start.get()};
+      count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, n_chunks.get(), 0, partials.get(),
+                                                     totals.get() + 1, stream, kScanBlocks,
+                                                     "pip_scan");
+      QPIP_HIP(hipMemcpyAsync(h_tot, totals.get(), sizeof(h_tot), hipMemcpyDeviceToHost, stream));
+      QPIP_HIP(hipMemcpyAsync(h_bad, bad.get(), sizeof(h_bad), hipMemcpyDeviceToHost, stream));
+      QPIP_HIP(hipStreamSynchronize(stream));
+      if (h_bad[0]) throw logic_error("cuSpatial failure: quadrant index out of range of the quadtree");
+      if (h_bad[1] & 1u) throw logic_error("cuSpatial failure: point or polygon index out of range");
+      if (!(h_bad[1] & 2u)) break;
+      // an exact capacity cannot overflow; past 2^31 chunks the starts saturate
+      QPIP_EXPECTS(!exact && h_tot[0] < (uint64_t{1} << 31), "too many candidate points");
+      chunk_cap = std::max<uint64_t>(h_tot[0], 1);
+      exact     = true;
+      QPIP_HIP(hipMemsetAsync(bad.get() + 1, 0, sizeof(uint32_t), stream));
+    }
+    uint64_t const T = h_tot[0];
     if (T == 0) return QPIP_SUCCESS;
     QPIP_EXPECTS(T < (uint64_t{1} << 31), "too many candidate points");
-
-    PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
-                 static_cast<uint64_t>(num_x), x, y, poly_offsets,
-                 static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
-                 ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), T,
-                 static_cast<uint64_t>(num_nodes), xy,
-                 xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
-                 pairs_per_wave(K), nullptr, T};
-    // 3. test
-    dbuf<uint64_t> masks(T);
-    dbuf<uint32_t> hit_counts(T);
-    dbuf<uint2> chunk_info(T);
-
-    {
-      KernelTimer kt("pip_test", stream);
-      launch_pip_test(args, K, masks.get(), hit_counts.get(), chunk_info.get(), bad.get(),
-                      stream);
-    }
-    // 4. scan
-    dbuf<uint64_t> hit_start(T);
-    HitScanOp hop{hit_counts.get(), hit_start.get()};
-    count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, nullptr, T, partials.get(),
-                                                   &meta.get()->num_hits, stream, scan_blocks_for(T),
-                                                   "pip_scan");
-    QPIP_HIP(hipMemcpyAsync(&h_meta, meta.get(), sizeof(PipMeta), hipMemcpyDeviceToHost, stream));
-    QPIP_HIP(hipMemcpyAsync(&h_bad, bad.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    QPIP_HIP(hipStreamSynchronize(stream));
-    if (h_bad) throw logic_error("cuSpatial failure: point or polygon index out of range");
-    uint64_t const H = h_meta.num_hits;
+    uint64_t const H = h_tot[1];
     if (H == 0) return QPIP_SUCCESS;
     QPIP_EXPECTS(H < (uint64_t{1} << 32), "more than 2^32 hits");
-    // 5. write
+    // 5. write (no wait: the outputs are ready for work queued on `stream` after
+    // this call)
     OutputAllocator oa(mr, stream);
     uint32_t* o_poly  = oa.make<uint32_t>(H);
     uint32_t* o_point = oa.make<uint32_t>(H);
@@ -692,10 +723,10 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
       KernelTimer kt("pip_write", stream);
       pip_write_kernel<<<grid_for(T, 4 * kWave), 256, 0, stream>>>(T, masks.get(), chunk_info.get(),
                                                    hit_start.get(), o_poly, o_point, nullptr, H,
-                                                   bad.get());
+                                                   bad.get() + 1);
       QPIP_LAUNCH_CHECK();
     }
-    QPIP_HIP(hipStreamSynchronize(stream));
+    oa.keep();
     *polygon_index = o_poly;
     *point_index   = o_point;
     *num_hits      = static_cast<int64_t>(H);
@@ -820,6 +851,7 @@ int point_in_polygon_entry(const double* x,
     uint64_t const n         = static_cast<uint64_t>(num_x);
     uint64_t const num_rings = num_ring_offsets > 0 ? static_cast<uint64_t>(num_ring_offsets - 1) : 0;
     if (num_polys > 0) QPIP_EXPECTS(poly_offsets && ring_offsets, "offsets must not be null");
+    StreamScope scope(stream);
 
     dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
     dbuf<EdgeB> edges_b(std::max<int64_t>(num_poly_x, 1));
@@ -859,7 +891,7 @@ int point_in_polygon_entry(const double* x,
       QPIP_LAUNCH_CHECK();
       *result = out;
     }
-    QPIP_HIP(hipStreamSynchronize(stream));
+    oa.keep();
     *num_results = num_x;
     return QPIP_SUCCESS;
   } catch (...) {

@@ -416,7 +416,7 @@ static int64_t quadtree_on_points_impl(double const* x,
       o_offset);
     QPIP_LAUNCH_CHECK();
   }
-  QPIP_HIP(hipStreamSynchronize(stream));
+  // no wait here: the outputs are ready for work queued on `stream` after this call
   *point_indices = pi;
   *key           = o_key;
   *level         = o_level;
@@ -463,11 +463,13 @@ int quadtree_on_points_entry(const double* x,
     QPIP_EXPECTS(num_points == num_y, "x and y columns must have the same length");
     if (num_points <= 0) return QPIP_SUCCESS;
     QPIP_EXPECTS(x && (interleaved || y), "x and y must not be null");
+    StreamScope scope(static_cast<hipStream_t>(stream));
     OutputAllocator oa(mr, static_cast<hipStream_t>(stream));
     *num_nodes = quadtree_on_points_impl(x, y, interleaved, num_points, x_min, x_max, y_min,
                                          y_max, scale, max_depth, max_size, oa,
                                          static_cast<hipStream_t>(stream), point_indices, key,
                                          level, is_internal_node, length, offset);
+    oa.keep();
     return QPIP_SUCCESS;
   } catch (...) {
     return qpip::error_status();

@@ -798,12 +798,12 @@ extern "C" int qpip_ranker_features(const qpip_ranker* model,
     QPIP_EXPECTS(num_points >= 0, "point count must not be negative");
     if (num_points == 0) return;
     auto stream = static_cast<hipStream_t>(stream_);
+    StreamScope scope(stream);
     Layout const lay(model->num_cells);
     features_kernel<<<grid_for(static_cast<uint64_t>(num_points), 256, 1u << 30), 256, 0,
                       stream>>>(g, model->params_bf16 + lay.e, xy, num_points, num_points,
                                 features, cells);
     QPIP_LAUNCH_CHECK();
-    QPIP_HIP(hipStreamSynchronize(stream));
   });
 }
 
@@ -820,12 +820,12 @@ extern "C" int qpip_ranker_logits(const qpip_ranker* model,
     QPIP_EXPECTS(num_points >= 0, "point count must not be negative");
     if (num_points == 0) return;
     auto stream        = static_cast<hipStream_t>(stream_);
+    StreamScope scope(stream);
     int64_t const rows = round16(num_points);
     Activations act(rows);
     forward(model, g, xy, num_points, rows, act, stream);
     QPIP_HIP(hipMemcpyAsync(logits, act.Z.get(), num_points * kCand * sizeof(float),
                             hipMemcpyDeviceToDevice, stream));
-    QPIP_HIP(hipStreamSynchronize(stream));
   });
 }
 
@@ -843,6 +843,7 @@ extern "C" int qpip_ranker_gradients(const qpip_ranker* model,
     check_model(model, grid);
     QPIP_EXPECTS(num_points > 0, "a training batch needs at least one point");
     auto stream        = static_cast<hipStream_t>(stream_);
+    StreamScope scope(stream);
     Layout const lay(model->num_cells);
     uint16_t const* pb = model->params_bf16;
     int64_t const rows = round16(num_points);
@@ -891,7 +892,6 @@ extern "C" int qpip_ranker_gradients(const qpip_ranker* model,
     embed_grad_kernel<<<grid_for(static_cast<uint64_t>(num_points) * kEmb, 256, 1u << 30), 256, 0,
                         stream>>>(dXe.get(), act.cells.get(), num_points, grad + lay.e);
     QPIP_LAUNCH_CHECK();
-    QPIP_HIP(hipStreamSynchronize(stream));
   });
 }
 
@@ -913,6 +913,7 @@ extern "C" int qpip_ranker_adam(float* params,
     QPIP_EXPECTS(step >= 1, "Adam step counts from 1");
     if (num_params == 0) return;
     auto stream    = static_cast<hipStream_t>(stream_);
+    StreamScope scope(stream);
     float const c1 = 1.f / (1.f - std::pow(beta1, static_cast<float>(step)));
     float const c2 = 1.f / (1.f - std::pow(beta2, static_cast<float>(step)));
     KernelTimer kt("ranker_adam", stream);
@@ -948,6 +949,7 @@ extern "C" int qpip_ranker_locate(const qpip_ranker* model,
                  "offset columns need at least one entry");
     if (num_points == 0) return;
     auto stream           = static_cast<hipStream_t>(stream_);
+    StreamScope scope(stream);
     int64_t const polys   = num_poly_offsets - 1;
     int64_t const rings   = num_ring_offsets - 1;
     dbuf<Edge> edges(std::max<int64_t>(num_vertices, 1));
@@ -999,6 +1001,5 @@ extern "C" int qpip_ranker_locate(const qpip_ranker* model,
       verify_kernel<<<grid_for(static_cast<uint64_t>(num_points), 256, 1u << 30), 256, 0, stream>>>(a);
       QPIP_LAUNCH_CHECK();
     }
-    QPIP_HIP(hipStreamSynchronize(stream));
   });
 }

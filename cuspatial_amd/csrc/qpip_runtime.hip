@@ -7,18 +7,27 @@
 // binned by size; a cached block is reused for a request of at least half its
 // size.
 //
-// A temporary is handed back when it leaves scope, which is often before the
-// kernels queued on the call's stream have finished with it. Reusing it in the
-// same call is safe (the same stream orders the work), and so is reusing it in a
-// later call of the same host thread (every entry point synchronizes its stream
-// before returning). Another thread, on another stream, must not get it: its
-// kernels could overwrite the block while the first stream still reads it. So
-// free blocks are kept per host thread; when a thread exits, its blocks are idle
-// and move to a shared list that any thread may draw from.
+// The pool is stream-ordered (as RMM's stream-ordered resources are). A block is
+// freed when its owner leaves scope, usually before the kernels queued on the
+// call's stream have finished with it. Each free block carries the stream it was
+// last used on and, once the call that used it has ended, that call's completion
+// event:
+//   * a request on the same stream takes it with no wait (the stream orders the
+//     old work before the new);
+//   * a request on another stream takes it behind hipStreamWaitEvent on the
+//     completion event (no host synchronisation);
+//   * a block whose call is still running (no event yet) goes only to its own
+//     stream.
+// The completion event is recorded once per outermost StreamScope (one per C
+// entry point call) and shared by every block the call allocated or freed.
+// Blocks freed outside any scope (qpip_free of an output) keep the completion
+// event of the call that produced them: the caller's own uses must be finished,
+// as with hipFree.
 #include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <unordered_map>
 #include <vector>
@@ -30,19 +39,44 @@ namespace qpip {
 namespace {
 thread_local std::string g_last_error;
 
-using FreeList = std::multimap<size_t, void*>;  // size -> block
+// A recorded completion event, shared by the blocks it orders; returned to its
+// device's event cache when the last block drops it.
+struct CompletionEvent;
+using EventPtr = std::shared_ptr<CompletionEvent>;
+
+struct BlockInfo {
+  size_t size         = 0;
+  hipStream_t stream  = nullptr;  // stream of the last use
+  uint64_t scope      = 0;        // scope still running on `stream` (0: none)
+  EventPtr event;                 // completion of the last use (null: not yet recorded)
+  bool idle           = false;    // no work can touch the block any more
+  bool bound          = false;    // `stream` is known (allocated or freed inside a scope)
+};
 
 struct DevicePool {
   std::mutex mu;
-  std::unordered_map<void*, size_t> live;  // block -> size
-  FreeList idle;                           // blocks of exited threads
+  std::unordered_map<void*, BlockInfo> live;
+  std::unordered_map<void*, BlockInfo> free_info;
+  std::multimap<size_t, void*> free_by_size;
+  std::mutex ev_mu;  // guards spare_events (events are dropped while mu is held)
+  std::vector<hipEvent_t> spare_events;
   size_t live_bytes   = 0;
   size_t cached_bytes = 0;
 };
 
 std::mutex g_pools_mu;
-std::vector<DevicePool*> g_pools;  // never freed: used by thread exit handlers
+std::vector<DevicePool*> g_pools;  // never freed: events return to them at exit
 size_t g_limit = 0;                // 0: no cap
+
+struct CompletionEvent {
+  hipEvent_t ev;
+  DevicePool* pool;
+  ~CompletionEvent()
+  {
+    std::lock_guard<std::mutex> g(pool->ev_mu);
+    pool->spare_events.push_back(ev);
+  }
+};
 
 DevicePool& pool_for_device(int dev)
 {
@@ -70,26 +104,20 @@ int current_device_nothrow()
   return dev;
 }
 
-// This thread's free blocks, per device. Touched only by its own thread; the
-// byte counters it shares are updated under the device pool's mutex.
-struct ThreadCache {
-  std::vector<std::pair<DevicePool*, FreeList>> lists;  // index: device
-  FreeList& list(int dev, DevicePool* p)
-  {
-    if (static_cast<size_t>(dev) >= lists.size()) lists.resize(dev + 1, {nullptr, FreeList()});
-    lists[dev].first = p;
-    return lists[dev].second;
-  }
-  ~ThreadCache()
-  {
-    for (auto& [p, fl] : lists) {
-      if (!p || fl.empty()) continue;
-      std::lock_guard<std::mutex> g(p->mu);
-      p->idle.insert(fl.begin(), fl.end());
-    }
-  }
+// Open scopes of this thread, innermost last. A scope on the stream of the
+// enclosing one joins it (one event for the outermost).
+struct ScopeState {
+  hipStream_t stream;
+  uint64_t id;
+  int depth;
+  bool completed;
+  DevicePool* pool;
+  std::vector<void*> touched;  // blocks allocated or freed in the scope
 };
-thread_local ThreadCache t_cache;
+thread_local std::vector<ScopeState> t_scopes;
+std::atomic<uint64_t> g_scope_ids{1};
+
+ScopeState* current_scope() { return t_scopes.empty() ? nullptr : &t_scopes.back(); }
 
 size_t round_size(size_t bytes)
 {
@@ -98,29 +126,60 @@ size_t round_size(size_t bytes)
   return (bytes + gran - 1) / gran * gran;
 }
 
-bool take(FreeList& fl, size_t size, void*& blk, size_t& bsz)
+bool stream_capturing(hipStream_t s)
 {
-  auto it = fl.lower_bound(size);
-  if (it == fl.end() || it->first > 2 * size) return false;
-  blk = it->second;
-  bsz = it->first;
-  fl.erase(it);
-  return true;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;  // unknown: act as if capturing (no cross-stream handoff)
+  }
+  return st != hipStreamCaptureStatusNone;
 }
 
-// Frees this thread's cached blocks and the shared idle ones (caller holds p.mu).
-void release_cached(DevicePool& p, FreeList& mine)
+// Takes a cached block of [size, 2 size] usable on `stream` (caller holds p.mu).
+bool take(DevicePool& p, size_t size, hipStream_t stream, void*& blk, BlockInfo& info)
 {
-  for (auto& kv : mine) {
+  bool may_wait = true;  // decided on first need: a capturing stream waits on nothing
+  bool checked  = false;
+  for (auto it = p.free_by_size.lower_bound(size);
+       it != p.free_by_size.end() && it->first <= 2 * size; ++it) {
+    auto fi = p.free_info.find(it->second);
+    BlockInfo const& b = fi->second;
+    // a block used outside any scope (unknown stream, no event) waits for
+    // release_cached, which synchronises the device
+    bool ok = b.idle || (b.bound && b.stream == stream);
+    if (!ok && b.event) {
+      if (!checked) {
+        may_wait = !stream_capturing(stream);
+        checked  = true;
+      }
+      if (may_wait) {
+        QPIP_HIP(hipStreamWaitEvent(stream, b.event->ev, 0));
+        ok = true;
+      }
+    }
+    if (!ok) continue;
+    blk  = it->second;
+    info = std::move(fi->second);
+    p.free_info.erase(fi);
+    p.free_by_size.erase(it);
+    return true;
+  }
+  return false;
+}
+
+// Frees every cached block (caller holds p.mu). Kernels of other threads may
+// still use some of them: the device is synchronised first.
+void release_cached(DevicePool& p)
+{
+  if (p.free_by_size.empty()) return;
+  (void)hipDeviceSynchronize();
+  for (auto& kv : p.free_by_size) {
     (void)hipFree(kv.second);
     p.cached_bytes -= kv.first;
   }
-  mine.clear();
-  for (auto& kv : p.idle) {
-    (void)hipFree(kv.second);
-    p.cached_bytes -= kv.first;
-  }
-  p.idle.clear();
+  p.free_by_size.clear();
+  p.free_info.clear();
 }
 }  // namespace
 
@@ -130,34 +189,40 @@ std::string const& last_error() { return g_last_error; }
 void* pool_alloc(size_t bytes)
 {
   if (bytes == 0) return nullptr;
-  int const dev     = current_device();
-  DevicePool& p     = pool_for_device(dev);
-  FreeList& mine    = t_cache.list(dev, &p);
-  size_t const size = round_size(bytes);
+  int const dev      = current_device();
+  DevicePool& p      = pool_for_device(dev);
+  ScopeState* sc     = current_scope();
+  hipStream_t stream = sc ? sc->stream : nullptr;
+  size_t const size  = round_size(bytes);
   std::lock_guard<std::mutex> g(p.mu);
   if (g_limit && p.live_bytes + size > g_limit)
     throw oom_error("out of memory: library memory limit of " + std::to_string(g_limit) +
                     " bytes reached (requested " + std::to_string(size) + ")");
-  void* blk  = nullptr;
-  size_t bsz = 0;
-  if (take(mine, size, blk, bsz) || take(p.idle, size, blk, bsz)) {
-    p.cached_bytes -= bsz;
-    p.live[blk] = bsz;
-    p.live_bytes += bsz;
-    return blk;
+  void* blk = nullptr;
+  BlockInfo info;
+  if (take(p, size, stream, blk, info)) {
+    p.cached_bytes -= info.size;
+  } else {
+    hipError_t err = hipMalloc(&blk, size);
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      release_cached(p);
+      err = hipMalloc(&blk, size);
+    }
+    if (err != hipSuccess) {
+      (void)hipGetLastError();
+      throw oom_error("out of memory: hipMalloc of " + std::to_string(size) + " bytes failed");
+    }
+    info.size = size;
   }
-  hipError_t err = hipMalloc(&blk, size);
-  if (err != hipSuccess) {
-    (void)hipGetLastError();
-    release_cached(p, mine);
-    err = hipMalloc(&blk, size);
-  }
-  if (err != hipSuccess) {
-    (void)hipGetLastError();
-    throw oom_error("out of memory: hipMalloc of " + std::to_string(size) + " bytes failed");
-  }
-  p.live[blk] = size;
-  p.live_bytes += size;
+  info.stream = stream;
+  info.scope  = sc ? sc->id : 0;
+  info.event.reset();
+  info.idle  = false;
+  info.bound = sc != nullptr;
+  p.live_bytes += info.size;
+  p.live[blk] = std::move(info);
+  if (sc) sc->touched.push_back(blk);
   return blk;
 }
 
@@ -166,26 +231,109 @@ void pool_free(void* blk)
   if (!blk) return;
   int const dev  = current_device_nothrow();
   DevicePool& p  = pool_for_device(dev);
-  FreeList& mine = t_cache.list(dev, &p);
+  ScopeState* sc = current_scope();
   std::lock_guard<std::mutex> g(p.mu);
   auto it = p.live.find(blk);
   if (it == p.live.end()) return;  // not ours
-  size_t const sz = it->second;
+  BlockInfo info = std::move(it->second);
   p.live.erase(it);
-  p.live_bytes -= sz;
-  mine.emplace(sz, blk);
-  p.cached_bytes += sz;
+  p.live_bytes -= info.size;
+  if (sc && sc->completed) {
+    info.idle = true;
+    info.event.reset();
+  } else if (sc) {
+    // last used by this call: ordered on its stream, event at the scope's end
+    info.stream = sc->stream;
+    info.scope  = sc->id;
+    info.bound  = true;
+    info.event.reset();
+    sc->touched.push_back(blk);
+  }
+  // else: a caller's qpip_free; the block keeps its producing call's event
+  p.cached_bytes += info.size;
+  p.free_by_size.emplace(info.size, blk);
+  p.free_info[blk] = std::move(info);
 }
+
+StreamScope::StreamScope(hipStream_t stream)
+{
+  ScopeState* cur = current_scope();
+  if (cur && cur->stream == stream && !cur->completed) {
+    ++cur->depth;
+    return;
+  }
+  t_scopes.push_back(ScopeState{stream, g_scope_ids.fetch_add(1), 1, false,
+                                &pool_for_device(current_device_nothrow()), {}});
+}
+
+StreamScope::~StreamScope()
+{
+  ScopeState& sc = t_scopes.back();
+  if (--sc.depth > 0) return;
+  DevicePool& p = *sc.pool;
+  EventPtr ev;
+  if (!sc.touched.empty() && !stream_capturing(sc.stream)) {
+    hipEvent_t e = nullptr;
+    {
+      std::lock_guard<std::mutex> g(p.ev_mu);
+      if (!p.spare_events.empty()) {
+        e = p.spare_events.back();
+        p.spare_events.pop_back();
+      }
+    }
+    if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) e = nullptr;
+    if (e && hipEventRecord(e, sc.stream) == hipSuccess) {
+      ev = EventPtr(new CompletionEvent{e, &p});
+    } else if (e) {
+      (void)hipGetLastError();
+      std::lock_guard<std::mutex> g(p.ev_mu);
+      p.spare_events.push_back(e);
+    }
+  }
+  if (ev) {
+    std::lock_guard<std::mutex> g(p.mu);
+    for (void* b : sc.touched) {
+      auto lv = p.live.find(b);
+      if (lv != p.live.end() && lv->second.scope == sc.id) {
+        lv->second.event = ev;
+        lv->second.scope = 0;
+        continue;
+      }
+      auto fv = p.free_info.find(b);
+      if (fv != p.free_info.end() && fv->second.scope == sc.id) {
+        fv->second.event = ev;
+        fv->second.scope = 0;
+      }
+    }
+  }
+  // without an event (none recorded, e.g. under capture) the blocks stay bound
+  // to their stream
+  t_scopes.pop_back();
+}
+
+CompletedScope::CompletedScope()
+{
+  t_scopes.push_back(ScopeState{nullptr, g_scope_ids.fetch_add(1), 1, true,
+                                &pool_for_device(current_device_nothrow()), {}});
+}
+
+CompletedScope::~CompletedScope() { t_scopes.pop_back(); }
 
 void pool_set_limit(size_t bytes) { g_limit = bytes; }
 
 void pool_trim()
 {
-  int const dev  = current_device();
-  DevicePool& p  = pool_for_device(dev);
-  FreeList& mine = t_cache.list(dev, &p);
+  DevicePool& p = pool_for_device(current_device());
   std::lock_guard<std::mutex> g(p.mu);
-  release_cached(p, mine);
+  release_cached(p);
+}
+
+void pool_stats(size_t* live, size_t* cached)
+{
+  DevicePool& p = pool_for_device(current_device());
+  std::lock_guard<std::mutex> g(p.mu);
+  *live   = p.live_bytes;
+  *cached = p.cached_bytes;
 }
 
 size_t pool_bytes_in_use()
@@ -269,20 +417,41 @@ void profile_end(void* token, hipStream_t stream)
 void* OutputAllocator::allocate(size_t bytes)
 {
   if (bytes == 0) return nullptr;
-  if (!mr_) return pool_alloc(bytes);
-  void* p = mr_->allocate(mr_->ctx, bytes, stream_);
-  if (!p) throw oom_error("out of memory: caller resource returned NULL for " +
-                          std::to_string(bytes) + " bytes");
+  owned_.reserve(owned_.size() + 1);  // the push below cannot throw after the allocation
+  void* p = nullptr;
+  if (!mr_) {
+    p = pool_alloc(bytes);
+  } else {
+    p = mr_->allocate(mr_->ctx, bytes, stream_);
+    if (!p) throw oom_error("out of memory: caller resource returned NULL for " +
+                            std::to_string(bytes) + " bytes");
+  }
+  owned_.push_back({p, bytes});
   return p;
 }
 
 void OutputAllocator::deallocate(void* p, size_t bytes)
 {
   if (!p) return;
+  for (size_t i = 0; i < owned_.size(); ++i)
+    if (owned_[i].p == p) {
+      owned_.erase(owned_.begin() + static_cast<std::ptrdiff_t>(i));
+      break;
+    }
   if (!mr_)
     pool_free(p);
   else
     mr_->deallocate(mr_->ctx, p, bytes, stream_);
+}
+
+OutputAllocator::~OutputAllocator()
+{
+  for (auto it = owned_.rbegin(); it != owned_.rend(); ++it) {
+    if (!mr_)
+      pool_free(it->p);
+    else
+      mr_->deallocate(mr_->ctx, it->p, it->bytes, stream_);
+  }
 }
 
 // --- tuning parameters ---------------------------------------------------------
@@ -378,6 +547,17 @@ int64_t qpip_profile_report(char* buf, size_t len)
     buf[n] = 0;
   }
   return static_cast<int64_t>(out.size());
+}
+
+void qpip_memory_info(size_t* live_bytes, size_t* cached_bytes)
+{
+  size_t live = 0, cached = 0;
+  try {
+    qpip::pool_stats(&live, &cached);
+  } catch (...) {
+  }
+  if (live_bytes) *live_bytes = live;
+  if (cached_bytes) *cached_bytes = cached;
 }
 
 void qpip_trim(void)

@@ -42,6 +42,7 @@ class Workload:
     max_depth: int
     max_size: int
     seed: int
+    centers: np.ndarray = None  # (P, 2) polygon centres, where the generator has them
 
     @property
     def num_points(self) -> int:
@@ -259,7 +260,7 @@ def gaussian_holes(seed: int = 1002, num_points: int = 100_000_000,
     ring_offset = np.concatenate([[0], np.cumsum(sizes)])
     part_offset = np.concatenate([[0], np.cumsum(rings_per_poly)]).astype(np.int32)
     return Workload("gaussian_holes", x, y, part_offset, ring_offset.astype(np.int32), px, py,
-                    0.0, ext, 0.0, ext, ext / 2**15, 15, 256, seed)
+                    0.0, ext, 0.0, ext, ext / 2**15, 15, 256, seed, np.stack([cx, cy], 1))
 
 
 def subsample(w: Workload, n: int) -> Workload:
@@ -372,17 +373,22 @@ WORKLOADS = {"uniform_stars": uniform_stars, "gaussian_holes": gaussian_holes,
 # ---------------------------------------------------------------------------
 # Voronoi partition (non-overlapping polygons covering the domain)
 # ---------------------------------------------------------------------------
-def voronoi_cells(seed: int, num_seeds: int, ext: float):
-    """Clipped Voronoi tessellation of `num_seeds` uniform seeds in [0, ext)^2 as
-    (part_offset, ring_offset, px, py): one closed counter-clockwise ring per
-    cell. The seeds are mirrored across the four sides of the square so that
-    every original cell is bounded and ends on the square's sides. Neighbouring
-    cells share their common vertices bit for bit (one Qhull vertex each), so the
-    cells do not overlap."""
+def voronoi_cells(seed: int, num_seeds: int, ext: float, seeds: np.ndarray = None):
+    """Clipped Voronoi tessellation of `num_seeds` uniform seeds in [0, ext)^2 (or
+    of the given `seeds`, (n, 2) points inside the square) as (part_offset,
+    ring_offset, px, py): one closed counter-clockwise ring per cell. The seeds
+    are mirrored across the four sides of the square so that every original cell
+    is bounded and ends on the square's sides. Neighbouring cells share their
+    common vertices bit for bit (one Qhull vertex each), so the cells do not
+    overlap."""
     from scipy.spatial import Voronoi
 
-    rng = np.random.Generator(np.random.PCG64([seed, 2]))
-    s = rng.uniform(0.0, ext, (num_seeds, 2))
+    if seeds is None:
+        rng = np.random.Generator(np.random.PCG64([seed, 2]))
+        s = rng.uniform(0.0, ext, (num_seeds, 2))
+    else:
+        s = np.asarray(seeds, np.float64)
+        num_seeds = s.shape[0]
     mirrored = [s,
                 np.stack([-s[:, 0], s[:, 1]], 1), np.stack([2 * ext - s[:, 0], s[:, 1]], 1),
                 np.stack([s[:, 0], -s[:, 1]], 1), np.stack([s[:, 0], 2 * ext - s[:, 1]], 1)]
@@ -419,3 +425,15 @@ def voronoi_partition(seed: int = 1004, num_points: int = 200_000_000, num_seeds
     part, ring, px, py = voronoi_cells(seed, num_seeds, ext)
     return Workload("voronoi_partition", x, y, part, ring, px, py, 0.0, ext, 0.0, ext,
                     ext / 2**15, 15, 256, seed)
+
+
+def gaussian_holes_partition(seed: int = 1002, num_points: int = 0, device=None) -> Workload:
+    """The Voronoi-partition variant of gaussian_holes' polygon set: the clipped
+    Voronoi cells of its 50,000 polygon centres (convex, non-overlapping,
+    covering [0, 1e5)^2), with gaussian_holes' points (num_points of them) and
+    quadtree parameters. The resident partition index of the serving workload."""
+    gh = gaussian_holes(seed=seed, num_points=max(num_points, 1), device=device)
+    part, ring, px, py = voronoi_cells(seed, 0, 1e5, seeds=gh.centers)
+    x, y = (gh.x, gh.y) if num_points > 0 else (gh.x[:0], gh.y[:0])
+    return Workload("gaussian_holes_partition", x, y, part, ring, px, py, gh.x_min, gh.x_max,
+                    gh.y_min, gh.y_max, gh.scale, gh.max_depth, gh.max_size, seed, gh.centers)

@@ -155,6 +155,16 @@ class CandidateGrid:
         return torch.where(ok, slot, torch.full_like(slot, -1))
 
 
+def random_permutation(n: int, gen: torch.Generator, device) -> torch.Tensor:
+    """A random permutation of 0..n-1 (int64): random 32-bit keys, stably sorted
+    with the library's radix sort (ties keep index order)."""
+    keys = torch.randint(-(1 << 31), 1 << 31, (n,), generator=gen, device=device,
+                         dtype=torch.int64).to(torch.int32)
+    idx = torch.arange(n, dtype=torch.int32, device=device)
+    _capi.sort_pairs(keys, idx, 32)
+    return idx.to(torch.int64)
+
+
 class Ranker:
     """MLP parameters (fp32 master copy, bf16 copy for the kernels) and Adam state."""
 
@@ -247,7 +257,7 @@ class Ranker:
         gen.manual_seed(seed)
         history = []
         for ep in range(epochs):
-            perm = torch.randperm(n, generator=gen, device=pts.device)
+            perm = random_permutation(n, gen, pts.device)
             tot = torch.zeros(3, dtype=torch.float64, device=pts.device)
             for s in range(0, n, batch):
                 idx = perm[s:s + batch]
@@ -272,15 +282,34 @@ class LocateResult:
     seconds: dict
 
     def pairs(self):
-        """(polygon_index, point_index) sorted by point, then polygon."""
-        pt = torch.nonzero(self.polygon >= 0, as_tuple=True)[0]
-        poly = self.polygon[pt].to(torch.int64)
-        if self.extra_pairs[0].numel():
-            poly = torch.cat([poly, self.extra_pairs[0]])
-            pt = torch.cat([pt, self.extra_pairs[1]])
-        key = pt * (1 << 32) + poly
-        order = torch.argsort(key)
-        return poly[order], pt[order]
+        """(polygon_index, point_index) sorted by point, then polygon (int64)."""
+        pt, poly = _capi.compact_nonnegative(self.polygon)
+        if not self.extra_pairs[0].numel():
+            return poly.to(torch.int64), pt
+        # A point with extra pairs has its smallest polygon in self.polygon and the
+        # rest, ascending, in extra_pairs: a stable sort by point keeps that order.
+        return sort_by_point(torch.cat([poly.to(torch.int64), self.extra_pairs[0]]),
+                             torch.cat([pt, self.extra_pairs[1]]), self.polygon.numel())
+
+
+def sort_by_point(poly: torch.Tensor, pt: torch.Tensor, n: int):
+    """(poly, pt) reordered by point with the library's stable radix sort (pairs
+    of one point keep their order); n bounds the point ids. int64 in and out."""
+    if pt.numel() == 0:
+        return poly.to(torch.int64), pt.to(torch.int64)
+    keys = pt.to(torch.int32).contiguous()
+    vals = poly.to(torch.int32).contiguous()
+    _capi.sort_pairs(keys, vals, max(1, int(max(n, 1) - 1).bit_length()))
+    return vals.to(torch.int64) & 0xFFFFFFFF, keys.to(torch.int64) & 0xFFFFFFFF
+
+
+def join_pairs_by_point(pi: torch.Tensor, hits, n: int):
+    """The quadtree PIP hits as (polygon_index, original point index), sorted by
+    point, then polygon: the PIP emits a point's hits in pair order, which is
+    ascending polygon within its leaf, so a stable sort by point suffices."""
+    pos = hits["point_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+    pt = pi.view(torch.int32)[pos]
+    return sort_by_point(hits["polygon_index"].view(torch.int32), pt, n)
 
 
 def plain_pairs(points: Points, polygons: Polygons, bbox, scale, max_depth, max_size):
@@ -291,11 +320,7 @@ def plain_pairs(points: Points, polygons: Polygons, bbox, scale, max_depth, max_
     bb = polygon_bounding_boxes(polygons)
     pairs = join_quadtree_and_bounding_boxes(tree, bb, x0, x1, y0, y1, scale, max_depth)
     hits = quadtree_point_in_polygon(pairs, tree, pi, points, polygons)
-    pos = hits["point_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-    pt = pi.view(torch.int32).to(torch.int64)[pos] & 0xFFFFFFFF
-    poly = hits["polygon_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-    order = torch.argsort(pt * (1 << 32) + poly)
-    return poly[order], pt[order]
+    return join_pairs_by_point(pi, hits, len(points))
 
 
 def per_point_polygon(poly: torch.Tensor, pt: torch.Tensor, n: int) -> torch.Tensor:
@@ -307,11 +332,143 @@ def per_point_polygon(poly: torch.Tensor, pt: torch.Tensor, n: int) -> torch.Ten
     return out
 
 
+def _polygon_edges(polygons: Polygons):
+    """Every ring edge (v_k, v_{k+1 mod n}) of every polygon, zero-length ones
+    dropped: (edge polygon id, ax, ay, bx, by) sorted by polygon, plus each
+    polygon's first edge and edge count."""
+    part = polygons.part_offset.to(torch.int64)
+    ring = polygons.ring_offset.to(torch.int64)
+    dev = ring.device
+    nr = ring.numel() - 1
+    rlen = ring[1:] - ring[:-1]
+    v = torch.arange(int(ring[-1]) if nr > 0 else 0, device=dev)
+    r_of_v = torch.repeat_interleave(torch.arange(nr, device=dev), rlen)
+    nxt = v + 1
+    nxt = torch.where(nxt == ring[1:][r_of_v], ring[:-1][r_of_v], nxt)
+    p_of_r = torch.repeat_interleave(torch.arange(part.numel() - 1, device=dev), part[1:] - part[:-1])
+    ax, ay = polygons.x[v], polygons.y[v]
+    bx, by = polygons.x[nxt], polygons.y[nxt]
+    keep = (ax != bx) | (ay != by)
+    pid = p_of_r[r_of_v][keep]
+    cnt = torch.bincount(pid, minlength=part.numel() - 1)
+    first = torch.cumsum(cnt, 0) - cnt
+    return pid, ax[keep], ay[keep], bx[keep], by[keep], first, cnt
+
+
+def _pair_rows(first_a, cnt_a, first_b, cnt_b):
+    """For pairs (a, b): every (row of a's range, row of b's range) combination."""
+    k = cnt_a * cnt_b
+    tot = int(k.sum())
+    pair = torch.repeat_interleave(torch.arange(k.numel(), device=k.device), k)
+    local = torch.arange(tot, device=k.device) - (torch.cumsum(k, 0) - k)[pair]
+    return pair, first_a[pair] + local // cnt_b[pair], first_b[pair] + local % cnt_b[pair]
+
+
+def overlapping_polygons(polygons: Polygons, grid: CandidateGrid, rel_tol: float = 2.0**-46,
+                         first_only: bool = False):
+    """Pairs of polygons listed together in a grid cell whose interiors overlap:
+    a vertex of one strictly inside the other (crossing parity over all rings,
+    off every edge), or two edges that cross properly. Shared vertices and
+    shared edges, as the cells of a partition have, do not count: their
+    orientation products are exactly zero. Orientations within rel_tol of the
+    largest squared coordinate count as zero (the rounding level of the
+    products), so an overlap thinner than that is not reported. Two polygons
+    whose interiors overlap have overlapping boxes, and so share a listed cell
+    unless the overlap lies only in overflowed cells, whose points take the
+    quadtree path. Returns (i, j) int64 tensors (empty for a partition); with
+    first_only, the pairs of the first batch that has any."""
+    dev = grid.candidates.device
+    cand = grid.candidates.view(-1, MAX_CANDIDATES).to(torch.int64)
+    listed = grid.counts.to(torch.int64) <= MAX_CANDIDATES
+    a, b = torch.triu_indices(MAX_CANDIDATES, MAX_CANDIDATES, 1, device=dev)
+    ci, cj = cand[:, a], cand[:, b]
+    ok = (ci >= 0) & (cj >= 0) & listed[:, None]
+    P = polygons.part_offset.numel() - 1
+    key = torch.unique(torch.minimum(ci, cj)[ok] * P + torch.maximum(ci, cj)[ok])
+    pi, pj = key // P, key % P
+    if pi.numel() == 0:
+        return pi, pj
+    pid, ax, ay, bx, by, first, cnt = _polygon_edges(polygons)
+    scale = float(torch.max(torch.cat([ax.abs(), ay.abs()])).item()) if ax.numel() else 1.0
+    tol = rel_tol * max(scale, 1.0) ** 2
+    hit_i, hit_j = [], []
+    # both directions of the vertex test; edges a -> b of one polygon are its
+    # vertices (each vertex starts one edge)
+    chunk = 1 << 22
+    pairs_i = torch.cat([pi, pj])
+    pairs_j = torch.cat([pj, pi])
+    work = cnt[pairs_i] * cnt[pairs_j]
+    bounds = torch.cumsum(work, 0)
+    start = 0
+    n = pairs_i.numel()
+    while start < n:
+        base = int(bounds[start - 1]) if start else 0
+        end = int(torch.searchsorted(bounds, torch.tensor(base + chunk, device=dev)).item())
+        end = max(end, start + 1)
+        end = min(end, n)
+        I, J = pairs_i[start:end], pairs_j[start:end]
+        pr, ei, ej = _pair_rows(first[I], cnt[I], first[J], cnt[J])
+        # vertex (ax[ei], ay[ei]) of polygon I against edge ej of polygon J
+        px, py = ax[ei], ay[ei]
+        qax, qay, qbx, qby = ax[ej], ay[ej], bx[ej], by[ej]
+        o = (qbx - qax) * (py - qay) - (qby - qay) * (px - qax)
+        on = (o.abs() <= tol) & (px >= torch.minimum(qax, qbx)) & (px <= torch.maximum(qax, qbx)) \
+            & (py >= torch.minimum(qay, qby)) & (py <= torch.maximum(qay, qby))
+        straddle = (qay > py) != (qby > py)
+        xint = qax + (qbx - qax) * (py - qay) / torch.where(qby != qay, qby - qay,
+                                                            torch.ones_like(qay))
+        cross = straddle & (px < xint)
+        nv = ax.numel()
+        # per (pair, vertex): crossing parity and any on-edge (a vertex is the
+        # start of one edge, so ei names it)
+        vk = pr * nv + ei
+        uk, inv = torch.unique(vk, return_inverse=True)
+        parity = torch.zeros(uk.numel(), dtype=torch.int64, device=dev).scatter_add_(
+            0, inv, cross.to(torch.int64)) & 1
+        onany = torch.zeros(uk.numel(), dtype=torch.int64, device=dev).scatter_add_(
+            0, inv, on.to(torch.int64)) > 0
+        inside = (parity == 1) & ~onany
+        # proper crossings of edge ei (of I) with edge ej (of J); each unordered
+        # pair appears in both directions, so one direction suffices
+        fwd = I[pr] < J[pr]
+        o1 = (qbx - qax) * (ay[ei] - qay) - (qby - qay) * (ax[ei] - qax)
+        o2 = (qbx - qax) * (by[ei] - qay) - (qby - qay) * (bx[ei] - qax)
+        o3 = (bx[ei] - ax[ei]) * (qay - ay[ei]) - (by[ei] - ay[ei]) * (qax - ax[ei])
+        o4 = (bx[ei] - ax[ei]) * (qby - ay[ei]) - (by[ei] - ay[ei]) * (qbx - ax[ei])
+        proper = fwd & (((o1 > tol) & (o2 < -tol)) | ((o1 < -tol) & (o2 > tol))) & \
+            (((o3 > tol) & (o4 < -tol)) | ((o3 < -tol) & (o4 > tol)))
+        bad_pairs = torch.cat([torch.unique(uk[inside] // nv), torch.unique(pr[proper])])
+        if bad_pairs.numel():
+            bp = torch.unique(bad_pairs)
+            hit_i.append(torch.minimum(I[bp], J[bp]))
+            hit_j.append(torch.maximum(I[bp], J[bp]))
+            if first_only:
+                break
+        start = end
+    if not hit_i:
+        z = torch.zeros(0, dtype=torch.int64, device=dev)
+        return z, z
+    k = torch.unique(torch.cat(hit_i) * P + torch.cat(hit_j))
+    return k // P, k % P
+
+
 class LearnedLocator:
-    """Grid + ranker + exact test, with the plain quadtree path as the fallback."""
+    """Grid + ranker + exact test, with the plain quadtree path as the fallback.
+
+    The locator answers each point with the first candidate whose exact test
+    holds, which is the plain join's answer only when no two polygons overlap.
+    Construction checks that (``overlapping_polygons``) and raises ValueError for
+    a polygon set that is not a partition."""
 
     def __init__(self, polygons: Polygons, grid: CandidateGrid, ranker: Ranker, bbox, scale,
-                 max_depth, max_size):
+                 max_depth, max_size, check_partition: bool = True):
+        if check_partition:
+            oi, oj = overlapping_polygons(polygons, grid, first_only=True)
+            if oi.numel():
+                raise ValueError(
+                    f"the learned locator needs non-overlapping polygons: {oi.numel()} pairs "
+                    f"overlap (e.g. polygons {int(oi[0])} and {int(oj[0])}); use the plain "
+                    "quadtree join for this polygon set")
         self.polygons = polygons
         self.grid = grid
         self.ranker = ranker
@@ -319,6 +476,50 @@ class LearnedLocator:
         self.scale = scale
         self.max_depth = max_depth
         self.max_size = max_size
+
+    def save(self, path, meta: dict | None = None) -> None:
+        """Writes the whole index to one versioned file (io.save_index): the
+        polygons, the candidate grid, the ranker's parameters and the quadtree
+        parameters of the fallback. ``load`` restores a locator that answers
+        every point exactly as this one does."""
+        from . import io
+
+        P, g = self.polygons, self.grid
+        arrays = {"part_offset": P.part_offset, "ring_offset": P.ring_offset, "px": P.x,
+                  "py": P.y, "grid_candidates": g.candidates, "grid_counts": g.counts,
+                  "ranker_params": self.ranker.params}
+        info = {"kind": "learned_locator", "grid": {"x0": g.x0, "y0": g.y0, "cell": g.cell,
+                                                     "dim": g.dim},
+                "num_cells": self.ranker.num_cells, "bbox": [float(v) for v in self.bbox],
+                "scale": float(self.scale), "max_depth": int(self.max_depth),
+                "max_size": int(self.max_size), "ranker_step": self.ranker.step,
+                "meta": meta or {}}
+        io.save_index(path, {k: v.detach().cpu().numpy() for k, v in arrays.items()}, info)
+
+    @classmethod
+    def load(cls, path, device=None) -> "LearnedLocator":
+        """The locator saved by ``save``, on `device` (the current GPU by
+        default). The partition was checked when the saved locator was built."""
+        from . import io
+
+        dev = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        arrays, info = io.load_index(path, device=dev)
+        if info.get("kind") != "learned_locator":
+            raise ValueError(f"{path}: not a saved learned locator")
+        P = Polygons(arrays["part_offset"], arrays["ring_offset"], arrays["px"], arrays["py"],
+                     device=dev)
+        gi = info["grid"]
+        grid = CandidateGrid(float(gi["x0"]), float(gi["y0"]), float(gi["cell"]), int(gi["dim"]),
+                             arrays["grid_candidates"].to(torch.int32).contiguous(),
+                             arrays["grid_counts"].to(torch.uint8).contiguous())
+        model = Ranker(int(info["num_cells"]), device=dev, zero=True)
+        if arrays["ranker_params"].numel() != model.params.numel():
+            raise ValueError(f"{path}: ranker parameters do not match the grid")
+        model.set_params(arrays["ranker_params"])
+        model.step = int(info.get("ranker_step", 0))
+        return cls(P, grid, model, tuple(info["bbox"]), info["scale"], info["max_depth"],
+                   info["max_size"], check_partition=False)
 
     def locate(self, points: Points, max_tests: int = MAX_CANDIDATES) -> LocateResult:
         if points.xy is None:
@@ -338,7 +539,8 @@ class LearnedLocator:
             P.x.numel(), xy.data_ptr(), n, int(max_tests), out.data_ptr(), tests.data_ptr(),
             flags.data_ptr(), _capi.stream_handle(dev)))
         t1 = time.perf_counter()
-        fb = torch.nonzero(out == -2, as_tuple=True)[0]
+        fb, _ = _capi.compact_nonnegative(
+            torch.where(out == -2, 0, -1).to(torch.int32), with_values=False)
         extra = (torch.zeros(0, dtype=torch.int64, device=dev),
                  torch.zeros(0, dtype=torch.int64, device=dev))
         if fb.numel():

@@ -5,9 +5,12 @@ and trained ranker) on the device. ``Server`` runs the continuous-batching loop:
 
 * requests (a slice of points each) arrive with their own arrival times;
 * the scheduler admits waiting requests into a batch whenever a worker slot is
-  free and the batch's device memory fits the budget (bytes per point in
-  flight x points, summed over the batches on the device); a request larger
-  than the whole budget runs alone once nothing else is in flight;
+  free and the batch's device memory fits what the device has free: the free
+  HBM that hipMemGetInfo reports, plus the bytes torch's and the library's
+  caching allocators hold unused, minus the reservations of the batches in
+  flight (bytes per point x points each) and a fixed reserve; an optional
+  budget caps the reservations further. A request that does not fit runs alone
+  once nothing else is in flight;
 * each worker owns a HIP stream (default priority) and a host thread, so
   batches that are admitted while others run overlap on the device (the library
   calls release the GIL);
@@ -45,9 +48,21 @@ from .spatial import (
 class ResidentIndex:
     """Polygons resident on the device plus the parameters each batch's quadtree
     uses (bbox, scale, max_depth, max_size). With a ``locator`` the batch is
-    answered by the learned locator (partitions only)."""
+    answered by the learned locator (partitions only).
 
-    def __init__(self, polygons: Polygons, bbox, scale, max_depth, max_size, locator=None):
+    Without a locator and with ``captured`` (the default), a batch is answered
+    by replaying a captured plan (one HIP graph launch, BatchPlan) instead of
+    the four eager entry points. Plans are kept per stream and per size bucket
+    (the next power of two of the batch's points, at least ``min_plan_points``):
+    the batch's points are copied into the plan's point buffer and the rest of
+    the bucket is filled with NaN points, which land in one quadtree cell and
+    are inside no polygon, so the answer is the same. A batch that outgrows the
+    plan's capacities (CapacityError) rebuilds the plan from its own points;
+    batches above ``max_plan_points`` take the eager path."""
+
+    def __init__(self, polygons: Polygons, bbox, scale, max_depth, max_size, locator=None,
+                 captured: bool = True, min_plan_points: int = 1 << 12,
+                 max_plan_points: int = 1 << 24, plan_headroom: float = 4.0):
         self.polygons = polygons
         self.bbox = tuple(float(v) for v in bbox)
         self.scale = float(scale)
@@ -55,6 +70,65 @@ class ResidentIndex:
         self.max_size = int(max_size)
         self.locator = locator
         self.boxes = polygon_bounding_boxes(polygons)  # resident across batches
+        self.captured = captured and locator is None
+        self.min_plan_points = int(min_plan_points)
+        self.max_plan_points = int(max_plan_points)
+        self.plan_headroom = float(plan_headroom)
+        self._plans: dict = {}  # (stream handle, bucket) -> (BatchPlan, point buffer)
+        self._plans_lock = threading.Lock()
+        self.plan_stats = collections.Counter()
+
+    def close(self):
+        """Releases the captured plans."""
+        with self._plans_lock:
+            for plan, _ in self._plans.values():
+                plan.close()
+            self._plans.clear()
+
+    def _plan_for(self, key, buf):
+        from .spatial import BatchPlan
+
+        plan = BatchPlan(Points.from_points_xy(buf), [0, buf.numel() // 2], self.polygons,
+                         [0, len(self.polygons)], [self.bbox], [self.scale], [self.max_depth],
+                         [self.max_size], headroom=self.plan_headroom)
+        with self._plans_lock:
+            self._plans[key] = (plan, buf)
+        self.plan_stats["created"] += 1
+        return plan
+
+    def _query_captured(self, xy: torch.Tensor):
+        from ._capi import CapacityError
+        from .ranker import sort_by_point
+
+        n = xy.numel() // 2
+        bucket = max(self.min_plan_points, 1 << max(n - 1, 1).bit_length())
+        if n == 0 or bucket > self.max_plan_points:
+            return None
+        stream = torch.cuda.current_stream(xy.device)
+        key = (stream.cuda_stream, bucket)
+        with self._plans_lock:
+            entry = self._plans.get(key)
+        if entry is None:
+            buf = torch.full((2 * bucket,), math.nan, dtype=torch.float64, device=xy.device)
+            buf[:2 * n].copy_(xy)
+            plan = self._plan_for(key, buf)
+        else:
+            plan, buf = entry
+            buf[:2 * n].copy_(xy)
+            buf[2 * n:].fill_(math.nan)
+        plan.launch()
+        try:
+            res = plan.result()
+        except CapacityError:
+            plan.close()
+            self.plan_stats["rebuilt"] += 1
+            plan = self._plan_for(key, buf)
+            plan.launch()
+            res = plan.result()
+        self.plan_stats["replays"] += 1
+        pos = res.point_index.view(torch.int32).to(torch.int64) & 0xFFFFFFFF
+        pt = res.point_indices.view(torch.int32)[pos]
+        return sort_by_point(res.polygon_index.view(torch.int32), pt, n)
 
     def query(self, xy: torch.Tensor):
         """(polygon_index, point_index) pairs of the points in `xy`, sorted by
@@ -62,17 +136,20 @@ class ResidentIndex:
         pts = Points.from_points_xy(xy)
         if self.locator is not None:
             return self.locator.locate(pts).pairs()
+        if self.captured:
+            out = self._query_captured(pts.xy)
+            if out is not None:
+                return out
+        self.plan_stats["eager"] += 1
         x0, x1, y0, y1 = self.bbox
         pi, tree = quadtree_on_points(pts, x0, x1, y0, y1, self.scale, self.max_depth,
                                       self.max_size)
         pairs = join_quadtree_and_bounding_boxes(tree, self.boxes, x0, x1, y0, y1, self.scale,
                                                  self.max_depth)
         hits = quadtree_point_in_polygon(pairs, tree, pi, pts, self.polygons)
-        pos = hits["point_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-        pt = pi.view(torch.int32).to(torch.int64)[pos] & 0xFFFFFFFF
-        poly = hits["polygon_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-        order = torch.argsort(pt * (1 << 32) + poly)
-        return poly[order], pt[order]
+        from .ranker import join_pairs_by_point
+
+        return join_pairs_by_point(pi, hits, len(pts))
 
 
 @dataclass
@@ -118,18 +195,37 @@ def worker_stream(device: torch.device, k: int) -> torch.cuda.Stream:
     return _STREAMS[key]
 
 
-class Server:
-    """Continuous batching over a ResidentIndex with `workers` device slots."""
+def free_device_bytes(device: torch.device) -> int:
+    """Device memory a new batch can use now: hipMemGetInfo's free bytes plus
+    what torch's caching allocator and the library's pool hold cached (both
+    reuse those blocks before asking the device for more)."""
+    from . import _capi
 
-    def __init__(self, index: ResidentIndex, workers: int = 3, memory_budget: int = 8 << 30,
+    free, _ = torch.cuda.mem_get_info(device)
+    torch_cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    _, lib_cached = _capi.memory_info()
+    return int(free) + int(torch_cached) + int(lib_cached)
+
+
+class Server:
+    """Continuous batching over a ResidentIndex with `workers` device slots.
+
+    Admission reserves bytes_per_point per point of a batch (a bound on what the
+    plain path allocates per point: sort buffers, tree, PIP workspace, outputs)
+    against the device's free memory (``free_device_bytes``) less the
+    reservations in flight and `reserve_bytes`; `memory_budget`, when given,
+    also caps the sum of the reservations."""
+
+    def __init__(self, index: ResidentIndex, workers: int = 3, memory_budget: int | None = None,
                  bytes_per_point: int = 256, max_batch_points: int = 1 << 24,
-                 keep_results: bool = False):
+                 keep_results: bool = False, reserve_bytes: int = 2 << 30):
         self.index = index
         self.workers = workers
         self.memory_budget = memory_budget
         self.bytes_per_point = bytes_per_point
         self.max_batch_points = max_batch_points
         self.keep_results = keep_results
+        self.reserve_bytes = reserve_bytes
 
     def _run_batch(self, stream, batch):
         with torch.cuda.stream(stream):
@@ -153,7 +249,8 @@ class Server:
         work: queue.Queue = queue.Queue()
         lock = threading.Condition()
         state = {"in_flight_bytes": 0, "in_flight": 0, "max_in_flight": 0, "done": 0,
-                 "error": None, "max_in_flight_bytes": 0}
+                 "error": None, "max_in_flight_bytes": 0, "min_free_bytes": None,
+                 "memory_waits": 0}
         t0 = time.perf_counter()
 
         def worker(k):
@@ -197,13 +294,19 @@ class Server:
                     admitted = False
                     if pending and state["in_flight"] < self.workers:
                         batch, pts = [], 0
-                        free = self.memory_budget - state["in_flight_bytes"]
+                        dev_free = free_device_bytes(dev)
+                        if state["min_free_bytes"] is None or dev_free < state["min_free_bytes"]:
+                            state["min_free_bytes"] = dev_free
+                        free = dev_free - state["in_flight_bytes"] - self.reserve_bytes
+                        if self.memory_budget is not None:
+                            free = min(free, self.memory_budget - state["in_flight_bytes"])
                         while pending:
                             r = pending[0]
                             need = (pts + r.points) * self.bytes_per_point
                             if batch and (pts + r.points > self.max_batch_points or need > free):
                                 break
                             if not batch and need > free and state["in_flight"] > 0:
+                                state["memory_waits"] += 1
                                 break  # wait for memory to free up
                             batch.append(pending.popleft())
                             pts += r.points
@@ -238,7 +341,9 @@ class Server:
         return ServeStats(total, sum(r.points for r in requests), elapsed, pct, batches,
                           float(np.mean(batch_sizes)) if batch_sizes else 0.0,
                           state["max_in_flight"],
-                          extra={"max_in_flight_bytes": state["max_in_flight_bytes"]})
+                          extra={"max_in_flight_bytes": state["max_in_flight_bytes"],
+                                 "min_free_bytes": state["min_free_bytes"],
+                                 "memory_waits": state["memory_waits"]})
 
 
 def poisson_requests(pool_xy: torch.Tensor, qps: float, count: int, seed: int = 0,

@@ -486,7 +486,31 @@ class BatchPlan:
 
     def __init__(self, points: Points, job_point_offsets, polygons: Polygons,
                  job_polygon_offsets, bboxes, scales, max_depths, max_sizes,
-                 headroom: float = 1.25):
+                 headroom: float = 1.25, tuning="default"):
+        """`tuning`: "default" looks the batch's shape bucket up in the package's
+        tuning table (autotune.DEFAULT_TABLE); a TuningTable or a path looks it up
+        there; a dict is used as is; None captures under the current values. The
+        values apply while the plan is sized and captured, and are restored
+        after."""
+        from . import autotune
+
+        ring_n = polygons.ring_offset.numel() - 1
+        edges = max(polygons.x.numel() - max(ring_n, 0), 0)
+        self.bucket = autotune.shape_bucket(len(job_point_offsets) - 1, len(points),
+                                            max(len(polygons), 1), edges)
+        if isinstance(tuning, dict) or tuning is None:
+            cfg = tuning
+        else:
+            table = tuning if isinstance(tuning, autotune.TuningTable) else autotune.TuningTable(
+                autotune.DEFAULT_TABLE if tuning == "default" else tuning)
+            cfg = table.lookup(self.bucket)
+        self.tuning = dict(cfg) if cfg else None
+        with autotune.applied(self.tuning):
+            self._create(points, job_point_offsets, polygons, job_polygon_offsets, bboxes,
+                         scales, max_depths, max_sizes, headroom)
+
+    def _create(self, points, job_point_offsets, polygons, job_polygon_offsets, bboxes, scales,
+                max_depths, max_sizes, headroom):
         if not isinstance(points, Points) or points.xy is None:
             raise ValueError("a plan needs interleaved points (Points.from_points_xy)")
         if not isinstance(polygons, Polygons):

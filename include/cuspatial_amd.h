@@ -10,9 +10,14 @@
  * Conventions
  *  - All array arguments are device pointers (HBM). Coordinates are float64;
  *    the reference's float32 instantiation is not provided.
- *  - `stream` is a hipStream_t (NULL: the default stream). Every entry point
- *    returns after its work on `stream` is complete, as the reference's column
- *    API does.
+ *  - `stream` is a hipStream_t (NULL: the default stream). Entry points are
+ *    stream-ordered, as the reference's header-only API is: the work is queued
+ *    on `stream`, and the outputs are ready for any work queued on `stream`
+ *    after the call returns. The host waits only where a data-dependent size
+ *    must be known to allocate an output (at most once per call of the
+ *    quadtree, bounding-box, join and point-in-polygon entry points; a second
+ *    time when a join's candidates outgrow the first guess of their count).
+ *    Synchronise `stream` before reading outputs from the host.
  *  - Outputs are allocated from `mr` when it is non-NULL (the counterpart of the
  *    reference's `rmm::device_async_resource_ref mr` argument); with mr == NULL
  *    they come from the library's pool and are released with qpip_free().
@@ -58,7 +63,9 @@ QPIP_API const char* qpip_last_error_message(void);
 /* Library version string. */
 QPIP_API const char* qpip_version(void);
 
-/* Releases a buffer returned with mr == NULL. NULL is ignored. */
+/* Releases a buffer returned with mr == NULL. NULL is ignored. The library's
+ * own writes to it are ordered before any reuse; the caller's uses must have
+ * finished (as with hipFree). */
 QPIP_API void qpip_free(void* ptr);
 
 /* Caps the bytes the library's pool may hold in live buffers (0: no cap).
@@ -67,6 +74,12 @@ QPIP_API void qpip_set_memory_limit(size_t bytes);
 
 /* Returns cached, unused pool memory to the device. */
 QPIP_API void qpip_trim(void);
+
+/* Bytes the library's pool holds on the current device: in live buffers
+ * (temporaries of running calls and pool-allocated outputs) and cached for
+ * reuse. No counterpart in the reference (RMM reports its own pools); a server
+ * admitting work by free device memory counts the cached bytes as free. */
+QPIP_API void qpip_memory_info(size_t* live_bytes, size_t* cached_bytes);
 
 /* Tuning parameters (no counterpart in the reference; set by the autotuner,
  * cuspatial_amd/autotune.py). Process-wide, read when a kernel is launched:
@@ -526,6 +539,36 @@ QPIP_API int qpip_quadtree_point_to_nearest_linestring(const uint32_t* linestrin
                                                        uint32_t** linestring_index_out,
                                                        double** distance,
                                                        int64_t* num_rows);
+
+/*
+ * Stable radix sort of (key, value) pairs on the low key_bits bits of the keys,
+ * in place (device arrays of n uint32). Replaces thrust::stable_sort_by_key on
+ * uint32 keys, the reference's sort primitive on this path
+ * (detail/index/construction/phase_1.cuh:92,
+ * detail/join/quadtree_bbox_filtering.cuh:161-165); the same kernels as the
+ * quadtree's point sort. Pairs with equal keys keep their order. The host does
+ * not wait.
+ */
+QPIP_API int qpip_radix_sort_pairs(uint32_t* keys,
+                                   uint32_t* values,
+                                   int64_t n,
+                                   int32_t key_bits,
+                                   void* stream);
+
+/*
+ * Stable compaction: the indices (and, when values_out is not NULL, the values)
+ * of the entries of values[0..n) that are >= 0, in index order; *count of them.
+ * indices_out / values_out need room for n entries. Replaces a stable
+ * thrust::copy_if (the reference compacts its hits this way,
+ * detail/join/quadtree_point_in_polygon.cuh:198-207). The host waits once, for
+ * the count.
+ */
+QPIP_API int qpip_compact_nonnegative(const int32_t* values,
+                                      int64_t n,
+                                      uint32_t* indices_out,
+                                      int32_t* values_out,
+                                      int64_t* count,
+                                      void* stream);
 
 /*
  * Learned candidate ranking over a coarse grid (no counterpart in the

@@ -14,6 +14,13 @@
  * Inputs are device arrays with their lengths (the reference takes
  * cudf::column_view); outputs are move-only device columns that own their
  * memory (the reference returns std::unique_ptr<cudf::column> / cudf::table).
+ * Every function ends with the reference's stream and memory resource
+ * arguments (cpp/include/cuspatial/spatial_join.cuh:73-74,
+ * point_quadtree.cuh:179-180): `stream` (a hipStream_t; the work is ordered on
+ * it and the outputs are ready for work queued on it after the call) and `mr`
+ * (a qpip_resource, the counterpart of rmm::device_async_resource_ref: every
+ * output column is allocated from it and released through it; nullptr means
+ * the library's pool).
  * Violated preconditions throw cuspatial_amd::logic_error with the reference's
  * reason; device failures throw cuspatial_amd::device_error (the reference's
  * cuda_error); exhausted memory throws std::bad_alloc.
@@ -52,16 +59,21 @@ inline void check(int rc)
 }
 }  // namespace detail
 
-/* A device column that owns its memory (allocated by the library's pool). */
+/* A device column that owns its memory: allocated from the caller's resource
+ * `mr` (released through it, on the stream of the call that made the column),
+ * or, with mr == nullptr, from the library's pool (released with qpip_free). */
 template <class T>
 class column {
  public:
   column() = default;
-  column(T* data, int64_t size) : data_(data), size_(size) {}
-  ~column() { qpip_free(data_); }
+  column(T* data, int64_t size, qpip_resource const* mr = nullptr, void* stream = nullptr)
+    : data_(data), size_(size), mr_(mr), stream_(stream)
+  {
+  }
+  ~column() { release(); }
   column(column const&)            = delete;
   column& operator=(column const&) = delete;
-  column(column&& o) noexcept : data_(o.data_), size_(o.size_)
+  column(column&& o) noexcept : data_(o.data_), size_(o.size_), mr_(o.mr_), stream_(o.stream_)
   {
     o.data_ = nullptr;
     o.size_ = 0;
@@ -69,9 +81,11 @@ class column {
   column& operator=(column&& o) noexcept
   {
     if (this != &o) {
-      qpip_free(data_);
+      release();
       data_   = o.data_;
       size_   = o.size_;
+      mr_     = o.mr_;
+      stream_ = o.stream_;
       o.data_ = nullptr;
       o.size_ = 0;
     }
@@ -79,10 +93,22 @@ class column {
   }
   T* data() const { return data_; }
   int64_t size() const { return size_; }
+  qpip_resource const* resource() const { return mr_; }
 
  private:
-  T* data_      = nullptr;
-  int64_t size_ = 0;
+  void release()
+  {
+    if (!data_) return;
+    if (mr_)
+      mr_->deallocate(mr_->ctx, data_, static_cast<size_t>(size_) * sizeof(T), stream_);
+    else
+      qpip_free(data_);
+    data_ = nullptr;
+  }
+  T* data_                  = nullptr;
+  int64_t size_             = 0;
+  qpip_resource const* mr_  = nullptr;
+  void* stream_             = nullptr;
 };
 
 /* The five quadtree columns (key UINT32, level UINT8, is_internal_node BOOL8,
@@ -126,18 +152,19 @@ struct point_linestring_distances {
 inline std::pair<column<uint32_t>, point_quadtree> quadtree_on_points(
   double const* x, int64_t num_x, double const* y, int64_t num_y, double x_min, double x_max,
   double y_min, double y_max, double scale, int8_t max_depth, int32_t max_size,
-  void* stream = nullptr)
+  void* stream = nullptr, qpip_resource const* mr = nullptr)
 {
   uint32_t *pi = nullptr, *key = nullptr, *length = nullptr, *offset = nullptr;
   uint8_t *level = nullptr, *internal = nullptr;
   int64_t m = 0;
   detail::check(qpip_quadtree_on_points(x, y, num_x, num_y, x_min, x_max, y_min, y_max, scale,
-                                        max_depth, max_size, nullptr, stream, &pi, &key, &level,
+                                        max_depth, max_size, mr, stream, &pi, &key, &level,
                                         &internal, &length, &offset, &m));
-  point_quadtree t{column<uint32_t>(key, m), column<uint8_t>(level, m),
-                   column<uint8_t>(internal, m), column<uint32_t>(length, m),
-                   column<uint32_t>(offset, m)};
-  return {column<uint32_t>(pi, m > 0 ? num_x : 0), std::move(t)};
+  point_quadtree t{column<uint32_t>(key, m, mr, stream), column<uint8_t>(level, m, mr, stream),
+                   column<uint8_t>(internal, m, mr, stream),
+                   column<uint32_t>(length, m, mr, stream),
+                   column<uint32_t>(offset, m, mr, stream)};
+  return {column<uint32_t>(pi, m > 0 ? num_x : 0, mr, stream), std::move(t)};
 }
 
 /* polygon_bounding_boxes(poly_offsets, ring_offsets, x, y, expansion_radius) */
@@ -145,30 +172,32 @@ inline bounding_boxes polygon_bounding_boxes(int32_t const* poly_offsets, int64_
                                              int32_t const* ring_offsets, int64_t num_ring_offsets,
                                              double const* x, int64_t num_x, double const* y,
                                              int64_t num_y, double expansion_radius = 0.0,
-                                             void* stream = nullptr)
+                                             void* stream = nullptr,
+                                             qpip_resource const* mr = nullptr)
 {
   double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
   int64_t n = 0;
   detail::check(qpip_polygon_bounding_boxes(poly_offsets, num_poly_offsets, ring_offsets,
                                             num_ring_offsets, x, y, num_x, num_y, expansion_radius,
-                                            nullptr, stream, &a, &b, &c, &d, &n));
-  return bounding_boxes{column<double>(a, n), column<double>(b, n), column<double>(c, n),
-                        column<double>(d, n)};
+                                            mr, stream, &a, &b, &c, &d, &n));
+  return bounding_boxes{column<double>(a, n, mr, stream), column<double>(b, n, mr, stream),
+                        column<double>(c, n, mr, stream), column<double>(d, n, mr, stream)};
 }
 
 /* linestring_bounding_boxes(linestring_offsets, x, y, expansion_radius) */
 inline bounding_boxes linestring_bounding_boxes(int32_t const* linestring_offsets,
                                                 int64_t num_linestring_offsets, double const* x,
                                                 int64_t num_x, double const* y, int64_t num_y,
-                                                double expansion_radius, void* stream = nullptr)
+                                                double expansion_radius, void* stream = nullptr,
+                                                qpip_resource const* mr = nullptr)
 {
   double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
   int64_t n = 0;
   detail::check(qpip_linestring_bounding_boxes(linestring_offsets, num_linestring_offsets, x, y,
-                                               num_x, num_y, expansion_radius, nullptr, stream, &a,
-                                               &b, &c, &d, &n));
-  return bounding_boxes{column<double>(a, n), column<double>(b, n), column<double>(c, n),
-                        column<double>(d, n)};
+                                               num_x, num_y, expansion_radius, mr, stream, &a, &b,
+                                               &c, &d, &n));
+  return bounding_boxes{column<double>(a, n, mr, stream), column<double>(b, n, mr, stream),
+                        column<double>(c, n, mr, stream), column<double>(d, n, mr, stream)};
 }
 
 /* join_quadtree_and_bounding_boxes(quadtree, bounding_boxes, x_min, x_max,
@@ -177,7 +206,8 @@ inline bbox_quad_pairs join_quadtree_and_bounding_boxes(point_quadtree const& qu
                                                         bounding_boxes const& boxes, double x_min,
                                                         double x_max, double y_min, double y_max,
                                                         double scale, int8_t max_depth,
-                                                        void* stream = nullptr)
+                                                        void* stream = nullptr,
+                                                        qpip_resource const* mr = nullptr)
 {
   uint32_t *bo = nullptr, *qo = nullptr;
   int64_t k = 0;
@@ -185,8 +215,8 @@ inline bbox_quad_pairs join_quadtree_and_bounding_boxes(point_quadtree const& qu
     quadtree.key.data(), quadtree.level.data(), quadtree.is_internal_node.data(),
     quadtree.length.data(), quadtree.offset.data(), quadtree.size(), boxes.minx.data(),
     boxes.miny.data(), boxes.maxx.data(), boxes.maxy.data(), boxes.size(), x_min, x_max, y_min,
-    y_max, scale, max_depth, nullptr, stream, &bo, &qo, &k));
-  return bbox_quad_pairs{column<uint32_t>(bo, k), column<uint32_t>(qo, k)};
+    y_max, scale, max_depth, mr, stream, &bo, &qo, &k));
+  return bbox_quad_pairs{column<uint32_t>(bo, k, mr, stream), column<uint32_t>(qo, k, mr, stream)};
 }
 
 /* quadtree_point_in_polygon(poly_quad_pairs, quadtree, point_indices, point_x,
@@ -196,7 +226,8 @@ inline polygon_point_pairs quadtree_point_in_polygon(
   column<uint32_t> const& point_indices, double const* x, int64_t num_x, double const* y,
   int64_t num_y, int32_t const* poly_offsets, int64_t num_poly_offsets,
   int32_t const* ring_offsets, int64_t num_ring_offsets, double const* poly_x, int64_t num_poly_x,
-  double const* poly_y, int64_t num_poly_y, void* stream = nullptr)
+  double const* poly_y, int64_t num_poly_y, void* stream = nullptr,
+  qpip_resource const* mr = nullptr)
 {
   uint32_t *po = nullptr, *pt = nullptr;
   int64_t h = 0;
@@ -206,8 +237,9 @@ inline polygon_point_pairs quadtree_point_in_polygon(
     quadtree.offset.data(), quadtree.size(), point_indices.data(), point_indices.size(), x, y,
     num_x, num_y, reinterpret_cast<uint32_t const*>(poly_offsets), num_poly_offsets,
     reinterpret_cast<uint32_t const*>(ring_offsets), num_ring_offsets, poly_x, poly_y,
-    num_poly_x, num_poly_y, nullptr, stream, &po, &pt, &h));
-  return polygon_point_pairs{column<uint32_t>(po, h), column<uint32_t>(pt, h)};
+    num_poly_x, num_poly_y, mr, stream, &po, &pt, &h));
+  return polygon_point_pairs{column<uint32_t>(po, h, mr, stream),
+                             column<uint32_t>(pt, h, mr, stream)};
 }
 
 /* quadtree_point_to_nearest_linestring(linestring_quad_pairs, quadtree,
@@ -218,7 +250,7 @@ inline point_linestring_distances quadtree_point_to_nearest_linestring(
   column<uint32_t> const& point_indices, double const* x, int64_t num_x, double const* y,
   int64_t num_y, uint32_t const* linestring_offsets, int64_t num_linestring_offsets,
   double const* linestring_x, int64_t num_linestring_x, double const* linestring_y,
-  int64_t num_linestring_y, void* stream = nullptr)
+  int64_t num_linestring_y, void* stream = nullptr, qpip_resource const* mr = nullptr)
 {
   uint32_t *pi = nullptr, *li = nullptr;
   double* d = nullptr;
@@ -228,9 +260,10 @@ inline point_linestring_distances quadtree_point_to_nearest_linestring(
     quadtree.level.data(), quadtree.is_internal_node.data(), quadtree.length.data(),
     quadtree.offset.data(), quadtree.size(), point_indices.data(), point_indices.size(), x, y,
     num_x, num_y, linestring_offsets, num_linestring_offsets, linestring_x, linestring_y,
-    num_linestring_x, num_linestring_y, nullptr, stream, &pi, &li, &d, &n));
-  return point_linestring_distances{column<uint32_t>(pi, n), column<uint32_t>(li, n),
-                                    column<double>(d, n)};
+    num_linestring_x, num_linestring_y, mr, stream, &pi, &li, &d, &n));
+  return point_linestring_distances{column<uint32_t>(pi, n, mr, stream),
+                                    column<uint32_t>(li, n, mr, stream),
+                                    column<double>(d, n, mr, stream)};
 }
 
 }  // namespace cuspatial_amd

@@ -32,6 +32,24 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def use_native(out_dir: str) -> str:
+    """Compile the oracle for this host's CPU (``-O3 -march=native``) into
+    out_dir and load that copy instead of the portable build (the CPU baseline
+    runs on the GPU machine's host, whose CPU is only known there). The
+    arithmetic is the same: floating-point contraction stays off."""
+    global _lib, _LIB_PATH
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "libqpip_oracle_native.so")
+    cmd = [os.environ.get("CXX", "g++"), "-O3", "-march=native", "-mtune=native", "-fopenmp",
+           "-ffp-contract=off", "-fPIC", "-std=c++17", "-shared",
+           os.path.join(_HERE, "qpip_oracle.cpp"), "-o", out + ".tmp"]
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    _LIB_PATH = out
+    _lib = None
+    return out
+
+
 def lib():
     global _lib
     if _lib is None:

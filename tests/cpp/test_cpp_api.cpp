@@ -6,7 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstring>
+#include <type_traits>
 #include <fstream>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -49,6 +52,38 @@ void write_col(char const* name, cuspatial_amd::column<T> const& c)
   std::ofstream f(g_dir + "/out_" + name, std::ios::binary);
   f.write(reinterpret_cast<char const*>(h.data()), static_cast<std::streamsize>(h.size() * sizeof(T)));
 }
+
+// A caller memory resource that counts what it hands out (the role of an
+// rmm::device_async_resource_ref in the reference's API).
+struct CountingResource {
+  std::map<void*, size_t> live;
+  size_t allocations   = 0;
+  size_t deallocations = 0;
+  size_t bad_frees     = 0;
+  static void* allocate(void* ctx, size_t bytes, void*)
+  {
+    auto* r  = static_cast<CountingResource*>(ctx);
+    void* p  = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    r->live[p] = bytes;
+    ++r->allocations;
+    return p;
+  }
+  static void deallocate(void* ctx, void* p, size_t bytes, void* stream)
+  {
+    auto* r = static_cast<CountingResource*>(ctx);
+    auto it = r->live.find(p);
+    if (it == r->live.end() || it->second != bytes) {
+      ++r->bad_frees;
+      return;
+    }
+    r->live.erase(it);
+    ++r->deallocations;
+    (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    (void)hipFree(p);
+  }
+  bool owns(void const* p) const { return live.count(const_cast<void*>(p)) != 0; }
+};
 
 }  // namespace
 
@@ -99,6 +134,69 @@ int main(int argc, char** argv)
       lpairs, tree, point_indices, dx, n, dy, n, reinterpret_cast<uint32_t const*>(dlo),
       static_cast<int64_t>(lo.size()), dlx, static_cast<int64_t>(lx.size()), dly,
       static_cast<int64_t>(ly.size()));
+
+    // the same pipeline with every output from a caller resource
+    CountingResource counter;
+    qpip_resource const res{&CountingResource::allocate, &CountingResource::deallocate, &counter};
+    hipStream_t stream = nullptr;
+    if (hipStreamCreate(&stream) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
+    {
+      auto [r_pi, r_tree] = cs::quadtree_on_points(dx, n, dy, n, prm[0], prm[1], prm[2], prm[3],
+                                                   prm[4], d, static_cast<int32_t>(prm[6]),
+                                                   stream, &res);
+      auto r_boxes = cs::polygon_bounding_boxes(dpart, static_cast<int64_t>(part.size()), dring,
+                                                static_cast<int64_t>(ring.size()), dpx,
+                                                static_cast<int64_t>(px.size()), dpy,
+                                                static_cast<int64_t>(py.size()), 0.0, stream, &res);
+      auto r_pairs = cs::join_quadtree_and_bounding_boxes(r_tree, r_boxes, prm[0], prm[1], prm[2],
+                                                          prm[3], prm[4], d, stream, &res);
+      auto r_hits  = cs::quadtree_point_in_polygon(
+        r_pairs, r_tree, r_pi, dx, n, dy, n, dpart, static_cast<int64_t>(part.size()), dring,
+        static_cast<int64_t>(ring.size()), dpx, static_cast<int64_t>(px.size()), dpy,
+        static_cast<int64_t>(py.size()), stream, &res);
+      auto r_lboxes = cs::linestring_bounding_boxes(dlo, static_cast<int64_t>(lo.size()), dlx,
+                                                    static_cast<int64_t>(lx.size()), dly,
+                                                    static_cast<int64_t>(ly.size()), prm[7],
+                                                    stream, &res);
+      auto r_lpairs = cs::join_quadtree_and_bounding_boxes(r_tree, r_lboxes, prm[0], prm[1],
+                                                           prm[2], prm[3], prm[4], d, stream, &res);
+      auto r_near = cs::quadtree_point_to_nearest_linestring(
+        r_lpairs, r_tree, r_pi, dx, n, dy, n, reinterpret_cast<uint32_t const*>(dlo),
+        static_cast<int64_t>(lo.size()), dlx, static_cast<int64_t>(lx.size()), dly,
+        static_cast<int64_t>(ly.size()), stream, &res);
+      if (hipStreamSynchronize(stream) != hipSuccess) throw std::runtime_error("stream failed");
+      std::vector<void const*> outs = {
+        r_pi.data(), r_tree.key.data(), r_tree.level.data(), r_tree.is_internal_node.data(),
+        r_tree.length.data(), r_tree.offset.data(), r_boxes.minx.data(), r_boxes.miny.data(),
+        r_boxes.maxx.data(), r_boxes.maxy.data(), r_pairs.bbox_offset.data(),
+        r_pairs.quad_offset.data(), r_hits.polygon_index.data(), r_hits.point_index.data(),
+        r_lboxes.minx.data(), r_lpairs.bbox_offset.data(), r_near.point_index.data(),
+        r_near.linestring_index.data(), r_near.distance.data()};
+      size_t from_resource = 0;
+      for (void const* p : outs) from_resource += counter.owns(p) ? 1 : 0;
+      std::printf("resource outputs: %zu of %zu columns from the caller resource, %zu live\n",
+                  from_resource, outs.size(), counter.live.size());
+      // same columns as the pool-backed run
+      auto same = [](auto const& a, auto const& b) {
+        using T = std::remove_pointer_t<decltype(a.data())>;
+        if (a.size() != b.size()) return false;
+        std::vector<T> ha(static_cast<size_t>(a.size())), hb(ha.size());
+        if (a.size() == 0) return true;
+        if (hipMemcpy(ha.data(), a.data(), ha.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(hb.data(), b.data(), hb.size() * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+          return false;
+        return std::memcmp(ha.data(), hb.data(), ha.size() * sizeof(T)) == 0;
+      };
+      bool const equal = same(r_pi, point_indices) && same(r_tree.key, tree.key) &&
+                         same(r_tree.offset, tree.offset) && same(r_pairs.quad_offset, pairs.quad_offset) &&
+                         same(r_hits.polygon_index, hits.polygon_index) &&
+                         same(r_hits.point_index, hits.point_index) &&
+                         same(r_near.distance, nearest.distance);
+      std::printf("resource outputs equal the pool outputs: %s\n", equal ? "yes" : "no");
+    }
+    std::printf("resource after release: %zu allocations, %zu deallocations, %zu live, %zu bad frees\n",
+                counter.allocations, counter.deallocations, counter.live.size(), counter.bad_frees);
+    (void)hipStreamDestroy(stream);
 
     write_col("point_indices", point_indices);
     write_col("key", tree.key);

@@ -33,7 +33,7 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 31
+    assert len(names) == 34
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)

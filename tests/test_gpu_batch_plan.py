@@ -145,3 +145,44 @@ def test_plan_capacity_exceeded_is_reported(cs):
     plan2.launch()
     assert_same(plan2.result(), ref)
     plan2.close()
+
+
+def test_plan_under_each_saved_tuning_matches_eager(cs, tmp_path):
+    """A plan looks its shape bucket up in a tuning table and is captured under
+    those values; every configuration of the table (and of the package's own
+    table, if it has any) gives the eager call's results exactly, and the
+    process-wide values are restored after the plan is made."""
+    from cuspatial_amd import autotune, datasets
+
+    w = datasets.batched_jobs(seed=23, num_jobs=40, min_points=100, max_points=40000,
+                              min_polygons=1, max_polygons=150, max_vertices=64, ext=1e4)
+    xy, P = device_inputs(cs, w)
+    want = eager(cs, w, xy, P)
+    configs = [
+        {"pip_min_pairs_per_wave": 1, "pip_pairs_shift": 10, "pip_waves_per_block": 8,
+         "pip_stage_edges": 128},
+        {"pip_min_pairs_per_wave": 64, "pip_pairs_shift": 16, "pip_waves_per_block": 4,
+         "pip_stage_edges": 64},
+    ]
+    if __import__("os").path.exists(autotune.DEFAULT_TABLE):
+        configs += [dict(b["config"]) for b in autotune.TuningTable(autotune.DEFAULT_TABLE)
+                    .buckets.values()]
+    before = autotune.current()
+    for k, cfg in enumerate(configs):
+        table = autotune.TuningTable(None)
+        probe = make_plan(cs, w, xy, P)
+        bucket = probe.bucket
+        probe.close()
+        table.record(bucket, {"config": cfg, "baseline_ms": 0.0, "best_ms": 0.0}, "test")
+        path = str(tmp_path / f"t{k}.json")
+        table.save(path)
+        print(f"[plan] tuning {cfg} for bucket {bucket}", flush=True)
+        plan = cs.BatchPlan(cs.Points.from_points_xy(xy), w.job_point_offsets, P,
+                            w.job_polygon_offsets, w.bboxes, w.scales, w.max_depths,
+                            w.max_sizes, tuning=path)
+        assert plan.tuning == cfg
+        assert autotune.current() == before
+        plan.launch()
+        plan.launch()
+        assert_same(plan.result(), want)
+        plan.close()

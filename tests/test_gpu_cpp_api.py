@@ -3,11 +3,15 @@
 tests/cpp/test_cpp_api.cpp runs quadtree_on_points -> polygon_bounding_boxes ->
 join_quadtree_and_bounding_boxes -> quadtree_point_in_polygon and
 linestring_bounding_boxes -> join -> quadtree_point_to_nearest_linestring on the
-GPU through the header, then a call with mismatched x/y that must throw
-cuspatial_amd::logic_error. Bar: every output column bit-identical to the oracle.
+GPU through the header, the same pipeline with a caller memory resource (the
+reference's `mr` argument) on a created stream, then a call with mismatched x/y
+that must throw cuspatial_amd::logic_error. Bar: every output column
+bit-identical to the oracle; with the resource, every output column allocated
+from it, equal to the pool-backed one, and released through it.
 """
 
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -51,6 +55,15 @@ def test_cpp_program_matches_oracle(tmp_path, oracle_mod, seed, n, npoly, d, ms)
     # the reference's message format: "cuSpatial failure at: <file>:<line>: <reason>"
     assert "logic_error: cuSpatial failure at: " in r.stdout, r.stdout
     assert "x and y columns must have the same length" in r.stdout, r.stdout
+    # with a caller resource (the reference's mr argument) every output column
+    # comes from it, equals the pool-backed column, and goes back through it
+    assert "resource outputs: 19 of 19 columns from the caller resource" in r.stdout, r.stdout
+    assert "resource outputs equal the pool outputs: yes" in r.stdout, r.stdout
+    m = re.search(r"resource after release: (\d+) allocations, (\d+) deallocations, "
+                  r"(\d+) live, (\d+) bad frees", r.stdout)
+    assert m, r.stdout
+    allocs, deallocs, live, bad = map(int, m.groups())
+    assert allocs >= 19 and allocs == deallocs and live == 0 and bad == 0, r.stdout
 
     opi, okey, olev, oint, olen, ooff = oracle_mod.quadtree_on_points(
         x, y, 0.0, ext, 0.0, ext, scale, d, ms)

@@ -253,3 +253,76 @@ def test_fused_ranking_matches_gemm_logits(env):
     sel = clear & (per_point >= 0) & (cnt > 1) & (cnt <= 16)
     agree = (first_right[sel] == gemm_right[sel]).to(torch.float64).mean().item()
     assert sel.sum().item() > 50000 and agree > 0.999, agree
+
+
+def test_locator_rejects_overlapping_polygons(env):
+    """The locator answers with the first confirmed candidate, which is the plain
+    join's answer only for non-overlapping polygons: overlapping star polygons
+    (and a polygon nested in another) are refused when the locator is built,
+    while the Voronoi cells with holes filled by islands are accepted."""
+    from conftest import star_polygons
+
+    torch, cs, ranker, rk, datasets = env
+    rng = np.random.default_rng(41)
+    part, ring, vx, vy = star_polygons(rng, 300, 1e5, 2000.0, 6000.0)
+    P = cs.Polygons(part, ring, vx, vy, device="cuda")
+    grid = ranker.CandidateGrid.build(P, 0.0, 0.0, 1e5, 32)
+    oi, oj = ranker.overlapping_polygons(P, grid)
+    assert oi.numel() > 0
+    with pytest.raises(ValueError, match="non-overlapping"):
+        ranker.LearnedLocator(P, grid, ranker.Ranker(grid.num_cells), (0.0, 1e5, 0.0, 1e5),
+                              1e5 / 1024, 10, 64)
+    # a square nested inside another, with no edge crossings
+    sq = lambda x0, y0, s: ([x0, x0 + s, x0 + s, x0, x0], [y0, y0, y0 + s, y0 + s, y0])  # noqa
+    ax, ay = sq(1000.0, 1000.0, 5000.0)
+    bx, by = sq(2000.0, 2000.0, 500.0)
+    Pn = cs.Polygons(np.array([0, 1, 2], np.int32), np.array([0, 5, 10], np.int32),
+                     np.array(ax + bx), np.array(ay + by), device="cuda")
+    gn = ranker.CandidateGrid.build(Pn, 0.0, 0.0, 1e5, 16)
+    oi, oj = ranker.overlapping_polygons(Pn, gn)
+    assert oi.tolist() == [0] and oj.tolist() == [1]
+    # a partition passes
+    _, Pv, gv, _, _, _ = voronoi(env, 10, 500, 42, 32)
+    assert ranker.overlapping_polygons(Pv, gv)[0].numel() == 0
+
+
+def test_locator_restart_from_file(env, tmp_path):
+    """A trained locator saved to one index file (polygons, grid, ranker weights,
+    fallback parameters) and loaded in a fresh process answers every point
+    exactly as before: identical pairs, tests per point and miss flags."""
+    import subprocess
+    import sys
+
+    torch, cs, ranker, rk, _ = env
+    _, P, grid, xy, x, y = voronoi(env, 100000, 400, 43, 32)
+    want_poly, want_pt = exact_pairs(env, P, xy)
+    labels = grid.slots_of(xy, ranker.per_point_polygon(want_poly, want_pt, len(x)))
+    model = ranker.Ranker(grid.num_cells, seed=44)
+    model.train(grid, xy, labels, epochs=3, batch=8192, lr=5e-3)
+    loc = ranker.LearnedLocator(P, grid, model, (0.0, 1e5, 0.0, 1e5), 1e5 / 1024, 10, 64)
+    res = loc.locate(cs.Points.from_points_xy(xy))
+    poly, pt = res.pairs()
+    assert torch.equal(poly, want_poly) and torch.equal(pt, want_pt)
+    path = tmp_path / "locator.arrow"
+    loc.save(path)
+    np.save(tmp_path / "xy.npy", xy.cpu().numpy())
+    code = (
+        "import sys, numpy as np, torch\n"
+        f"sys.path.insert(0, {repr(str(__import__('os').path.dirname(__import__('os').path.dirname(__file__))))})\n"
+        "import cuspatial_amd as cs\n"
+        "from cuspatial_amd import ranker\n"
+        f"loc = ranker.LearnedLocator.load({str(path)!r})\n"
+        f"xy = torch.as_tensor(np.load({str(tmp_path / 'xy.npy')!r}), device='cuda')\n"
+        "r = loc.locate(cs.Points.from_points_xy(xy))\n"
+        "poly, pt = r.pairs()\n"
+        f"np.savez({str(tmp_path / 'out.npz')!r}, poly=poly.cpu().numpy(), pt=pt.cpu().numpy(), "
+        "tests=r.tests.cpu().numpy(), flags=r.flags.cpu().numpy())\n"
+        "print('restart ok', int(poly.numel()))\n")
+    print("fresh process: load the saved locator and locate 100000 points", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.load(tmp_path / "out.npz")
+    assert np.array_equal(got["poly"], poly.cpu().numpy())
+    assert np.array_equal(got["pt"], pt.cpu().numpy())
+    assert np.array_equal(got["tests"], res.tests.cpu().numpy())
+    assert np.array_equal(got["flags"], res.flags.cpu().numpy())

@@ -137,3 +137,33 @@ def test_concurrent_calls_match_serial(env):
     for k in range(4):
         for it in range(3):
             assert torch.equal(got[k][it][0], want[k][0]) and torch.equal(got[k][it][1], want[k][1])
+
+
+def test_captured_batches_equal_eager_batches(env):
+    """The plain index answers a batch by replaying a captured plan of the batch's
+    size bucket (NaN points fill the bucket): the pairs equal the eager path's
+    for batch sizes across bucket edges, and a batch that outgrows the plan's
+    capacities (many more hits than the batch the plan was sized on) rebuilds the
+    plan and still answers exactly."""
+    torch, cs, ranker, serving, _ = env
+    captured, (part, ring, vx, vy) = star_index(env)
+    P = captured.polygons
+    eager = serving.ResidentIndex(P, captured.bbox, captured.scale, captured.max_depth,
+                                  captured.max_size, captured=False)
+    rng = np.random.default_rng(47)
+    for n in (1, 100, 4095, 4096, 4097, 20000, 70000):
+        xy = torch.as_tensor(rng.uniform(0, 1000.0, 2 * n), device="cuda")
+        a, b = captured.query(xy), eager.query(xy)
+        assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]), n
+    assert captured.plan_stats["replays"] == 7 and eager.plan_stats["eager"] == 7
+    # same bucket, sized on uniform points, then every point inside one polygon
+    xy = torch.as_tensor(rng.uniform(0, 1000.0, 2 * 5000), device="cuda")
+    captured.query(xy)
+    i = int(np.argmax(np.diff(ring)))  # a ring with many vertices
+    cx, cy = float(np.mean(vx[ring[i]:ring[i + 1] - 1])), float(np.mean(vy[ring[i]:ring[i + 1] - 1]))
+    dense = np.stack([cx + rng.uniform(-0.5, 0.5, 5000), cy + rng.uniform(-0.5, 0.5, 5000)], 1)
+    xy = torch.as_tensor(dense.reshape(-1), device="cuda")
+    a, b = captured.query(xy), eager.query(xy)
+    assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    assert b[0].numel() > 0
+    captured.close()
