@@ -68,6 +68,7 @@ EXPORTED = (
     "qpip_quadtree_point_in_polygon",
     "qpip_quadtree_point_in_polygon_xy",
     "qpip_point_in_polygon",
+    "qpip_point_in_polygon_noalloc",
     "qpip_pairwise_point_in_polygon",
     "qpip_batched_quadtree_point_in_polygon",
     "qpip_batch_plan_create",
@@ -138,6 +139,8 @@ def lib():
             getattr(L, name).argtypes = [
                 _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64,
                 P(qpip_resource), _vp, P(_vp), P(_i64)]
+        L.qpip_point_in_polygon_noalloc.argtypes = [
+            _vp, _vp, _i64, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64, _i64, _vp, _vp]
         L.qpip_batched_quadtree_point_in_polygon.argtypes = [
             _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _i64,
             _i64, P(qpip_resource), _vp, P(_vp), P(_vp), P(_vp), _vp, P(_i64)]
@@ -173,6 +176,7 @@ def lib():
             "qpip_batched_quadtree_point_in_polygon", "qpip_batch_plan_create",
             "qpip_batch_plan_launch", "qpip_batch_plan_result", "qpip_batch_plan_destroy",
             "qpip_point_in_polygon", "qpip_pairwise_point_in_polygon",
+            "qpip_point_in_polygon_noalloc",
             "qpip_quadtree_on_points", "qpip_quadtree_on_points_xy", "qpip_polygon_bounding_boxes",
             "qpip_join_quadtree_and_bounding_boxes", "qpip_quadtree_point_in_polygon",
             "qpip_quadtree_point_in_polygon_xy",

@@ -32,6 +32,7 @@
 //   No candidate-sized buffer is ever allocated, so the reference's out-of-memory
 //   fallback (count, then allocate; :220-234) is the only path here.
 #include <algorithm>
+#include <mutex>
 
 #include "qpip_pip.hpp"
 #include "qpip_scan.hpp"
@@ -945,4 +946,112 @@ extern "C" int qpip_pairwise_point_in_polygon(const double* x,
                                 ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
                                 num_poly_y, mr, stream, true, reinterpret_cast<void**>(result),
                                 num_results);
+}
+
+// --- allocation-free bitmask point-in-polygon ---------------------------------------
+// qpip_point_in_polygon_noalloc runs the kernels of qpip_point_in_polygon with the
+// edge table in `scratch`, a fixed device array of this file, instead of pool
+// memory: edges (32 B per vertex), their cold halves (16 B per vertex) and the
+// edge count of each ring (4 B per ring). 12.25 MiB per device, held by every
+// device the library runs on until the process ends.
+namespace {
+
+constexpr size_t kNoAllocEdgesOff  = 0;
+constexpr size_t kNoAllocEdgesBOff = kNoAllocEdgesOff + sizeof(qpip::Edge) * QPIP_NOALLOC_MAX_VERTICES;
+constexpr size_t kNoAllocRingsOff  = kNoAllocEdgesBOff + sizeof(qpip::EdgeB) * QPIP_NOALLOC_MAX_VERTICES;
+constexpr size_t kNoAllocBytes     = kNoAllocRingsOff + sizeof(uint32_t) * QPIP_NOALLOC_MAX_RINGS;
+static_assert(kNoAllocEdgesBOff % alignof(qpip::EdgeB) == 0 && kNoAllocRingsOff % 4 == 0,
+              "scratch carving keeps each table aligned");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+// Device address of `scratch` on each device, looked up once per device (with that
+// device current) on the first call there. The host-side name `scratch` is not a
+// device address and is never passed to a kernel or a copy.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+unsigned char* scratch_on_current_device()
+{
+  int dev = 0;
+  QPIP_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) throw qpip::device_error("device ordinal out of range");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    QPIP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw qpip::device_error("scratch symbol has no device address");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+
+}  // namespace
+
+extern "C" int qpip_point_in_polygon_noalloc(const double* x,
+                                             const double* y,
+                                             int64_t num_x,
+                                             int64_t num_y,
+                                             const int32_t* poly_offsets,
+                                             int64_t num_poly_offsets,
+                                             const int32_t* ring_offsets,
+                                             int64_t num_ring_offsets,
+                                             const double* poly_x,
+                                             const double* poly_y,
+                                             int64_t num_poly_x,
+                                             int64_t num_poly_y,
+                                             void* stream_,
+                                             int32_t* result)
+{
+  using namespace qpip;
+  try {
+    hipStream_t stream = static_cast<hipStream_t>(stream_);
+    // the checks of qpip_point_in_polygon, in its order
+    QPIP_EXPECTS(num_x == num_y && num_poly_x == num_poly_y,
+                 "All points must have both x and y values");
+    int64_t const num_polys = num_poly_offsets > 0 ? num_poly_offsets - 1 : 0;
+    QPIP_EXPECTS(num_polys <= 31, "Number of polygons cannot exceed 31");
+    if (num_x <= 0) return QPIP_SUCCESS;
+    QPIP_EXPECTS(num_x < (int64_t{1} << 32), "inputs must have fewer than 2^32 rows");
+    QPIP_EXPECTS(result != nullptr, "result must not be null");
+    uint64_t const num_rings = num_ring_offsets > 0 ? static_cast<uint64_t>(num_ring_offsets - 1) : 0;
+    if (num_polys > 0) QPIP_EXPECTS(poly_offsets && ring_offsets, "offsets must not be null");
+    if (num_poly_x > QPIP_NOALLOC_MAX_VERTICES || num_rings > QPIP_NOALLOC_MAX_RINGS)
+      throw capacity_error(
+        "qpip_point_in_polygon_noalloc: at most " QPIP_STR(QPIP_NOALLOC_MAX_VERTICES)
+        " polygon vertices and " QPIP_STR(QPIP_NOALLOC_MAX_RINGS)
+        " rings fit its fixed scratch; use qpip_point_in_polygon");
+    unsigned char* const base = scratch_on_current_device();
+    Edge* const edges         = reinterpret_cast<Edge*>(base + kNoAllocEdgesOff);
+    EdgeB* const edges_b      = reinterpret_cast<EdgeB*>(base + kNoAllocEdgesBOff);
+    uint32_t* const ring_edges = reinterpret_cast<uint32_t*>(base + kNoAllocRingsOff);
+    // Every ring's entry and every edge a ring keeps are written below before the
+    // test reads them; the test reads nothing else of the scratch, so nothing a
+    // previous call left there needs clearing.
+    if (num_rings > 0) {
+      build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
+        reinterpret_cast<uint32_t const*>(ring_offsets), static_cast<int64_t>(num_rings), poly_x,
+        poly_y, num_poly_x, edges, edges_b, ring_edges);
+      QPIP_LAUNCH_CHECK();
+    }
+    uint64_t const n = static_cast<uint64_t>(num_x);
+    PipArgs a{};
+    a.num_points   = n;
+    a.x            = x;
+    a.y            = y;
+    a.poly_offsets = reinterpret_cast<uint32_t const*>(poly_offsets);
+    a.num_polys    = static_cast<uint64_t>(num_polys);
+    a.num_rings    = num_rings;
+    a.ring_offsets = reinterpret_cast<uint32_t const*>(ring_offsets);
+    a.ring_edges   = ring_edges;
+    a.edges        = edges;
+    a.edges_b      = edges_b;
+    uint64_t const waves = (n + 4 * kWave - 1) / (4 * kWave);
+    pip_bitmask_kernel<<<static_cast<unsigned>((waves + 3) / 4), 256, 0, stream>>>(a, n, result);
+    QPIP_LAUNCH_CHECK();
+    return QPIP_SUCCESS;
+  } catch (...) {
+    return qpip::error_status();
+  }
 }

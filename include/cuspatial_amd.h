@@ -339,6 +339,37 @@ QPIP_API int qpip_point_in_polygon(const double* x,
                                    int64_t* num_results);
 
 /*
+ * point_in_polygon without allocation. No counterpart in the reference: a variant of
+ * qpip_point_in_polygon for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops that must not call hipMalloc, harnesses that
+ * forbid it). Same checks, test and output as qpip_point_in_polygon, but the caller
+ * passes the output, `result` (num_x INT32 in device memory), and the edge table
+ * lives in a fixed device array of the library (12.25 MiB per device, kept until
+ * the process ends). The call allocates nothing, queues its two kernels on `stream`
+ * and returns without waiting.
+ * Limits: at most QPIP_NOALLOC_MAX_VERTICES polygon vertices and
+ * QPIP_NOALLOC_MAX_RINGS rings; larger inputs fail with QPIP_ERROR_CAPACITY.
+ * Every call on one GPU uses that GPU's one scratch array: calls that overlap on
+ * a GPU (other streams, other threads) race and must be serialised by the caller.
+ */
+#define QPIP_NOALLOC_MAX_VERTICES 262144
+#define QPIP_NOALLOC_MAX_RINGS    65536
+QPIP_API int qpip_point_in_polygon_noalloc(const double* x,
+                                           const double* y,
+                                           int64_t num_x,
+                                           int64_t num_y,
+                                           const int32_t* poly_offsets,
+                                           int64_t num_poly_offsets,
+                                           const int32_t* ring_offsets,
+                                           int64_t num_ring_offsets,
+                                           const double* poly_x,
+                                           const double* poly_y,
+                                           int64_t num_poly_x,
+                                           int64_t num_poly_y,
+                                           void* stream,
+                                           int32_t* result);
+
+/*
  * pairwise_point_in_polygon (point i against polygon i)
  * Replaces cuspatial::pairwise_point_in_polygon
  *   (cpp/include/cuspatial/point_in_polygon.hpp:124-131,

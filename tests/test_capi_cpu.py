@@ -33,7 +33,7 @@ def declared_symbols():
 def test_exports_every_declared_symbol(capi):
     L = capi.lib()
     names = declared_symbols()
-    assert len(names) == 34
+    assert len(names) == 35
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(capi.EXPORTED)
@@ -223,6 +223,29 @@ def test_brute_force_pip_preconditions(capi, pairwise):
 def test_brute_force_pip_empty(capi):
     assert _brute(capi, False) == (capi.QPIP_SUCCESS, 0, None)
     assert _brute(capi, True, npo=1) == (capi.QPIP_SUCCESS, 0, None)
+
+
+def test_noalloc_pip_preconditions_and_limits(capi):
+    """qpip_point_in_polygon_noalloc: the bitmask entry point's checks, then its
+    scratch limits (262,144 vertices, 65,536 rings), all decided before any device
+    work (the pointers below are never read)."""
+    L = capi.lib()
+    fake = ctypes.c_void_p(0x1000)
+
+    def call(nx=4, ny=4, npo=2, nro=2, nvx=8, nvy=8, out=fake):
+        return L.qpip_point_in_polygon_noalloc(fake, fake, nx, ny, fake, npo, fake, nro, fake,
+                                               fake, nvx, nvy, None, out)
+
+    msg = lambda: capi.lib().qpip_last_error_message().decode()  # noqa: E731
+    assert call(nx=3) == capi.QPIP_ERROR_LOGIC
+    assert msg().endswith("All points must have both x and y values")
+    assert call(npo=33) == capi.QPIP_ERROR_LOGIC
+    assert msg().endswith("Number of polygons cannot exceed 31")
+    assert call(nx=0, ny=0) == capi.QPIP_SUCCESS
+    assert call(out=None) == capi.QPIP_ERROR_LOGIC
+    assert call(nvx=262145, nvy=262145) == capi.QPIP_ERROR_CAPACITY
+    assert "262144" in msg() and "65536" in msg()
+    assert call(nro=65538) == capi.QPIP_ERROR_CAPACITY
 
 
 def test_batched_preconditions(capi):
