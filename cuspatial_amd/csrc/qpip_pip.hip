@@ -113,28 +113,62 @@ inline uint32_t pairs_per_wave(uint64_t num_pairs)
 // at once. Masks are stored per (pair, chunk) in pair order: chunk j of pair k is
 // chunk_start[k] + j.
 //
-// Collinearity filter. If float_equal(rhs, lhs) (4 ULP), then for normal values
-// |rhs - lhs| is exact (Sterbenz) and below 2^-50 * max(|rhs|, |lhs|) <= 2^-48 *
-// |rhs|, and near zero it is below 8 * 2^-1074. So fma(|rhs|, -2^-48, |rhs - lhs|)
-// <= 2^-1020 holds for every float_equal pair (NaN compares as "maybe"); lanes
-// where some edge passes are re-examined exactly in a second, rare pass.
+// Certain-sign filter. The reference decides each (point, edge) by the sign of
+// lhs - rhs, lhs = (px - ax) * rise and rhs = run * (py - ay), rounded as written,
+// and calls the point on the edge when the two are within 4 ULP. Here each edge
+// is evaluated in the line form t = px * rise - py * run + c, c = ay * run -
+// ax * rise (c per edge, two fmas per point), and t decides only when it is far
+// from zero. With u = 2^-53, s >= |px|, |py|, |ax|, |ay| for the wave's points and
+// the batch's edges, and n = |run| + |rise|, every term is at most M = s * n:
+//   |t - (L - R)| <= 7.1 u M  (c, the inner and the outer fma; L - R exact),
+//   |lhs - L| + |rhs - R| <= 2.01 u M  (two roundings each),
+//   4 ULP of lhs or rhs <= 8.01 u M, plus 2^-1070 where values are subnormal.
+// So |t| > tol = 2^-48 * s * n + 2^-1000 (over 31.9 u M after rounding) gives
+// sign(lhs - rhs) = sign(t) and rules out the 4-ULP collinearity; every other
+// (point, edge), NaN and infinity included (their comparisons are false), marks the
+// point "suspect". Suspect points of the polygon are retested with the reference's
+// arithmetic in a second, rare pass. s and n are capped at 2^400 (beyond, tol is
+// infinite and every point is suspect), so no product overflows.
 //
 // Lane state is kept as wave-uniform 64-bit masks (one bit per lane and point
 // slot), so the loop's bookkeeping is a handful of scalar ops per edge.
 // Ring staging: a wave copies up to kStageEdges edges of the ring into its own LDS
 // slot with one coalesced vector load per lane (one memory latency per batch),
-// then walks them with broadcast LDS reads. Walking them from global memory took
-// a dependent scalar load (and its latency) every two edges, which dominated
-// when the edge table is not in L2: a small join right after the table is built,
-// or the batched join's 6.8 GB table.
+// computes each edge's c and tol there (one edge per lane), then walks them with
+// broadcast LDS reads. Walking them from global memory took a dependent scalar
+// load (and its latency) every two edges, which dominated when the edge table is
+// not in L2: a small join right after the table is built, or the batched join's
+// 6.8 GB table.
 // The depth S (edges per batch, 64 or 128) and the waves per workgroup are runtime
 // tuning parameters (pip_stage_edges, pip_waves_per_block).
 constexpr int kStageEdges = kWave;  // the default depth, and the bitmask kernel's
 
+// An edge as the test's fast loop reads it from LDS.
+struct alignas(32) LineEdge {
+  double ay, run, rise, c;
+};
+template <int S>
+struct PipStage {
+  LineEdge e[S];
+  double tol[S];
+};
+
+// Largest |coordinate| of the wave's points (0 for invalid slots; NaN ignored).
+__device__ __forceinline__ double wave_max_abs(double const (&px)[4], double const (&py)[4])
+{
+  double m = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u) m = fmax(m, fmax(fabs(px[u]), fabs(py[u])));
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
+  return m;
+}
+
 template <int U, int S = kStageEdges>
 __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, double const (&px)[4],
                                              double const (&py)[4], uint64_t const (&valid_m)[4],
-                                             uint64_t (&hit_m)[4], bool& err, Edge* stage)
+                                             double pmax, uint64_t (&hit_m)[4], bool& err,
+                                             PipStage<S>* stage)
 {
   uint64_t inside_m[U], suspect_m[U];
 #pragma unroll
@@ -147,6 +181,7 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
   } else {
     err = true;
   }
+  int const lane = lane_id();
   for (uint32_t r = r0; r < r1; ++r) {
     uint32_t const e0 = uniform(a.ring_offsets[r]);
     uint32_t const ne = uniform(a.ring_edges[r]);
@@ -158,20 +193,17 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
     uint64_t y0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) y0[u] = ballot(by0 > py[u]);
-    auto edge = [&](Edge const& ed) {
+    auto edge = [&](LineEdge const& ed, double tol) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
-        double const lhs  = (px[u] - ed.ax) * ed.rise;  // run_to_point * rise
-        double const rhs  = ed.run * (py[u] - ed.ay);   // run * rise_to_point
-        double const gap  = fma(fabs(rhs), -0x1p-48, fabs(rhs - lhs));
+        double const t    = fma(px[u], ed.rise, fma(-py[u], ed.run, ed.c));
         uint64_t const y1 = ballot(ed.ay > py[u]);
-        uint64_t const lt = ballot(lhs < rhs);
-        suspect_m[u] |= ballot(!(gap > 0x1p-1020));
+        uint64_t const lt = ballot(t < 0.0);  // lhs < rhs
+        suspect_m[u] |= ballot(!(fabs(t) > tol));
         inside_m[u] ^= (y1 ^ y0[u]) & (lt ^ y1);
         y0[u] = y1;
       }
     };
-    int const lane = lane_id();
     for (uint32_t base = 0; base < ne; base += S) {
       uint32_t const n = ne - base < S ? ne - base : S;
       // the previous batch has been read by every lane before it is overwritten,
@@ -179,47 +211,75 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double amax = 0.0, nrm[S / kWave];
+#pragma unroll
+      for (int i = 0; i < S; i += kWave) {
+        nrm[i / kWave] = 0.0;
+        if (static_cast<uint32_t>(i + lane) < n) {
+          Edge const ed = E[base + i + lane];
+          double const ax_rise = ed.ax * ed.rise;
+          stage->e[i + lane]   = LineEdge{ed.ay, ed.run, ed.rise, fma(ed.ay, ed.run, -ax_rise)};
+          amax                 = fmax(amax, fmax(fabs(ed.ax), fabs(ed.ay)));
+          nrm[i / kWave]       = fabs(ed.run) + fabs(ed.rise);
+        }
+      }
+#pragma unroll
+      for (int o = kWave / 2; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o, kWave));
+      double const s = pmax + amax;
 #pragma unroll
       for (int i = 0; i < S; i += kWave)
-        if (static_cast<uint32_t>(i + lane) < n) stage[i + lane] = E[base + i + lane];
+        if (static_cast<uint32_t>(i + lane) < n)
+          stage->tol[i + lane] = s < 0x1p400 && nrm[i / kWave] < 0x1p400
+                                   ? fma(s * nrm[i / kWave], 0x1p-48, 0x1p-1000)
+                                   : __builtin_inf();
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       uint32_t e = 0;
       if (U <= 2) {  // unrolled so that the loop's scalar bookkeeping is amortised
         for (; e + 2 <= n; e += 2) {
-          edge(stage[e]);
-          edge(stage[e + 1]);
+          edge(stage->e[e], stage->tol[e]);
+          edge(stage->e[e + 1], stage->tol[e + 1]);
         }
       }
-      for (; e < n; ++e) edge(stage[e]);
+      for (; e < n; ++e) edge(stage->e[e], stage->tol[e]);
     }
   }
-  int const lane = lane_id();
+  // Suspect points: the reference's test over the whole polygon
+  // (is_point_in_polygon.cuh:46-100 on the effective edges).
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     uint64_t const sus = suspect_m[u] & valid_m[u];
-    uint64_t on_m      = 0;
+    uint64_t exact_m   = 0;
     if (sus) {
       bool const suspect = (sus >> lane) & 1ull;
-      bool on_edge       = false;
+      bool inside = false, on_edge = false;
       for (uint32_t r = r0; r < r1; ++r) {
         uint32_t const e0 = uniform(a.ring_offsets[r]);
         uint32_t const ne = uniform(a.ring_edges[r]);
+        if (ne == 0) continue;
+        bool y0 = a.edges_b[e0].by > py[u];
         for (uint32_t e = 0; e < ne; ++e) {
-          Edge const E    = a.edges[e0 + e];
-          double const bx = a.edges_b[e0 + e].bx;
-          // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
-          if (suspect && float_equal(E.run * (py[u] - E.ay), (px[u] - E.ax) * E.rise)) {
-            double const minx = E.ax < bx ? E.ax : bx;
-            double const maxx = E.ax < bx ? bx : E.ax;
-            if (minx <= px[u] && px[u] <= maxx) on_edge = true;
+          Edge const E = a.edges[e0 + e];
+          if (suspect) {
+            double const lhs = (px[u] - E.ax) * E.rise;  // run_to_point * rise
+            double const rhs = E.run * (py[u] - E.ay);   // run * rise_to_point
+            if (float_equal(rhs, lhs)) {
+              // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
+              double const bx   = a.edges_b[e0 + e].bx;
+              double const minx = E.ax < bx ? E.ax : bx;
+              double const maxx = E.ax < bx ? bx : E.ax;
+              if (minx <= px[u] && px[u] <= maxx) on_edge = true;
+            }
+            bool const y1 = E.ay > py[u];
+            if (y1 != y0 && ((lhs < rhs) != y1)) inside = !inside;
+            y0 = y1;
           }
         }
       }
-      on_m = ballot(on_edge);
+      exact_m = ballot(suspect && inside && !on_edge);
     }
-    hit_m[u] = inside_m[u] & ~on_m & valid_m[u];
+    hit_m[u] = ((inside_m[u] & ~sus) | exact_m) & valid_m[u];
   }
 }
 
@@ -233,8 +293,8 @@ __global__ void __launch_bounds__(W * kWave)
                   uint32_t* __restrict__ hit_counts, uint2* __restrict__ chunk_info,
                   uint32_t* __restrict__ bad)
 {
-  __shared__ Edge s_stage[W][S];
-  Edge* const stage = s_stage[threadIdx.x / kWave];
+  __shared__ PipStage<S> s_stage[W];
+  PipStage<S>* const stage = &s_stage[threadIdx.x / kWave];
   uint64_t const kk = base + (blockIdx.x * static_cast<uint64_t>(W) + threadIdx.x / kWave);
   uint64_t const K  = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
   if (kk >= K) return;
@@ -292,15 +352,16 @@ __global__ void __launch_bounds__(W * kWave)
       }
       valid_m[u] = ballot(valid);
     }
+    double const pmax = wave_max_abs(px, py);
     for (uint32_t k = k0; k < K; ++k) {
       if (k > k0 && (k % G == 0 || uniform(a.pair_quad[k]) != quad)) break;
       uint32_t const poly = uniform(a.pair_poly[k]);
       uint64_t hit_m[4]   = {0, 0, 0, 0};
       switch (nu) {
-        case 1: test_polygon<1, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
-        case 2: test_polygon<2, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
-        case 3: test_polygon<3, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
-        default: test_polygon<4, S>(a, poly, px, py, valid_m, hit_m, err, stage); break;
+        case 1: test_polygon<1, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
+        case 2: test_polygon<2, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
+        case 3: test_polygon<3, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
+        default: test_polygon<4, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
       }
       uint64_t const slot = slot0 + static_cast<uint64_t>(k - k0) * chunks + j0;
       if (slot + nu > a.chunk_cap) {
@@ -446,8 +507,8 @@ __device__ __forceinline__ void load_point(PipArgs const& a, uint64_t i, double&
 __global__ void __launch_bounds__(256)
   pip_bitmask_kernel(PipArgs a, uint64_t n, int32_t* __restrict__ out)
 {
-  __shared__ Edge s_stage[256 / kWave][kStageEdges];
-  Edge* const stage = s_stage[threadIdx.x / kWave];
+  __shared__ PipStage<kStageEdges> s_stage[256 / kWave];
+  PipStage<kStageEdges>* const stage = &s_stage[threadIdx.x / kWave];
   uint64_t const w  = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   uint64_t const p0 = w * 4 * kWave;
   if (p0 >= n) return;
@@ -463,15 +524,16 @@ __global__ void __launch_bounds__(256)
     if (valid) load_point(a, i, px[u], py[u]);
     valid_m[u] = ballot(valid);
   }
+  double const pmax = wave_max_abs(px, py);
   uint32_t bits[4] = {0, 0, 0, 0};
   bool err         = false;
   for (uint32_t j = 0; j < a.num_polys; ++j) {
     uint64_t hit_m[4] = {0, 0, 0, 0};
     switch (nu) {
-      case 1: test_polygon<1>(a, j, px, py, valid_m, hit_m, err, stage); break;
-      case 2: test_polygon<2>(a, j, px, py, valid_m, hit_m, err, stage); break;
-      case 3: test_polygon<3>(a, j, px, py, valid_m, hit_m, err, stage); break;
-      default: test_polygon<4>(a, j, px, py, valid_m, hit_m, err, stage); break;
+      case 1: test_polygon<1>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
+      case 2: test_polygon<2>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
+      case 3: test_polygon<3>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
+      default: test_polygon<4>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) bits[u] |= static_cast<uint32_t>((hit_m[u] >> lane) & 1ull) << j;
