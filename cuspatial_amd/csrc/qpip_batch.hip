@@ -1804,15 +1804,19 @@ extern "C" int qpip_batch_plan_result(qpip_batch_plan* plan,
     BatchPlan& P = *plan_of(plan);
     check_device(P);
     QPIP_EXPECTS(P.launched, "the plan has not been launched");
-    QPIP_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream_)));
+    hipStream_t const rs = static_cast<hipStream_t>(stream_);
+    QPIP_HIP(hipStreamSynchronize(rs));
     QPIP_HIP(hipStreamSynchronize(P.last_stream));
     size_t const G = P.groups.size();
     std::vector<DeviceCounts> hc(G);
     std::vector<int64_t> first(static_cast<size_t>(P.num_jobs));
-    QPIP_HIP(hipMemcpy(hc.data(), P.counts.get(), G * sizeof(DeviceCounts),
-                       hipMemcpyDeviceToHost));
-    QPIP_HIP(hipMemcpy(first.data(), P.job_first.get(), first.size() * sizeof(int64_t),
-                       hipMemcpyDeviceToHost));
+    // Copies on the caller's stream, never the legacy stream: another thread may be
+    // capturing a plan, and a legacy-stream copy would have to wait on its stream.
+    QPIP_HIP(hipMemcpyAsync(hc.data(), P.counts.get(), G * sizeof(DeviceCounts),
+                            hipMemcpyDeviceToHost, rs));
+    QPIP_HIP(hipMemcpyAsync(first.data(), P.job_first.get(), first.size() * sizeof(int64_t),
+                            hipMemcpyDeviceToHost, rs));
+    QPIP_HIP(hipStreamSynchronize(rs));
     uint32_t flags = 0;
     for (auto const& c : hc) flags |= c.flags;
     if (flags & 1u) throw logic_error("cuSpatial failure: forest child range or index out of range");
