@@ -337,9 +337,10 @@ def section_c2(args, dev, steps, warmup, cpu_seconds):
 # ---------------------------------------------------------------------------
 def batched_sort_bytes(w):
     """Algorithmic bytes of the batched point sort's scatter passes per step:
-    the library sorts each group of consecutive jobs whose composite keys
-    (job, Morton) fit 32 bits in one stable radix sort of ceil(bits / 8) passes
-    (the grouping restated from qpip_batch.hip's job_groups)."""
+    the library groups consecutive jobs whose composite keys (job, Morton) fit
+    32 bits (the grouping restated from qpip_batch.hip's job_groups) and sorts
+    each job's points on their own in a segmented radix sort of
+    ceil(Morton key bits / 8) passes."""
     import numpy as np
 
     d = np.clip(np.asarray(w.max_depths, np.int64), 0, 15)
@@ -355,8 +356,7 @@ def batched_sort_bytes(w):
                 break
             k, j1 = k2, j1 + 1
         n = int(jpo[j1] - jpo[j0])
-        bits = min(32, k + int(j1 - j0 - 1).bit_length())
-        passes = max(1, math.ceil(bits / 8))
+        passes = max(1, math.ceil(k / 8))
         if n:
             total += n * (12 + 16 * (passes - 1))
             launches += passes

@@ -10,8 +10,11 @@
 //
 // The jobs are processed as one forest in a shared index space (a group of jobs
 // whose job number and Morton key fit one 32-bit sort key):
-//   * keys are (job << key_bits) | Morton key, so one radix sort leaves every
-//     job's points in its own input range, in its own key order;
+//   * each job's points are sorted on their own Morton keys (a segmented radix
+//     sort: tiles never straddle two jobs, digit offsets restart at every job), so
+//     every job's points stay in its own input range, in its own key order; the
+//     last pass writes the composite key (job << key_bits) | Morton key that the
+//     forest's searches use;
 //   * level 0 is found per job; the level expansion is the single-tree kernel
 //     with per-node split rules (the job's depth and max_size);
 //   * the join is the single-tree breadth-first traversal, its top frontier built
@@ -109,40 +112,32 @@ __device__ __forceinline__ uint32_t wave_find_job(JobDev const* jobs, uint32_t l
 }
 
 // --- keys ----------------------------------------------------------------------------
-// One block per radix-sort tile; the tile's jobs are found once per block, each
-// point's job by a search among them. Also counts the sort's first digit.
+// Keys of the segmented point sort: one block per tile of the group's tile table
+// (a tile never holds two jobs' points), the Morton key alone (the sort's last pass
+// adds the job number above it). Also counts the sort's first digit.
 __global__ void __launch_bounds__(256)
-  batch_keygen_kernel(double const* __restrict__ xy, uint64_t n, JobDev const* __restrict__ jobs,
-                      uint32_t num_jobs, int key_bits, uint32_t* __restrict__ keys,
-                      uint32_t* __restrict__ counts)
+  batch_keygen_seg_kernel(double const* __restrict__ xy, SortTile const* __restrict__ tiles,
+                          JobDev const* __restrict__ jobs, uint32_t* __restrict__ keys,
+                          uint32_t* __restrict__ counts)
 {
   constexpr int NW = 256 / kWave;
   __shared__ uint32_t h[NW][kRadix];
-  __shared__ uint32_t s_jobs[2];
   int const w = threadIdx.x / kWave;
 #pragma unroll
   for (int ww = 0; ww < NW; ++ww) h[ww][threadIdx.x] = 0;
-  uint64_t const lo = static_cast<uint64_t>(blockIdx.x) * kSortTile;
-  uint64_t const hi = lo + kSortTile < n ? lo + kSortTile : n;
-  if (threadIdx.x == 0) {
-    s_jobs[0] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(lo));
-    s_jobs[1] = find_job(jobs, 0, num_jobs - 1, static_cast<uint32_t>(hi - 1));
-  }
   __syncthreads();
-  uint32_t const j0 = s_jobs[0], j1 = s_jobs[1];
+  SortTile const T = tiles[blockIdx.x];
+  JobDev const J   = jobs[T.segment];
 #pragma unroll 4
   for (int k = 0; k < kSortItems; ++k) {
-    uint64_t const i = lo + static_cast<uint64_t>(k) * 256 + threadIdx.x;
+    uint32_t const i = static_cast<uint32_t>(k) * 256 + threadIdx.x;
     uint32_t digit   = kRadix;
-    if (i < n) {
-      uint32_t const j = find_job(jobs, j0, j1, static_cast<uint32_t>(i));
-      JobDev const& J  = jobs[j];
-      double2 const p  = make_double2(xy[2 * i], xy[2 * i + 1]);
-      uint32_t const m = point_key(p.x, p.y, J.min_x, J.min_y, J.max_x, J.max_y, J.scale,
-                                   J.outside_key);
-      uint32_t const key = (key_bits < 32 ? (j << key_bits) : 0u) | m;
-      keys[i]            = key;
-      digit              = key & 0xFFu;
+    if (i < T.count) {
+      uint64_t const p = static_cast<uint64_t>(T.start) + i;
+      uint32_t const m = point_key(xy[2 * p], xy[2 * p + 1], J.min_x, J.min_y, J.max_x, J.max_y,
+                                   J.scale, J.outside_key);
+      keys[p] = m;
+      digit   = m & 0xFFu;
     }
     wave_count_digit(h[w], digit);
   }
@@ -719,6 +714,20 @@ uint64_t forest_capacity(std::vector<HostJob> const& hj, int64_t j0, int64_t j1,
   return cap;
 }
 
+// Tile table of a group's point sort: each job's points are a segment of their own
+// (the job order is the input order), sorted on the Morton key bits only; the last
+// pass writes (job << key_bits) | Morton key, the forest's composite key.
+void group_sort_tiles(int64_t j0, int64_t j1, int64_t const* jpo, int key_bits,
+                      std::vector<SortTile>& tiles, std::vector<SortSegment>& segs)
+{
+  size_t const J = static_cast<size_t>(j1 - j0);
+  std::vector<uint32_t> begin(J + 1), key_or(J);
+  for (size_t k = 0; k <= J; ++k) begin[k] = static_cast<uint32_t>(jpo[j0 + k] - jpo[j0]);
+  for (size_t k = 0; k < J; ++k)
+    key_or[k] = key_bits < 32 ? static_cast<uint32_t>(k) << key_bits : 0u;
+  make_sort_tiles(begin, tiles, segs, key_or);
+}
+
 // Groups of consecutive jobs whose job number and key fit 32 bits, < 2^31 points.
 std::vector<std::pair<int64_t, int64_t>> job_groups(std::vector<HostJob> const& hj,
                                                     int64_t const* jpo)
@@ -816,29 +825,38 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
     key_bits = std::max(key_bits, hj[j].key_bits);
     maxd     = std::max(maxd, hj[j].d);
   }
-  int const job_bits = bits_for(J - 1);
-  int const sort_bits = std::min(32, key_bits + job_bits);
 
   std::vector<JobDev> hd = job_descs(hj, j0, j1, jpo, jpoly);
+  std::vector<SortTile> tile_tab;
+  std::vector<SortSegment> seg_tab;
+  group_sort_tiles(j0, j1, jpo, key_bits, tile_tab, seg_tab);
   dbuf<JobDev> jobs(J);
+  dbuf<SortTile> stiles(tile_tab.size());
+  dbuf<SortSegment> ssegs(seg_tab.size());
   QPIP_HIP(hipMemcpyAsync(jobs.get(), hd.data(), J * sizeof(JobDev), hipMemcpyHostToDevice, stream));
-  QPIP_HIP(hipStreamSynchronize(stream));  // hd is pageable host memory
+  if (!tile_tab.empty())
+    QPIP_HIP(hipMemcpyAsync(stiles.get(), tile_tab.data(), tile_tab.size() * sizeof(SortTile),
+                            hipMemcpyHostToDevice, stream));
+  QPIP_HIP(hipMemcpyAsync(ssegs.get(), seg_tab.data(), seg_tab.size() * sizeof(SortSegment),
+                          hipMemcpyHostToDevice, stream));
+  QPIP_HIP(hipStreamSynchronize(stream));  // hd, tile_tab, seg_tab are pageable host memory
   for (uint32_t k = 0; k < J; ++k) job_hits[j0 + k] = 0;
   if (n == 0) return;
 
-  // 1-2. keys and the sort (values: group-local point index)
-  uint64_t const tiles = sort_tiles(n);
+  // 1-2. keys and the sort, each job's points on their own (values: group-local
+  // point index)
   dbuf<uint32_t> keys_a(n), keys_b(n), vals(n), vals_tmp(n);
-  dbuf<uint32_t> counts0(static_cast<size_t>(tiles) * kRadix);
+  dbuf<uint32_t> counts0(tile_tab.size() * kRadix);
   {
     KernelTimer kt("batch_keygen", stream);
-    batch_keygen_kernel<<<static_cast<unsigned>(tiles), 256, 0, stream>>>(
-      points_xy + 2 * P0, n, jobs.get(), J, key_bits, keys_a.get(), counts0.get());
+    batch_keygen_seg_kernel<<<static_cast<unsigned>(tile_tab.size()), 256, 0, stream>>>(
+      points_xy + 2 * P0, stiles.get(), jobs.get(), keys_a.get(), counts0.get());
     QPIP_LAUNCH_CHECK();
   }
-  SortResult sorted = radix_sort_pairs(keys_a.get(), keys_b.get(), nullptr, vals.get(),
-                                       vals_tmp.get(), n, sort_bits, stream, counts0.get(),
-                                       "batch_sort");
+  SortResult sorted = segmented_radix_sort_pairs(keys_a.get(), keys_b.get(), vals.get(),
+                                                 vals_tmp.get(), n, key_bits, stiles.get(),
+                                                 tile_tab.size(), ssegs.get(), seg_tab.size(), counts0.get(),
+                                                 stream, "batch_sort");
   vals_tmp.release();
 
   // 3. forest levels
@@ -1152,13 +1170,16 @@ struct PlanGroup {
   uint32_t J = 0;
   int64_t P0 = 0, PO0 = 0;
   uint64_t n = 0, npg = 0;
-  int key_bits = 1, maxd = 0, sort_bits = 1;
+  int key_bits = 1, maxd = 0;
   uint64_t node_cap = 0;
   bool tree = false, join = false;  // stages that run (host-known)
   PlanCaps cap;
   dbuf<JobDev> jobs;
   // tree
-  dbuf<uint32_t> keys_a, keys_b, vals, vals_tmp, counts0, sort_sums;
+  dbuf<uint32_t> keys_a, keys_b, vals, vals_tmp, counts0;
+  dbuf<SortTile> stiles;  // the segmented point sort's tile table
+  dbuf<SortSegment> ssegs;
+  uint64_t num_stiles = 0;
   dbuf<uint32_t> nkey, nstart, ncount, cfirst, ccount, njob, partials;
   dbuf<uint8_t> nlevel, f_internal;
   dbuf<uint32_t> f_length, f_offset;
@@ -1270,17 +1291,16 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
     return;
   }
   double const* xy = P.points_xy + 2 * g.P0;
-  // 1-2. keys and the sort
-  uint64_t const tiles = sort_tiles(g.n);
+  // 1-2. keys and the sort, each job's points on their own
   {
     KernelTimer kt("batch_keygen", s);
-    batch_keygen_kernel<<<static_cast<unsigned>(tiles), 256, 0, s>>>(
-      xy, g.n, g.jobs.get(), g.J, g.key_bits, g.keys_a.get(), g.counts0.get());
+    batch_keygen_seg_kernel<<<static_cast<unsigned>(g.num_stiles), 256, 0, s>>>(
+      xy, g.stiles.get(), g.jobs.get(), g.keys_a.get(), g.counts0.get());
     QPIP_LAUNCH_CHECK();
   }
-  SortResult sorted = radix_sort_pairs(g.keys_a.get(), g.keys_b.get(), nullptr, g.vals.get(),
-                                       g.vals_tmp.get(), g.n, g.sort_bits, s, g.counts0.get(),
-                                       "batch_sort", nullptr, g.sort_sums.get());
+  SortResult sorted = segmented_radix_sort_pairs(
+    g.keys_a.get(), g.keys_b.get(), g.vals.get(), g.vals_tmp.get(), g.n, g.key_bits,
+    g.stiles.get(), g.num_stiles, g.ssegs.get(), g.J, g.counts0.get(), s, "batch_sort");
   // 3. forest levels
   zero_struct(g.fmeta.get(), s);
   {
@@ -1426,22 +1446,30 @@ void plan_group_setup(BatchPlan& P, PlanGroup& g, hipStream_t s)
     g.key_bits = std::max(g.key_bits, hj[j].key_bits);
     g.maxd     = std::max(g.maxd, hj[j].d);
   }
-  g.sort_bits = std::min(32, g.key_bits + bits_for(g.J - 1));
   g.tree      = g.n > 0;
   g.join      = g.tree && g.npg > 0;
   std::vector<JobDev> hd = job_descs(hj, g.j0, g.j1, jpo, jpoly);
+  std::vector<SortTile> ht;
+  std::vector<SortSegment> hs;
+  group_sort_tiles(g.j0, g.j1, jpo, g.key_bits, ht, hs);
+  g.num_stiles = ht.size();
   g.jobs.reset(g.J);
+  g.stiles.reset(ht.size());
+  g.ssegs.reset(hs.size());
   QPIP_HIP(hipMemcpyAsync(g.jobs.get(), hd.data(), g.J * sizeof(JobDev), hipMemcpyHostToDevice,
                           s));
-  QPIP_HIP(hipStreamSynchronize(s));  // hd is pageable host memory
+  if (!ht.empty())
+    QPIP_HIP(hipMemcpyAsync(g.stiles.get(), ht.data(), ht.size() * sizeof(SortTile),
+                            hipMemcpyHostToDevice, s));
+  QPIP_HIP(hipMemcpyAsync(g.ssegs.get(), hs.data(), hs.size() * sizeof(SortSegment),
+                          hipMemcpyHostToDevice, s));
+  QPIP_HIP(hipStreamSynchronize(s));  // hd, ht, hs are pageable host memory
   if (!g.tree) return;
-  uint64_t const tiles = sort_tiles(g.n);
   g.keys_a.reset(g.n);
   g.keys_b.reset(g.n);
   g.vals.reset(g.n);
   g.vals_tmp.reset(g.n);
-  g.counts0.reset(tiles * kRadix);
-  g.sort_sums.reset(sort_sums_elems(g.n));
+  g.counts0.reset(g.num_stiles * kRadix);
   g.node_cap = forest_capacity(hj, g.j0, g.j1, jpo);
   uint64_t const cap = g.node_cap;
   g.nkey.reset(cap);

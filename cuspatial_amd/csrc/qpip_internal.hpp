@@ -388,6 +388,37 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
                             char const* tag              = "sort",
                             uint32_t* counts_ws          = nullptr,
                             uint32_t* sums_ws            = nullptr);
+// Segmented variant: the pairs form segments (consecutive ranges) sorted each on
+// its own, with tiles that never straddle two segments. Tile t covers positions
+// [start, start + count) of segment `segment`; segment s covers positions
+// [begin, ...) and tiles [tile_begin, tile_end). The values are the positions (the
+// identity); the first pass's digit counts per tile (tiles x 256) come from the
+// producer of the keys; the last pass writes each key OR-ed with its segment's
+// `key_or` (the job number above the Morton key, for the batched forest).
+struct SortTile {
+  uint32_t start, count, segment, pad;
+};
+struct SortSegment {
+  uint32_t begin, tile_begin, tile_end, key_or;
+};
+SortResult segmented_radix_sort_pairs(uint32_t* keys_a,
+                                      uint32_t* keys_b,
+                                      uint32_t* values_out,
+                                      uint32_t* values_tmp,
+                                      uint64_t n,
+                                      int bits,
+                                      SortTile const* tiles,
+                                      uint64_t num_tiles,
+                                      SortSegment const* segments,
+                                      uint64_t num_segments,
+                                      uint32_t* first_counts,
+                                      hipStream_t stream,
+                                      char const* tag);
+// Tiles of segments [begin[s], begin[s + 1]) of kSortTile positions (the last of a
+// segment shorter; empty segments get none).
+void make_sort_tiles(std::vector<uint32_t> const& begin, std::vector<SortTile>& tiles,
+                     std::vector<SortSegment>& segments, std::vector<uint32_t> const& key_or);
+
 // Workspace of radix_sort_pairs for n pairs when the caller provides it (no
 // allocation inside the call, as a captured launch sequence needs): counts_ws
 // (unused when first_counts is given) and sums_ws.

@@ -18,7 +18,9 @@
 //             digit's run at its position.
 // The first pass's counts usually come from the producer of the keys (the keygen
 // kernel counts digit 0 while it writes the keys).
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "qpip_sort.hpp"
 
@@ -212,7 +214,10 @@ __device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t tiles)
   return x * per + (x < rem ? x : rem) + r;
 }
 
-template <int NB>
+// SEG: the tile is entry `tile` of a segment tile table (its own start and count,
+// never straddling two segments), and with `compose` the keys are written OR-ed with
+// the segment's key_or.
+template <int NB, bool SEG>
 __global__ void __launch_bounds__(kSortThreads)
   radix_scatter_kernel(uint32_t const* __restrict__ keys_in,
                        uint32_t const* __restrict__ vals_in,  // null: identity
@@ -220,7 +225,10 @@ __global__ void __launch_bounds__(kSortThreads)
                        uint32_t* __restrict__ vals_out,
                        uint64_t n,
                        int shift,
-                       uint32_t const* __restrict__ offsets)  // tiles x 256, from the colscan
+                       uint32_t const* __restrict__ offsets,  // tiles x 256, from the colscan
+                       SortTile const* __restrict__ seg_tiles,
+                       SortSegment const* __restrict__ segments,
+                       bool compose)
 {
   constexpr int NW = kSortThreads / kWave;
   __shared__ uint32_t s_keys[kSortTile];
@@ -239,9 +247,17 @@ __global__ void __launch_bounds__(kSortThreads)
 #pragma unroll
   for (int ww = 0; ww < NW; ++ww) s_wcount[ww][d] = 0;
   __syncthreads();
-  uint64_t const base = tile * kSortTile + static_cast<uint64_t>(w) * (kSortItems * kWave);
+  uint64_t tile_lo = tile * kSortTile, end = n;
+  uint32_t key_or  = 0;
+  if constexpr (SEG) {
+    SortTile const T = seg_tiles[tile];
+    tile_lo          = T.start;
+    end              = static_cast<uint64_t>(T.start) + T.count;
+    if (compose) key_or = segments[T.segment].key_or;
+  }
+  uint64_t const base = tile_lo + static_cast<uint64_t>(w) * (kSortItems * kWave);
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
-  rank_tile<NB>(keys_in, vals_in, n, base, shift, key, val, rank, s_wcount[w]);
+  rank_tile<NB>(keys_in, vals_in, end, base, shift, key, val, rank, s_wcount[w]);
   __syncthreads();
   // per digit: exclusive prefix over waves and the tile's first slot of the digit
   uint32_t count = 0;
@@ -257,7 +273,7 @@ __global__ void __launch_bounds__(kSortThreads)
 #pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
     uint64_t const idx = base + static_cast<uint64_t>(j) * kWave + lane;
-    if (idx < n) {
+    if (idx < end) {
       uint32_t const dj  = digit_of(key[j], shift, mask);
       uint32_t const loc = s_tile_start[dj] + s_wcount[w][dj] + rank[j];
       s_keys[loc]        = key[j];
@@ -265,10 +281,9 @@ __global__ void __launch_bounds__(kSortThreads)
     }
   }
   __syncthreads();
-  uint64_t const tile_lo = tile * kSortTile;
   uint32_t const valid_n =
-    static_cast<uint32_t>((n - tile_lo) < static_cast<uint64_t>(kSortTile) ? (n - tile_lo)
-                                                                            : kSortTile);
+    static_cast<uint32_t>((end - tile_lo) < static_cast<uint64_t>(kSortTile) ? (end - tile_lo)
+                                                                              : kSortTile);
 #pragma unroll 4
   for (int k = 0; k < kSortItems; ++k) {
     uint32_t const i = static_cast<uint32_t>(k * kSortThreads + threadIdx.x);
@@ -276,8 +291,69 @@ __global__ void __launch_bounds__(kSortThreads)
       uint32_t const kk  = s_keys[i];
       uint32_t const dk  = digit_of(kk, shift, mask);
       uint64_t const pos = static_cast<uint64_t>(s_global[dk]) + (i - s_tile_start[dk]);
-      keys_out[pos]      = kk;
+      keys_out[pos]      = SEG ? (kk | key_or) : kk;
       vals_out[pos]      = s_vals[i];
+    }
+  }
+}
+
+// Digit counts of tile `blockIdx.x` of a segment tile table.
+__global__ void __launch_bounds__(kSortThreads)
+  seg_count_kernel(uint32_t const* __restrict__ keys, SortTile const* __restrict__ seg_tiles,
+                   int shift, uint32_t mask, uint32_t* __restrict__ counts)
+{
+  constexpr int NW = kSortThreads / kWave;
+  __shared__ uint32_t h[NW][kRadix];
+  int const w = threadIdx.x / kWave;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) h[ww][threadIdx.x] = 0;
+  __syncthreads();
+  SortTile const T = seg_tiles[blockIdx.x];
+  uint32_t d[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    uint32_t const i = static_cast<uint32_t>(j) * kSortThreads + threadIdx.x;
+    d[j]             = i < T.count ? digit_of(keys[T.start + i], shift, mask) : kRadix;
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) wave_count_digit(h[w], d[j]);
+  __syncthreads();
+  uint32_t s = 0;
+#pragma unroll
+  for (int ww = 0; ww < NW; ++ww) s += h[ww][threadIdx.x];
+  counts[static_cast<uint64_t>(blockIdx.x) * kRadix + threadIdx.x] = s;
+}
+
+// One block per segment: counts[t][d] (in place) -> first output position of tile
+// t's digit-d pairs: the segment's begin + the digit base within the segment
+// (exclusive scan of its digit totals) + the digit's count in the segment's
+// earlier tiles.
+__global__ void __launch_bounds__(kRadix)
+  seg_colscan_kernel(uint32_t* __restrict__ counts, SortSegment const* __restrict__ segments)
+{
+  __shared__ uint32_t lds[kRadix / kWave + 1];
+  SortSegment const S = segments[blockIdx.x];
+  int const d         = threadIdx.x;
+  uint32_t total      = 0;
+  for (uint32_t t = S.tile_begin; t < S.tile_end; t += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      v[u] = t + u < S.tile_end ? counts[static_cast<uint64_t>(t + u) * kRadix + d] : 0u;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) total += v[u];
+  }
+  uint32_t tot;
+  uint32_t run = S.begin + block_excl_scan<kRadix>(total, lds, &tot);
+  for (uint32_t t = S.tile_begin; t < S.tile_end; t += kBatch) {
+    uint32_t v[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u)
+      v[u] = t + u < S.tile_end ? counts[static_cast<uint64_t>(t + u) * kRadix + d] : 0u;
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      if (t + u < S.tile_end) counts[static_cast<uint64_t>(t + u) * kRadix + d] = run;
+      run += v[u];
     }
   }
 }
@@ -373,9 +449,89 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
     }
     {
       KernelTimer kt(t_scatter.c_str(), stream);
-#define QPIP_SCATTER(NB)                                                               \
-  radix_scatter_kernel<NB><<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>( \
-    ksrc, vsrc, kdst, vdst, n, shift, counts)
+#define QPIP_SCATTER(NB)                                                                      \
+  radix_scatter_kernel<NB, false><<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>( \
+    ksrc, vsrc, kdst, vdst, n, shift, counts, nullptr, nullptr, false)
+      QPIP_DISPATCH_NB(nb, QPIP_SCATTER);
+#undef QPIP_SCATTER
+      QPIP_LAUNCH_CHECK();
+    }
+    std::swap(ksrc, kdst);
+    vsrc = vdst;
+  }
+  return {ksrc, values_out};
+}
+
+void make_sort_tiles(std::vector<uint32_t> const& begin, std::vector<SortTile>& tiles,
+                     std::vector<SortSegment>& segments, std::vector<uint32_t> const& key_or)
+{
+  size_t const S = begin.size() - 1;
+  tiles.clear();
+  segments.resize(S);
+  for (size_t s = 0; s < S; ++s) {
+    SortSegment& g = segments[s];
+    g.begin        = begin[s];
+    g.tile_begin   = static_cast<uint32_t>(tiles.size());
+    g.key_or       = key_or[s];
+    for (uint64_t p = begin[s]; p < begin[s + 1]; p += kSortTile) {
+      uint64_t const c = std::min<uint64_t>(kSortTile, begin[s + 1] - p);
+      tiles.push_back(SortTile{static_cast<uint32_t>(p), static_cast<uint32_t>(c),
+                               static_cast<uint32_t>(s), 0u});
+    }
+    g.tile_end = static_cast<uint32_t>(tiles.size());
+  }
+}
+
+SortResult segmented_radix_sort_pairs(uint32_t* keys_a,
+                                      uint32_t* keys_b,
+                                      uint32_t* values_out,
+                                      uint32_t* values_tmp,
+                                      uint64_t n,
+                                      int bits,
+                                      SortTile const* tiles,
+                                      uint64_t num_tiles,
+                                      SortSegment const* segments,
+                                      uint64_t num_segments,
+                                      uint32_t* first_counts,
+                                      hipStream_t stream,
+                                      char const* tag)
+{
+  if (bits < 1) bits = 1;
+  if (bits > 32) bits = 32;
+  int const passes = radix_passes(bits);
+  if (n == 0 || num_tiles == 0) return {keys_a, values_out};
+  if (n >= (uint64_t{1} << 32) || num_tiles > 0x7FFFFFFFull)
+    throw logic_error("cuSpatial failure: sort input too large");
+  std::string const t_count   = std::string(tag) + "_count";
+  std::string const t_scan    = std::string(tag) + "_colscan";
+  std::string const t_scatter = std::string(tag) + "_scatter";
+  auto vdst_of = [&](int p) { return ((passes - 1 - p) % 2 == 0) ? values_out : values_tmp; };
+  uint32_t* ksrc       = keys_a;
+  uint32_t* kdst       = keys_b;
+  uint32_t const* vsrc = nullptr;  // the first pass's values are the positions
+  for (int p = 0; p < passes; ++p) {
+    int const shift     = p * kRadixBits;
+    int const nb        = (bits - shift) < kRadixBits ? (bits - shift) : kRadixBits;
+    uint32_t const mask = (1u << nb) - 1u;
+    bool const last     = p + 1 == passes;
+    uint32_t* vdst      = vdst_of(p);
+    if (p > 0) {
+      KernelTimer kt(t_count.c_str(), stream);
+      seg_count_kernel<<<static_cast<unsigned>(num_tiles), kSortThreads, 0, stream>>>(
+        ksrc, tiles, shift, mask, first_counts);
+      QPIP_LAUNCH_CHECK();
+    }
+    {
+      KernelTimer kt(t_scan.c_str(), stream);
+      seg_colscan_kernel<<<static_cast<unsigned>(num_segments), kRadix, 0, stream>>>(
+        first_counts, segments);
+      QPIP_LAUNCH_CHECK();
+    }
+    {
+      KernelTimer kt(t_scatter.c_str(), stream);
+#define QPIP_SCATTER(NB)                                                                         \
+  radix_scatter_kernel<NB, true><<<static_cast<unsigned>(num_tiles), kSortThreads, 0, stream>>>( \
+    ksrc, vsrc, kdst, vdst, n, shift, first_counts, tiles, segments, last)
       QPIP_DISPATCH_NB(nb, QPIP_SCATTER);
 #undef QPIP_SCATTER
       QPIP_LAUNCH_CHECK();
