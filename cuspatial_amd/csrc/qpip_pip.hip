@@ -193,14 +193,17 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
     uint64_t y0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) y0[u] = ballot(by0 > py[u]);
+    // The reference toggles on (y1 != y0) && ((lhs < rhs) != y1). Where the edge
+    // straddles the point (y1 != y0), y1 = (ay > by) = (rise < 0): the staging
+    // negates such edges' line form, so that t < 0 is (lhs < rhs) != y1 there.
     auto edge = [&](LineEdge const& ed, double tol) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         double const t    = fma(px[u], ed.rise, fma(-py[u], ed.run, ed.c));
         uint64_t const y1 = ballot(ed.ay > py[u]);
-        uint64_t const lt = ballot(t < 0.0);  // lhs < rhs
+        uint64_t const cr = ballot(t < 0.0);
         suspect_m[u] |= ballot(!(fabs(t) > tol));
-        inside_m[u] ^= (y1 ^ y0[u]) & (lt ^ y1);
+        inside_m[u] ^= (y1 ^ y0[u]) & cr;
         y0[u] = y1;
       }
     };
@@ -218,7 +221,9 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
         if (static_cast<uint32_t>(i + lane) < n) {
           Edge const ed = E[base + i + lane];
           double const ax_rise = ed.ax * ed.rise;
-          stage->e[i + lane]   = LineEdge{ed.ay, ed.run, ed.rise, fma(ed.ay, ed.run, -ax_rise)};
+          double const c       = fma(ed.ay, ed.run, -ax_rise);
+          stage->e[i + lane]   = ed.rise < 0.0 ? LineEdge{ed.ay, -ed.run, -ed.rise, -c}
+                                               : LineEdge{ed.ay, ed.run, ed.rise, c};
           amax                 = fmax(amax, fmax(fabs(ed.ax), fabs(ed.ay)));
           nrm[i / kWave]       = fabs(ed.run) + fabs(ed.rise);
         }
