@@ -265,6 +265,18 @@ def section_c2(args, dev, steps, warmup, cpu_seconds):
     n = w.num_points
     ms_per_step = (t1 - t0) * 1e3 / steps
     value = n * steps / (t1 - t0)
+    # the points' PCIe transfer, for the rate of a caller whose points start in
+    # (pinned) host memory: 16 bytes per point host -> device
+    host_xy = torch.empty(pts.xy.numel(), dtype=torch.float64, pin_memory=True)
+    host_xy.copy_(pts.xy)
+    dev_xy = torch.empty_like(pts.xy)
+    sync()
+    t = time.perf_counter()
+    for _ in range(3):
+        dev_xy.copy_(host_xy, non_blocking=True)
+    sync()
+    h2d_ms = (time.perf_counter() - t) * 1e3 / 3
+    del host_xy, dev_xy
     kernel_ms = sum(v for _, v in prof.values()) / steps
     log(f"[bench] c2: {ms_per_step:.3f} ms/step ({kernel_ms:.3f} ms in kernels), "
         f"{value:.4g} points/s; nodes={len(tree)} pairs={len(pairs)} hits={len(res)}")
@@ -297,6 +309,10 @@ def section_c2(args, dev, steps, warmup, cpu_seconds):
     out_d = {
         "value": value, "unit": "points/s", "ms_per_step": ms_per_step,
         "kernel_ms_per_step": round(kernel_ms, 4),
+        "with_h2d": {"value": n / ((ms_per_step + h2d_ms) / 1e3), "unit": "points/s",
+                     "h2d_ms": round(h2d_ms, 3),
+                     "what": "the same step plus copying the 1.6 GB of points from pinned "
+                             "host memory first (not the headline: inputs are resident)"},
         "config": {"workload": "gaussian_holes: plain quadtree point-in-polygon join (tree + "
                                "boxes + join + PIP), inputs resident in HBM",
                    "points": n, "point_layout": "interleaved x, y (vec_2d)",
@@ -788,8 +804,13 @@ def section_c5(args, dev, rates):
     sync()
     sweep = {}
     for name, index in (("plain_stars", plain_index), ("learned_voronoi", learned_index)):
-        # warm the allocators and the kernels once
+        # warm the allocators and the kernels once, and capture the plain index's
+        # plans of every batch size bucket on every worker stream
         serving.Server(index).serve(serving.poisson_requests(pool_xy, 1000.0, 20, seed=7))
+        t = time.perf_counter()
+        index.warm(pool_xy, [serving.worker_stream(dev, k) for k in range(3)])
+        warm_s = time.perf_counter() - t
+        index.plan_stats.clear()
         rows = []
         for qps in rates:
             count = max(200, int(qps * args.serve_seconds))
@@ -799,7 +820,9 @@ def section_c5(args, dev, rates):
                    "points_per_s": st.points_per_s, "achieved_qps": st.requests / st.seconds,
                    "latency_ms": st.latency_ms, "batches": st.batches,
                    "mean_batch_requests": st.mean_batch_requests,
-                   "max_in_flight": st.max_in_flight, "admission": st.extra}
+                   "max_in_flight": st.max_in_flight, "admission": st.extra,
+                   "plans": dict(index.plan_stats), "plan_warm_s": round(warm_s, 3)}
+            index.plan_stats.clear()
             rows.append(row)
             log(f"[bench] c5 {name} qps {qps:g}: p50 {st.latency_ms['p50']:.2f} ms, p95 "
                 f"{st.latency_ms['p95']:.2f} ms, p99 {st.latency_ms['p99']:.2f} ms, "

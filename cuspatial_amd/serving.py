@@ -55,8 +55,10 @@ class ResidentIndex:
     the four eager entry points. Plans are kept per stream and per size bucket
     (the next power of two of the batch's points, at least ``min_plan_points``):
     the batch's points are copied into the plan's point buffer and the rest of
-    the bucket is filled with NaN points, which land in one quadtree cell and
-    are inside no polygon, so the answer is the same. A batch that outgrows the
+    the bucket is filled with one point far outside the bounding box (``pad``),
+    which the quadtree puts in its outside cell and which is inside no polygon,
+    so the answer is the same. ``warm`` creates the plans ahead of serving. A
+    batch that outgrows the
     plan's capacities (CapacityError) rebuilds the plan from its own points;
     batches above ``max_plan_points`` take the eager path."""
 
@@ -77,6 +79,25 @@ class ResidentIndex:
         self._plans: dict = {}  # (stream handle, bucket) -> (BatchPlan, point buffer)
         self._plans_lock = threading.Lock()
         self.plan_stats = collections.Counter()
+        # padding of a plan's point buffer: finite, outside the bbox on both axes
+        x0, x1, y0, y1 = self.bbox
+        self.pad = (max(x0, x1) + abs(x1 - x0) + 1.0, max(y0, y1) + abs(y1 - y0) + 1.0)
+
+    def warm(self, xy_pool: torch.Tensor, streams, max_points: int | None = None):
+        """Creates the captured plan of every size bucket up to ``max_points``
+        (default ``max_plan_points``) on each of ``streams``, sized on the first
+        points of ``xy_pool``, so that no request waits for a capture."""
+        if not self.captured:
+            return
+        top = min(self.max_plan_points, max_points or self.max_plan_points,
+                  xy_pool.numel() // 2)
+        b = self.min_plan_points
+        while b <= top:
+            for s in streams:
+                with torch.cuda.stream(s):
+                    self._query_captured(xy_pool[:2 * b])
+                s.synchronize()
+            b *= 2
 
     def close(self):
         """Releases the captured plans."""
@@ -109,13 +130,18 @@ class ResidentIndex:
         with self._plans_lock:
             entry = self._plans.get(key)
         if entry is None:
-            buf = torch.full((2 * bucket,), math.nan, dtype=torch.float64, device=xy.device)
+            buf = torch.empty(2 * bucket, dtype=torch.float64, device=xy.device)
             buf[:2 * n].copy_(xy)
+            tail = buf.view(-1, 2)[n:]
+            tail[:, 0].fill_(self.pad[0])
+            tail[:, 1].fill_(self.pad[1])
             plan = self._plan_for(key, buf)
         else:
             plan, buf = entry
             buf[:2 * n].copy_(xy)
-            buf[2 * n:].fill_(math.nan)
+            tail = buf.view(-1, 2)[n:]
+            tail[:, 0].fill_(self.pad[0])
+            tail[:, 1].fill_(self.pad[1])
         plan.launch()
         try:
             res = plan.result()

@@ -141,7 +141,8 @@ def test_concurrent_calls_match_serial(env):
 
 def test_captured_batches_equal_eager_batches(env):
     """The plain index answers a batch by replaying a captured plan of the batch's
-    size bucket (NaN points fill the bucket): the pairs equal the eager path's
+    size bucket (a point outside the bounding box fills the bucket): the pairs
+    equal the eager path's
     for batch sizes across bucket edges, and a batch that outgrows the plan's
     capacities (many more hits than the batch the plan was sized on) rebuilds the
     plan and still answers exactly."""
@@ -156,6 +157,20 @@ def test_captured_batches_equal_eager_batches(env):
         a, b = captured.query(xy), eager.query(xy)
         assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]), n
     assert captured.plan_stats["replays"] == 7 and eager.plan_stats["eager"] == 7
+    # plans created ahead of serving: every bucket up to 2^14 on two streams,
+    # then queries of those sizes replay them without a capture
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    pool = torch.as_tensor(rng.uniform(0, 1000.0, 2 * 20000), device="cuda")
+    created = captured.plan_stats["created"]
+    captured.warm(pool, streams, max_points=1 << 14)
+    assert captured.plan_stats["created"] == created + 2 * 3  # 2^12, 2^13, 2^14
+    for s in streams:
+        with torch.cuda.stream(s):
+            a = captured.query(pool[:2 * 9000])
+        s.synchronize()
+        b = eager.query(pool[:2 * 9000])
+        assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1])
+    assert captured.plan_stats["created"] == created + 6
     # same bucket, sized on uniform points, then every point inside one polygon
     xy = torch.as_tensor(rng.uniform(0, 1000.0, 2 * 5000), device="cuda")
     captured.query(xy)
