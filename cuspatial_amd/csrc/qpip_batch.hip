@@ -468,35 +468,6 @@ __global__ void __launch_bounds__(256)
 }
 
 // --- results back to job-local indices ---------------------------------------------
-// Tiles of 4096 sorted positions per block iteration: the tile's first and last
-// jobs are found once (waves 0 and 1 search in parallel), each position's job by
-// a search between them.
-__global__ void __launch_bounds__(256)
-  local_point_indices_kernel(uint32_t const* __restrict__ sorted_vals, uint64_t n,
-                             JobDev const* __restrict__ jobs, uint32_t num_jobs,
-                             uint32_t* __restrict__ out)
-{
-  constexpr uint64_t kTile = 4096;
-  __shared__ uint32_t s_jobs[2];
-  int const w = threadIdx.x / kWave;
-  for (uint64_t lo = static_cast<uint64_t>(blockIdx.x) * kTile; lo < n;
-       lo += static_cast<uint64_t>(gridDim.x) * kTile) {
-    uint64_t const hi = lo + kTile < n ? lo + kTile : n;
-    __syncthreads();  // the previous tile's readers of s_jobs are done
-    if (w < 2) {  // wave-uniform
-      uint32_t const j = wave_find_job(jobs, 0, num_jobs - 1,
-                                       static_cast<uint32_t>(w == 0 ? lo : hi - 1));
-      if (lane_id() == 0) s_jobs[w] = j;
-    }
-    __syncthreads();
-    uint32_t const j0 = s_jobs[0], j1 = s_jobs[1];
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-      uint32_t const j = j0 == j1 ? j0 : find_job(jobs, j0, j1, static_cast<uint32_t>(i));
-      out[i]           = sorted_vals[i] - jobs[j].point_begin;
-    }
-  }
-}
-
 // Job-local (polygon, point) of hits [0, h), written at rows base + i. Hits are in
 // job order (pairs follow the leaves' sorted positions), so a tile's first and
 // last hits bound its jobs: per 4096-hit tile, waves 0 and 1 find those two jobs
@@ -845,7 +816,7 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
 
   // 1-2. keys and the sort, each job's points on their own (values: group-local
   // point index)
-  dbuf<uint32_t> keys_a(n), keys_b(n), vals(n), vals_tmp(n);
+  dbuf<uint32_t> keys_a(n), keys_b(n), vals(n), vals_tmp(n), samples((n + 31) / 32);
   dbuf<uint32_t> counts0(tile_tab.size() * kRadix);
   {
     KernelTimer kt("batch_keygen", stream);
@@ -853,10 +824,11 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
       points_xy + 2 * P0, stiles.get(), jobs.get(), keys_a.get(), counts0.get());
     QPIP_LAUNCH_CHECK();
   }
+  // the last pass also writes the job-local point_indices output
   SortResult sorted = segmented_radix_sort_pairs(keys_a.get(), keys_b.get(), vals.get(),
                                                  vals_tmp.get(), n, key_bits, stiles.get(),
                                                  tile_tab.size(), ssegs.get(), seg_tab.size(), counts0.get(),
-                                                 stream, "batch_sort");
+                                                 out_pi, samples.get(), stream, "batch_sort");
   vals_tmp.release();
 
   // 3. forest levels
@@ -882,7 +854,7 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
     for (int l = 0; l + 2 <= maxd; ++l) {
       LevelArgs la{sorted.keys, nkey.get(), nstart.get(), ncount.get(), nlevel.get(),
                    cfirst.get(), ccount.get(), split.get(), partials.get(), fmeta.get()->sizes,
-                   &fmeta.get()->overflow, cap, l};
+                   &fmeta.get()->overflow, cap, l, samples.get()};
       level_split_kernel<<<kLevelBlocks, 256, 0, stream>>>(la, rule);
       QPIP_LAUNCH_CHECK();
       level_emit_kernel<<<kLevelBlocks, 256, 0, stream>>>(la, rule);
@@ -907,11 +879,6 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
       jobs.get(), trees.get(), M, nullptr, 0, f_internal.get(), f_length.get(), f_offset.get());
     QPIP_LAUNCH_CHECK();
   }
-  // point_indices (job-local)
-  // one block per 4096-position tile
-  local_point_indices_kernel<<<grid_for(n, 4096), 256, 0, stream>>>(vals.get(), n, jobs.get(), J,
-                                                                     out_pi);
-  QPIP_LAUNCH_CHECK();
   if (M == 0 || npg == 0) {
     QPIP_HIP(hipStreamSynchronize(stream));
     return;
@@ -1176,7 +1143,7 @@ struct PlanGroup {
   PlanCaps cap;
   dbuf<JobDev> jobs;
   // tree
-  dbuf<uint32_t> keys_a, keys_b, vals, vals_tmp, counts0;
+  dbuf<uint32_t> keys_a, keys_b, vals, vals_tmp, counts0, samples;
   dbuf<SortTile> stiles;  // the segmented point sort's tile table
   dbuf<SortSegment> ssegs;
   uint64_t num_stiles = 0;
@@ -1298,9 +1265,11 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
       xy, g.stiles.get(), g.jobs.get(), g.keys_a.get(), g.counts0.get());
     QPIP_LAUNCH_CHECK();
   }
+  // the last pass also writes the job-local point_indices output
   SortResult sorted = segmented_radix_sort_pairs(
     g.keys_a.get(), g.keys_b.get(), g.vals.get(), g.vals_tmp.get(), g.n, g.key_bits,
-    g.stiles.get(), g.num_stiles, g.ssegs.get(), g.J, g.counts0.get(), s, "batch_sort");
+    g.stiles.get(), g.num_stiles, g.ssegs.get(), g.J, g.counts0.get(), P.out_pi.get() + g.P0,
+    g.samples.get(), s, "batch_sort");
   // 3. forest levels
   zero_struct(g.fmeta.get(), s);
   {
@@ -1317,7 +1286,8 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
     for (int l = 0; l + 2 <= g.maxd; ++l) {
       LevelArgs la{sorted.keys, g.nkey.get(), g.nstart.get(), g.ncount.get(), g.nlevel.get(),
                    g.cfirst.get(), g.ccount.get(), g.split.get(), g.partials.get(),
-                   g.fmeta.get()->sizes, &g.fmeta.get()->overflow, g.node_cap, l};
+                   g.fmeta.get()->sizes, &g.fmeta.get()->overflow, g.node_cap, l,
+                   g.samples.get()};
       level_split_kernel<<<kLevelBlocks, 256, 0, s>>>(la, rule);
       QPIP_LAUNCH_CHECK();
       level_emit_kernel<<<kLevelBlocks, 256, 0, s>>>(la, rule);
@@ -1334,9 +1304,6 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
     g.nkey.get(), g.nstart.get(), g.ncount.get(), g.nlevel.get(), g.njob.get(), g.cfirst.get(),
     g.ccount.get(), g.jobs.get(), g.trees.get(), 0, g.fmeta.get(), levels, g.f_internal.get(),
     g.f_length.get(), g.f_offset.get());
-  QPIP_LAUNCH_CHECK();
-  local_point_indices_kernel<<<grid_for(g.n, 4096), 256, 0, s>>>(g.vals.get(), g.n, g.jobs.get(),
-                                                                g.J, P.out_pi.get() + g.P0);
   QPIP_LAUNCH_CHECK();
   if (!g.join) {
     plan_empty_kernel<<<grid_for(g.J, 256), 256, 0, s>>>(cnt, prev, g.fmeta.get(), job_first,
@@ -1469,6 +1436,7 @@ void plan_group_setup(BatchPlan& P, PlanGroup& g, hipStream_t s)
   g.keys_b.reset(g.n);
   g.vals.reset(g.n);
   g.vals_tmp.reset(g.n);
+  g.samples.reset((g.n + 31) / 32);
   g.counts0.reset(g.num_stiles * kRadix);
   g.node_cap = forest_capacity(hj, g.j0, g.j1, jpo);
   uint64_t const cap = g.node_cap;

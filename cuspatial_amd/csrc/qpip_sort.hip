@@ -216,7 +216,9 @@ __device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t tiles)
 
 // SEG: the tile is entry `tile` of a segment tile table (its own start and count,
 // never straddling two segments), and with `compose` the keys are written OR-ed with
-// the segment's key_or.
+// the segment's key_or and, when `vals_local` is given, each value minus its
+// segment's first position is written there too; `key_samples`, when given,
+// receives the key written at every position that is a multiple of 32.
 template <int NB, bool SEG>
 __global__ void __launch_bounds__(kSortThreads)
   radix_scatter_kernel(uint32_t const* __restrict__ keys_in,
@@ -228,7 +230,9 @@ __global__ void __launch_bounds__(kSortThreads)
                        uint32_t const* __restrict__ offsets,  // tiles x 256, from the colscan
                        SortTile const* __restrict__ seg_tiles,
                        SortSegment const* __restrict__ segments,
-                       bool compose)
+                       bool compose,
+                       uint32_t* __restrict__ vals_local,
+                       uint32_t* __restrict__ key_samples)
 {
   constexpr int NW = kSortThreads / kWave;
   __shared__ uint32_t s_keys[kSortTile];
@@ -248,12 +252,15 @@ __global__ void __launch_bounds__(kSortThreads)
   for (int ww = 0; ww < NW; ++ww) s_wcount[ww][d] = 0;
   __syncthreads();
   uint64_t tile_lo = tile * kSortTile, end = n;
-  uint32_t key_or  = 0;
+  uint32_t key_or = 0, seg_begin = 0;
   if constexpr (SEG) {
     SortTile const T = seg_tiles[tile];
     tile_lo          = T.start;
     end              = static_cast<uint64_t>(T.start) + T.count;
-    if (compose) key_or = segments[T.segment].key_or;
+    if (compose) {
+      key_or    = segments[T.segment].key_or;
+      seg_begin = segments[T.segment].begin;
+    }
   }
   uint64_t const base = tile_lo + static_cast<uint64_t>(w) * (kSortItems * kWave);
   uint32_t key[kSortItems], val[kSortItems], rank[kSortItems];
@@ -291,8 +298,12 @@ __global__ void __launch_bounds__(kSortThreads)
       uint32_t const kk  = s_keys[i];
       uint32_t const dk  = digit_of(kk, shift, mask);
       uint64_t const pos = static_cast<uint64_t>(s_global[dk]) + (i - s_tile_start[dk]);
-      keys_out[pos]      = SEG ? (kk | key_or) : kk;
-      vals_out[pos]      = s_vals[i];
+      uint32_t const v  = s_vals[i];
+      uint32_t const ko = SEG ? (kk | key_or) : kk;
+      keys_out[pos]      = ko;
+      vals_out[pos]      = v;
+      if (SEG && vals_local) vals_local[pos] = v - seg_begin;
+      if (SEG && key_samples && (pos & 31u) == 0) key_samples[pos >> 5] = ko;
     }
   }
 }
@@ -451,7 +462,7 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
       KernelTimer kt(t_scatter.c_str(), stream);
 #define QPIP_SCATTER(NB)                                                                      \
   radix_scatter_kernel<NB, false><<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>( \
-    ksrc, vsrc, kdst, vdst, n, shift, counts, nullptr, nullptr, false)
+    ksrc, vsrc, kdst, vdst, n, shift, counts, nullptr, nullptr, false, nullptr, nullptr)
       QPIP_DISPATCH_NB(nb, QPIP_SCATTER);
 #undef QPIP_SCATTER
       QPIP_LAUNCH_CHECK();
@@ -493,6 +504,8 @@ SortResult segmented_radix_sort_pairs(uint32_t* keys_a,
                                       SortSegment const* segments,
                                       uint64_t num_segments,
                                       uint32_t* first_counts,
+                                      uint32_t* values_local,
+                                      uint32_t* key_samples,
                                       hipStream_t stream,
                                       char const* tag)
 {
@@ -531,7 +544,8 @@ SortResult segmented_radix_sort_pairs(uint32_t* keys_a,
       KernelTimer kt(t_scatter.c_str(), stream);
 #define QPIP_SCATTER(NB)                                                                         \
   radix_scatter_kernel<NB, true><<<static_cast<unsigned>(num_tiles), kSortThreads, 0, stream>>>( \
-    ksrc, vsrc, kdst, vdst, n, shift, first_counts, tiles, segments, last)
+    ksrc, vsrc, kdst, vdst, n, shift, first_counts, tiles, segments, last,               \
+    last ? values_local : nullptr, last ? key_samples : nullptr)
       QPIP_DISPATCH_NB(nb, QPIP_SCATTER);
 #undef QPIP_SCATTER
       QPIP_LAUNCH_CHECK();

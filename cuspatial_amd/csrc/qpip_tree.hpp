@@ -66,6 +66,13 @@ __device__ __forceinline__ uint32_t point_key(double x, double y, double min_x, 
 // one node at a time (three interleaved binary searches).
 constexpr int kLevelBlocks = 2048;
 
+// Every kKeySample-th sorted key, written by the sort's last pass (samples[j] =
+// K[j * kKeySample]): a search first narrows its range on the samples (an array
+// 1/32 the size of the keys, which stays in the caches) and then reads one
+// 128-byte line of keys, instead of a line per probe.
+constexpr uint32_t kKeySampleShift = 5;
+constexpr uint32_t kKeySample      = 1u << kKeySampleShift;
+
 struct LevelArgs {
   uint32_t const* K;  // sorted keys
   uint32_t* nkey;
@@ -80,6 +87,7 @@ struct LevelArgs {
   uint32_t* overflow;  // set when the node capacity is exceeded
   uint64_t cap;
   int level;
+  uint32_t const* samples = nullptr;  // K[j * kKeySample], or null: search K alone
 };
 
 // Split rule of one quadtree (every node has the same parameters): a node of
@@ -200,6 +208,83 @@ __device__ __forceinline__ void thread_split(uint32_t const* __restrict__ K, uin
   b3 = lo[2];
 }
 
+// thread_split over the key samples: three interleaved lower bounds on the
+// samples inside [s, e), then on the (at most 32) keys each one leaves.
+// Same results as thread_split: with a = ceil(s / 32), b = floor((e - 1) / 32)
+// and j* the first sample in [a, b] at or above the target, the first key at or
+// above the target lies in [s, 32a] (j* = a), (32(j* - 1), 32 j*] (a < j* <= b)
+// or (32 b, e] (no sample above it).
+__device__ __forceinline__ void sampled_split(uint32_t const* __restrict__ K,
+                                              uint32_t const* __restrict__ S, uint32_t s,
+                                              uint32_t e, uint32_t key, int sh, uint64_t prefix,
+                                              uint32_t& b1, uint32_t& b2, uint32_t& b3)
+{
+  uint32_t const a = (s + kKeySample - 1) >> kKeySampleShift;
+  uint32_t const b = (e - 1) >> kKeySampleShift;  // e > s
+  uint64_t t[3];
+  uint32_t lo[3], hi[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    t[q]  = prefix | ((static_cast<uint64_t>(key) * 4 + static_cast<uint64_t>(q + 1)) << sh);
+    lo[q] = a;
+    hi[q] = a <= b ? b + 1 : a;  // no sample inside: the window is [s, e)
+  }
+  while (lo[0] < hi[0] || lo[1] < hi[1] || lo[2] < hi[2]) {
+    uint32_t mid[3], v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      mid[q] = lo[q] + ((hi[q] - lo[q]) >> 1);
+      v[q]   = lo[q] < hi[q] ? S[mid[q]] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (lo[q] < hi[q]) {
+        if (static_cast<uint64_t>(v[q]) < t[q])
+          lo[q] = mid[q] + 1;
+        else
+          hi[q] = mid[q];
+      }
+    }
+  }
+  // lo[q] = j*; the window of keys that holds the answer
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    uint32_t const j = lo[q];
+    uint32_t wlo, whi;
+    if (a > b) {
+      wlo = s, whi = e;
+    } else if (j == a) {
+      wlo = s, whi = (a << kKeySampleShift) + 1;
+    } else if (j <= b) {
+      wlo = ((j - 1) << kKeySampleShift) + 1, whi = (j << kKeySampleShift) + 1;
+    } else {
+      wlo = (b << kKeySampleShift) + 1, whi = e;
+    }
+    lo[q] = wlo;
+    hi[q] = whi;
+  }
+  while (lo[0] < hi[0] || lo[1] < hi[1] || lo[2] < hi[2]) {
+    uint32_t mid[3], v[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      mid[q] = lo[q] + ((hi[q] - lo[q]) >> 1);
+      v[q]   = lo[q] < hi[q] ? K[mid[q]] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      if (lo[q] < hi[q]) {
+        if (static_cast<uint64_t>(v[q]) < t[q])
+          lo[q] = mid[q] + 1;
+        else
+          hi[q] = mid[q];
+      }
+    }
+  }
+  b1 = lo[0];
+  b2 = lo[1];
+  b3 = lo[2];
+}
+
 template <class Rule>
 __global__ void __launch_bounds__(256) level_split_kernel(LevelArgs a, Rule rule)
 {
@@ -241,7 +326,10 @@ __global__ void __launch_bounds__(256) level_split_kernel(LevelArgs a, Rule rule
       if (rule.rule(node, a.level, ms, sh, prefix) && c > ms) {
         uint32_t const s = a.nstart[node];
         uint32_t b1, b2, b3;
-        thread_split(a.K, s, s + c, a.nkey[node], sh, prefix, b1, b2, b3);
+        if (a.samples)
+          sampled_split(a.K, a.samples, s, s + c, a.nkey[node], sh, prefix, b1, b2, b3);
+        else
+          thread_split(a.K, s, s + c, a.nkey[node], sh, prefix, b1, b2, b3);
         sp = make_uint4(b1, b2, b3, nonempty(s, s + c, b1, b2, b3));
       }
       a.split[node] = sp;
