@@ -601,7 +601,8 @@ def train_locator(cs, ranker, polys, w, xy, n, train_points, epochs, dev, log_pr
     m = min(train_points, n)
     gen = torch.Generator(device=dev)
     gen.manual_seed(w.seed)
-    sample = torch.randperm(n, generator=gen, device=dev)[:m]
+    # a random training sample (with replacement: no sort on the device)
+    sample = torch.randint(0, n, (m,), generator=gen, device=dev)
     train_xy = xy.view(-1, 2)[sample].reshape(-1).contiguous()
     poly, pt = ranker.plain_pairs(cs.Points.from_points_xy(train_xy), polys, bbox, w.scale,
                                   w.max_depth, w.max_size)
@@ -654,13 +655,8 @@ def section_c4(args, dev, steps, warmup, cpu_seconds, by_batch=False):
     plain = plain_step_fn(cs, pts, polys, bbox, w.scale, w.max_depth, w.max_size)
     (pi, _, _, hits), plain_ms = timed(plain, steps, warmup)
     plain_value = n / (plain_ms / 1e3)
-    pos = hits["point_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-    want_pt = pi.view(torch.int32).to(torch.int64)[pos] & 0xFFFFFFFF
-    want_poly = hits["polygon_index"].view(torch.int32).to(torch.int64) & 0xFFFFFFFF
-    del pi, hits, pos
-    order = torch.argsort(want_pt * (1 << 32) + want_poly)
-    want_pt, want_poly = want_pt[order], want_poly[order]
-    del order
+    want_poly, want_pt = ranker.join_pairs_by_point(pi, hits, n)  # the library's sort
+    del pi, hits
     per_point = ranker.per_point_polygon(want_poly, want_pt, n)
     log(f"[bench] c4 plain join: {plain_ms:.2f} ms per step, {plain_value:.4g} points/s")
 

@@ -248,16 +248,33 @@ __global__ void __launch_bounds__(256)
   atomicAdd(stats + 2, 1.f);
 }
 
-// out[c] += sum over rows of D[row][c] (bf16, `cols` columns), 256 rows per block.
+// out[c] += sum over rows of D[row][c] (bf16, `cols` columns, a divisor of 256).
+// Thread t reads column t % cols of rows t / cols, t / cols + 256 / cols, ...
+// (consecutive threads read consecutive bf16 of a row: coalesced), 16 rows each;
+// the block's partial sums per column meet in LDS, one atomic per column.
+constexpr int kColsumIters = 16;
+inline int64_t colsum_rows_per_block(int cols) { return int64_t{256} / cols * kColsumIters; }
+
 __global__ void __launch_bounds__(256)
   colsum_kernel(uint16_t const* __restrict__ D, int64_t rows, int cols, float* __restrict__ out)
 {
-  int64_t const r0 = static_cast<int64_t>(blockIdx.x) * 256;
-  int64_t const r1 = r0 + 256 < rows ? r0 + 256 : rows;
-  for (int c = threadIdx.x; c < cols; c += blockDim.x) {
-    float s = 0.f;
-    for (int64_t r = r0; r < r1; ++r) s += from_bf16(D[r * cols + c]);
-    atomicAdd(out + c, s);
+  __shared__ float part[256];
+  int const c            = threadIdx.x % cols;
+  int64_t const step     = 256 / cols;
+  int64_t const r0       = static_cast<int64_t>(blockIdx.x) * step * kColsumIters;
+  int64_t const r_end    = r0 + step * kColsumIters < rows ? r0 + step * kColsumIters : rows;
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < kColsumIters; ++k) {
+    int64_t const r = r0 + threadIdx.x / cols + k * step;
+    if (r < r_end) s += from_bf16(D[r * cols + c]);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < cols) {
+    float t = 0.f;
+    for (int k = threadIdx.x; k < 256; k += cols) t += part[k];
+    atomicAdd(out + threadIdx.x, t);
   }
 }
 
@@ -862,8 +879,12 @@ extern "C" int qpip_ranker_gradients(const qpip_ranker* model,
     GemmArgs w3 = gemm(dZ.get(), 1, kCand, act.H2.get(), kHid, 1, kCand, kHid, rows);
     w3.kchunk = kc, w3.atomic = 1, w3.C = grad + lay.w3, w3.c_m = kHid, w3.c_n = 1;
     launch_gemm(w3, stream);
-    unsigned const cb = grid_for(static_cast<uint64_t>(rows), 256, 1u << 30);
-    colsum_kernel<<<cb, 256, 0, stream>>>(dZ.get(), rows, kCand, grad + lay.b3);
+    static_assert(256 % kCand == 0 && 256 % kHid == 0, "colsum_kernel needs cols | 256");
+    auto colsum_grid = [&](int cols) {
+      return static_cast<unsigned>((rows + colsum_rows_per_block(cols) - 1) /
+                                   colsum_rows_per_block(cols));
+    };
+    colsum_kernel<<<colsum_grid(kCand), 256, 0, stream>>>(dZ.get(), rows, kCand, grad + lay.b3);
     QPIP_LAUNCH_CHECK();
     // dH2 = (dZ W3) * (H2 > 0)
     GemmArgs h2 = gemm(dZ.get(), kCand, 1, pb + lay.w3, kHid, 1, rows, kHid, kCand);
@@ -873,7 +894,7 @@ extern "C" int qpip_ranker_gradients(const qpip_ranker* model,
     GemmArgs w2 = gemm(dH2.get(), 1, kHid, act.H1.get(), kHid, 1, kHid, kHid, rows);
     w2.kchunk = kc, w2.atomic = 1, w2.C = grad + lay.w2, w2.c_m = kHid, w2.c_n = 1;
     launch_gemm(w2, stream);
-    colsum_kernel<<<cb, 256, 0, stream>>>(dH2.get(), rows, kHid, grad + lay.b2);
+    colsum_kernel<<<colsum_grid(kHid), 256, 0, stream>>>(dH2.get(), rows, kHid, grad + lay.b2);
     QPIP_LAUNCH_CHECK();
     // dH1 = (dH2 W2) * (H1 > 0)
     GemmArgs h1 = gemm(dH2.get(), kHid, 1, pb + lay.w2, kHid, 1, rows, kHid, kHid);
@@ -883,7 +904,7 @@ extern "C" int qpip_ranker_gradients(const qpip_ranker* model,
     GemmArgs w1 = gemm(dH1.get(), 1, kHid, act.X.get(), kIn, 1, kHid, kIn, rows);
     w1.kchunk = kc, w1.atomic = 1, w1.C = grad + lay.w1, w1.c_m = kIn, w1.c_n = 1;
     launch_gemm(w1, stream);
-    colsum_kernel<<<cb, 256, 0, stream>>>(dH1.get(), rows, kHid, grad + lay.b1);
+    colsum_kernel<<<colsum_grid(kHid), 256, 0, stream>>>(dH1.get(), rows, kHid, grad + lay.b1);
     QPIP_LAUNCH_CHECK();
     // dX[:, :64] = dH1 W1[:, :64], scattered into the cells' embeddings
     GemmArgs xe = gemm(dH1.get(), kHid, 1, pb + lay.w1, kIn, 1, rows, kEmb, kHid);
