@@ -60,11 +60,14 @@ class ResidentIndex:
     so the answer is the same. ``warm`` creates the plans ahead of serving. A
     batch that outgrows the
     plan's capacities (CapacityError) rebuilds the plan from its own points;
-    batches above ``max_plan_points`` take the eager path."""
+    batches above ``max_plan_points`` take the eager path (default 2^20 points:
+    above it the eager path's launches and host waits are small next to the
+    work, while a plan pads the batch up to its bucket's power of two, up to
+    twice the points)."""
 
     def __init__(self, polygons: Polygons, bbox, scale, max_depth, max_size, locator=None,
                  captured: bool = True, min_plan_points: int = 1 << 12,
-                 max_plan_points: int = 1 << 24, plan_headroom: float = 4.0):
+                 max_plan_points: int = 1 << 20, plan_headroom: float = 4.0):
         self.polygons = polygons
         self.bbox = tuple(float(v) for v in bbox)
         self.scale = float(scale)
