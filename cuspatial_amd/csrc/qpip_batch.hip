@@ -302,7 +302,8 @@ __device__ __forceinline__ void finalize_node(
   uint32_t const* __restrict__ njob, uint32_t const* __restrict__ cfirst,
   uint32_t const* __restrict__ ccount, JobDev const* __restrict__ jobs,
   JobTree const* __restrict__ trees, uint64_t i, uint8_t* __restrict__ internal_out,
-  uint32_t* __restrict__ length_out, uint32_t* __restrict__ offset_out)
+  uint32_t* __restrict__ length_out, uint32_t* __restrict__ offset_out,
+  uint32_t* __restrict__ base_out)
 {
   uint32_t const j     = njob[i];
   JobDev const& J      = jobs[j];
@@ -332,6 +333,7 @@ __device__ __forceinline__ void finalize_node(
   internal_out[i] = internal ? 1 : 0;
   length_out[i]   = length;
   offset_out[i]   = offset;
+  if (base_out) base_out[i] = J.point_begin;
 }
 
 __global__ void __launch_bounds__(256)
@@ -342,7 +344,7 @@ __global__ void __launch_bounds__(256)
                         JobTree const* __restrict__ trees, uint64_t m,
                         ForestMeta const* __restrict__ meta, int levels,
                         uint8_t* __restrict__ internal_out, uint32_t* __restrict__ length_out,
-                        uint32_t* __restrict__ offset_out)
+                        uint32_t* __restrict__ offset_out, uint32_t* __restrict__ base_out)
 {
   if (meta) {  // node count on the device: the sum of the level sizes
     m = 0;
@@ -351,7 +353,7 @@ __global__ void __launch_bounds__(256)
   for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < m;
        i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
     finalize_node(nkey, nstart, ncount, nlevel, njob, cfirst, ccount, jobs, trees, i,
-                  internal_out, length_out, offset_out);
+                  internal_out, length_out, offset_out, base_out);
 }
 
 // --- join ----------------------------------------------------------------------------
@@ -828,7 +830,7 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   SortResult sorted = segmented_radix_sort_pairs(keys_a.get(), keys_b.get(), vals.get(),
                                                  vals_tmp.get(), n, key_bits, stiles.get(),
                                                  tile_tab.size(), ssegs.get(), seg_tab.size(), counts0.get(),
-                                                 out_pi, samples.get(), stream, "batch_sort");
+                                                 out_pi, samples.get(), false, stream, "batch_sort");
   vals_tmp.release();
 
   // 3. forest levels
@@ -876,7 +878,8 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   if (M > 0) {
     batch_finalize_kernel<<<grid_for(M, 256), 256, 0, stream>>>(
       nkey.get(), nstart.get(), ncount.get(), nlevel.get(), njob.get(), cfirst.get(), ccount.get(),
-      jobs.get(), trees.get(), M, nullptr, 0, f_internal.get(), f_length.get(), f_offset.get());
+      jobs.get(), trees.get(), M, nullptr, 0, f_internal.get(), f_length.get(), f_offset.get(),
+      nullptr);
     QPIP_LAUNCH_CHECK();
   }
   if (M == 0 || npg == 0) {
@@ -1143,13 +1146,13 @@ struct PlanGroup {
   PlanCaps cap;
   dbuf<JobDev> jobs;
   // tree
-  dbuf<uint32_t> keys_a, keys_b, vals, vals_tmp, counts0, samples;
+  dbuf<uint32_t> keys_a, keys_b, vals_tmp, counts0, samples;
   dbuf<SortTile> stiles;  // the segmented point sort's tile table
   dbuf<SortSegment> ssegs;
   uint64_t num_stiles = 0;
   dbuf<uint32_t> nkey, nstart, ncount, cfirst, ccount, njob, partials;
   dbuf<uint8_t> nlevel, f_internal;
-  dbuf<uint32_t> f_length, f_offset;
+  dbuf<uint32_t> f_length, f_offset, f_base;
   dbuf<uint4> split, staging;
   dbuf<JobTree> trees;
   dbuf<ForestMeta> fmeta;
@@ -1265,11 +1268,12 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
       xy, g.stiles.get(), g.jobs.get(), g.keys_a.get(), g.counts0.get());
     QPIP_LAUNCH_CHECK();
   }
-  // the last pass also writes the job-local point_indices output
+  // the last pass writes the job-local point_indices output, which the PIP test
+  // reads with each node's job base (point_base)
   SortResult sorted = segmented_radix_sort_pairs(
-    g.keys_a.get(), g.keys_b.get(), g.vals.get(), g.vals_tmp.get(), g.n, g.key_bits,
-    g.stiles.get(), g.num_stiles, g.ssegs.get(), g.J, g.counts0.get(), P.out_pi.get() + g.P0,
-    g.samples.get(), s, "batch_sort");
+    g.keys_a.get(), g.keys_b.get(), P.out_pi.get() + g.P0, g.vals_tmp.get(), g.n, g.key_bits,
+    g.stiles.get(), g.num_stiles, g.ssegs.get(), g.J, g.counts0.get(), nullptr, g.samples.get(),
+    true, s, "batch_sort");
   // 3. forest levels
   zero_struct(g.fmeta.get(), s);
   {
@@ -1303,7 +1307,7 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   batch_finalize_kernel<<<grid_for(g.node_cap, 256), 256, 0, s>>>(
     g.nkey.get(), g.nstart.get(), g.ncount.get(), g.nlevel.get(), g.njob.get(), g.cfirst.get(),
     g.ccount.get(), g.jobs.get(), g.trees.get(), 0, g.fmeta.get(), levels, g.f_internal.get(),
-    g.f_length.get(), g.f_offset.get());
+    g.f_length.get(), g.f_offset.get(), g.f_base.get());
   QPIP_LAUNCH_CHECK();
   if (!g.join) {
     plan_empty_kernel<<<grid_for(g.J, 256), 256, 0, s>>>(cnt, prev, g.fmeta.get(), job_first,
@@ -1363,7 +1367,8 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.length         = g.f_length.get();
   d.offset         = g.f_offset.get();
   d.num_nodes      = g.node_cap;
-  d.point_indices  = g.vals.get();
+  d.point_indices  = P.out_pi.get() + g.P0;
+  d.point_base     = g.f_base.get();
   d.num_points     = g.n;
   d.xy             = xy;
   d.poly_offsets   = reinterpret_cast<uint32_t const*>(P.poly_offsets + g.PO0);
@@ -1434,7 +1439,6 @@ void plan_group_setup(BatchPlan& P, PlanGroup& g, hipStream_t s)
   if (!g.tree) return;
   g.keys_a.reset(g.n);
   g.keys_b.reset(g.n);
-  g.vals.reset(g.n);
   g.vals_tmp.reset(g.n);
   g.samples.reset((g.n + 31) / 32);
   g.counts0.reset(g.num_stiles * kRadix);
@@ -1450,6 +1454,7 @@ void plan_group_setup(BatchPlan& P, PlanGroup& g, hipStream_t s)
   g.f_internal.reset(cap);
   g.f_length.reset(cap);
   g.f_offset.reset(cap);
+  g.f_base.reset(cap);
   g.split.reset(cap);
   g.staging.reset(static_cast<size_t>(g.J) * 64);
   g.trees.reset(g.J);

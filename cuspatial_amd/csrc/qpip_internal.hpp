@@ -395,8 +395,9 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
 // identity); the first pass's digit counts per tile (tiles x 256) come from the
 // producer of the keys; the last pass writes each key OR-ed with its segment's
 // `key_or` (the job number above the Morton key, for the batched forest) and,
-// when `values_local` is given, each value minus its segment's `begin` there;
-// `key_samples` (ceil(n / 32) entries, or null) receives every 32nd sorted key.
+// when `values_local` is given, each value minus its segment's `begin` there
+// (with `values_job_local`, values_out itself gets those); `key_samples`
+// (ceil(n / 32) entries, or null) receives every 32nd sorted key.
 struct SortTile {
   uint32_t start, count, segment, pad;
 };
@@ -416,6 +417,7 @@ SortResult segmented_radix_sort_pairs(uint32_t* keys_a,
                                       uint32_t* first_counts,
                                       uint32_t* values_local,
                                       uint32_t* key_samples,
+                                      bool values_job_local,
                                       hipStream_t stream,
                                       char const* tag);
 // Tiles of segments [begin[s], begin[s + 1]) of kSortTile positions (the last of a

@@ -89,6 +89,9 @@ struct PipArgs {
   uint32_t pairs_per_wave;  // longest segment of a leaf's pairs one wave takes
   uint64_t const* num_pairs_dev;  // non-null: the pair count lives on the device
   uint64_t chunk_cap;             // entries of masks / hit_counts / chunk_info
+  // non-null: point_indices hold indices local to each node's job, and
+  // point_base[node] is the job's first point (the batched forest)
+  uint32_t const* point_base = nullptr;
 };
 
 // Segments of at most max(m, pairs >> s) pairs (m = 2, s = 24 by default; both
@@ -323,6 +326,7 @@ __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(
   }
   uint32_t const len    = uniform(a.length[quad]);
   uint32_t const first  = uniform(a.offset[quad]);
+  uint32_t const pbase  = a.point_base ? uniform(a.point_base[quad]) : 0u;
   uint32_t const chunks = (len + kWave - 1) / kWave;
   uint64_t const slot0  = uniform(a.chunk_start[k0]);
   int const lane        = lane_id();
@@ -340,7 +344,7 @@ __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(
           err   = true;
           valid = false;
         } else {
-          uint32_t const idx = a.point_indices[pos];
+          uint32_t const idx = a.point_indices[pos] + pbase;
           if (idx >= a.num_points) {
             err   = true;
             valid = false;
@@ -613,7 +617,7 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
                d.num_points, nullptr, nullptr, d.poly_offsets, d.num_polys, d.num_rings,
                d.ring_offsets, d.ring_edges, d.edges, d.edges_b, d.chunk_start, d.chunk_cap,
                d.num_nodes, d.xy, reinterpret_cast<uintptr_t>(d.xy) % 16 == 0, d.pairs_per_wave,
-               n_pairs, d.chunk_cap};
+               n_pairs, d.chunk_cap, d.point_base};
   {
     KernelTimer kt("pip_test", stream);
     launch_pip_test(args, d.pair_cap, d.masks, d.hit_counts, d.chunk_info, flags, stream);

@@ -30,6 +30,9 @@ struct PipDevice {
   uint32_t const* point_indices;
   uint64_t num_points;
   double const* xy;  // interleaved points
+  // non-null: point_indices are local to each node's job; point_base[node] is
+  // the job's first point
+  uint32_t const* point_base;
   // polygons and their edge table
   uint32_t const* poly_offsets;
   uint64_t num_polys;

@@ -217,8 +217,9 @@ __device__ __forceinline__ uint64_t xcd_tile(uint64_t b, uint64_t tiles)
 // SEG: the tile is entry `tile` of a segment tile table (its own start and count,
 // never straddling two segments), and with `compose` the keys are written OR-ed with
 // the segment's key_or and, when `vals_local` is given, each value minus its
-// segment's first position is written there too; `key_samples`, when given,
-// receives the key written at every position that is a multiple of 32.
+// segment's first position is written there too (with `local_out`, in place of
+// the value itself); `key_samples`, when given, receives the key written at
+// every position that is a multiple of 32.
 template <int NB, bool SEG>
 __global__ void __launch_bounds__(kSortThreads)
   radix_scatter_kernel(uint32_t const* __restrict__ keys_in,
@@ -232,7 +233,8 @@ __global__ void __launch_bounds__(kSortThreads)
                        SortSegment const* __restrict__ segments,
                        bool compose,
                        uint32_t* __restrict__ vals_local,
-                       uint32_t* __restrict__ key_samples)
+                       uint32_t* __restrict__ key_samples,
+                       bool local_out)
 {
   constexpr int NW = kSortThreads / kWave;
   __shared__ uint32_t s_keys[kSortTile];
@@ -301,7 +303,7 @@ __global__ void __launch_bounds__(kSortThreads)
       uint32_t const v  = s_vals[i];
       uint32_t const ko = SEG ? (kk | key_or) : kk;
       keys_out[pos]      = ko;
-      vals_out[pos]      = v;
+      vals_out[pos]      = (SEG && local_out) ? v - seg_begin : v;
       if (SEG && vals_local) vals_local[pos] = v - seg_begin;
       if (SEG && key_samples && (pos & 31u) == 0) key_samples[pos >> 5] = ko;
     }
@@ -462,7 +464,7 @@ SortResult radix_sort_pairs(uint32_t* keys_a,
       KernelTimer kt(t_scatter.c_str(), stream);
 #define QPIP_SCATTER(NB)                                                                      \
   radix_scatter_kernel<NB, false><<<static_cast<unsigned>(tiles), kSortThreads, 0, stream>>>( \
-    ksrc, vsrc, kdst, vdst, n, shift, counts, nullptr, nullptr, false, nullptr, nullptr)
+    ksrc, vsrc, kdst, vdst, n, shift, counts, nullptr, nullptr, false, nullptr, nullptr, false)
       QPIP_DISPATCH_NB(nb, QPIP_SCATTER);
 #undef QPIP_SCATTER
       QPIP_LAUNCH_CHECK();
@@ -506,6 +508,7 @@ SortResult segmented_radix_sort_pairs(uint32_t* keys_a,
                                       uint32_t* first_counts,
                                       uint32_t* values_local,
                                       uint32_t* key_samples,
+                                      bool values_job_local,
                                       hipStream_t stream,
                                       char const* tag)
 {
@@ -545,7 +548,7 @@ SortResult segmented_radix_sort_pairs(uint32_t* keys_a,
 #define QPIP_SCATTER(NB)                                                                         \
   radix_scatter_kernel<NB, true><<<static_cast<unsigned>(num_tiles), kSortThreads, 0, stream>>>( \
     ksrc, vsrc, kdst, vdst, n, shift, first_counts, tiles, segments, last,               \
-    last ? values_local : nullptr, last ? key_samples : nullptr)
+    last ? values_local : nullptr, last ? key_samples : nullptr, last && values_job_local)
       QPIP_DISPATCH_NB(nb, QPIP_SCATTER);
 #undef QPIP_SCATTER
       QPIP_LAUNCH_CHECK();
