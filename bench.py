@@ -970,6 +970,9 @@ def main():
                     help="batched workload: run the autotuner on the batch before timing")
     ap.add_argument("--save-tuning", default=None, metavar="PATH",
                     help="with --autotune: record the result in the tuning table at PATH")
+    ap.add_argument("--tune", default="", metavar="NAME=VALUE,...",
+                    help="set library tuning parameters for the whole run (a saved tuning "
+                         "table still applies its own values to the batched workload)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the known "
                          "traffic that scales the HBM counters) and exit")
@@ -986,6 +989,12 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.tune:
+        from cuspatial_amd import autotune
+
+        for item in args.tune.split(","):
+            name, value = item.split("=")
+            autotune.set(name.strip(), int(value))
     if args.copy_calibration:
         src = torch.ones(args.copy_calibration << 20, dtype=torch.uint8, device=dev)
         dst = torch.empty_like(src)

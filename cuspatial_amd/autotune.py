@@ -32,6 +32,8 @@ PARAMETERS = {
     "pip_pairs_shift": ([10, 12, 14, 16, 18, 20, 24], ("pip_test",)),
     "pip_waves_per_block": ([2, 4, 8], ("pip_test",)),
     "pip_stage_edges": ([64, 128], ("pip_test",)),
+    "pip_xcd_run": ([1, 4, 16, 64], ("pip_test",)),
+    "pip_box_filter": ([0, 1], ("pip_test",)),
 }
 
 

@@ -1186,6 +1186,9 @@ struct BatchPlan {
   dbuf<Edge> edges;
   dbuf<EdgeB> edges_b;
   dbuf<uint32_t> ring_edges;
+  dbuf<LineEdge> lines;    // per edge: the PIP test's line form
+  dbuf<double> ring_amax;  // per ring
+  dbuf<PolyBox> poly_box;  // per polygon of the batch
   std::vector<std::unique_ptr<PlanGroup>> groups;
   dbuf<DeviceCounts> counts;  // one per group
   dbuf<int64_t> job_first;    // per job: first output row
@@ -1211,6 +1214,9 @@ struct BatchPlan {
     edges.release();
     edges_b.release();
     ring_edges.release();
+    lines.release();
+    ring_amax.release();
+    poly_box.release();
     counts.release();
     job_first.release();
     out_pi.release();
@@ -1378,6 +1384,9 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.ring_edges     = P.ring_edges.get();
   d.edges          = P.edges.get();
   d.edges_b        = P.edges_b.get();
+  d.lines          = P.lines.get();
+  d.ring_amax      = P.ring_amax.get();
+  d.poly_box       = P.poly_box.get() + g.PO0;
   d.pairs_per_wave = g.pairs_per_wave;
   d.chunk_start    = g.chunk_start.get();
   d.partials       = g.pip_partials.get();
@@ -1663,11 +1672,26 @@ extern "C" int qpip_batch_plan_create(int64_t num_jobs,
     P->edges.reset(std::max<int64_t>(num_poly_x, 1));
     P->edges_b.reset(std::max<int64_t>(num_poly_x, 1));
     P->ring_edges.reset(std::max<int64_t>(rings, 1));
-    if (rings > 0) {
-      build_edges_kernel<<<grid_for(static_cast<uint64_t>(rings), 256 / kWave), 256, 0, s>>>(
-        reinterpret_cast<uint32_t const*>(ring_offsets), rings, poly_x, poly_y, num_poly_x,
-        P->edges.get(), P->edges_b.get(), P->ring_edges.get());
-      QPIP_LAUNCH_CHECK();
+    P->lines.reset(std::max<int64_t>(num_poly_x, 1));
+    P->ring_amax.reset(std::max<int64_t>(rings, 1));
+    uint64_t const polys = num_poly_offsets > 1 ? static_cast<uint64_t>(num_poly_offsets - 1) : 0;
+    P->poly_box.reset(std::max<uint64_t>(polys, 1));
+    {
+      dbuf<PolyBox> ring_box(std::max<int64_t>(rings, 1));
+      if (rings > 0) {
+        build_edges_kernel<<<grid_for(static_cast<uint64_t>(rings), 256 / kWave), 256, 0, s>>>(
+          reinterpret_cast<uint32_t const*>(ring_offsets), rings, poly_x, poly_y, num_poly_x,
+          P->edges.get(), P->edges_b.get(), P->ring_edges.get(), P->lines.get(),
+          P->ring_amax.get(), ring_box.get());
+        QPIP_LAUNCH_CHECK();
+      }
+      if (polys > 0) {
+        poly_box_kernel<<<grid_for(polys, 256), 256, 0, s>>>(
+          reinterpret_cast<uint32_t const*>(poly_offsets), polys, static_cast<uint64_t>(rings),
+          ring_box.get(), P->poly_box.get());
+        QPIP_LAUNCH_CHECK();
+      }
+      QPIP_HIP(hipStreamSynchronize(s));  // ring_box goes back to the pool
     }
     for (auto const& [j0, j1] : job_groups(P->hj, P->jpo.data())) {
       auto g = std::make_unique<PlanGroup>();

@@ -105,9 +105,11 @@ enum class Tuning : int {
   PipMinPairsPerWave = 0,
   PipPairsShift      = 1,
   PipWavesPerBlock   = 2,
-  PipStageEdges      = 3
+  PipStageEdges      = 3,
+  PipXcdRun          = 4,
+  PipBoxFilter       = 5
 };
-constexpr int kNumTuning = 4;
+constexpr int kNumTuning = 6;
 int64_t tuning(Tuning t);
 
 // ----------------------------------------------------------------------------

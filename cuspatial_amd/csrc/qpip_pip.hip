@@ -19,13 +19,14 @@
 //      positions; a scan gives each pair its first chunk, and a map gives each
 //      chunk its pair (get_quad_and_local_point_indices.cuh:26-45 searches for it).
 //   3. Test: one wave per segment of a leaf's pairs (they are adjacent in pair
-//      order), up to 256 points (4 chunks) at a time. The lanes read
+//      order), up to 128 points (2 chunks) at a time. The lanes read
 //      consecutive point_indices entries (coalesced) and gather the coordinates
-//      once; each polygon's edges are wave-uniform and are staged through the
-//      wave's LDS slot in coalesced batches. The FP64 arithmetic is the
-//      reference's; the 4-ULP collinearity test is preceded by an exact
-//      floating-point filter that rejects pairs of products too far apart to be
-//      within 4 ULP. Each wave stores its 64-bit hit ballot and its popcount.
+//      once; points outside a polygon's vertex box are dropped and the rest
+//      packed into fewer lanes; each polygon's edges are wave-uniform and are
+//      staged through the wave's LDS slot in coalesced batches. A certain-sign
+//      form of the crossing test decides almost every (point, edge); the rest
+//      are retested with the reference's own FP64 arithmetic. Each wave stores
+//      its 64-bit hit ballot and its popcount.
 //   4. Scan of hit counts -> output positions; one host sync for the total.
 //   5. Write: each wave expands the ballots of 64 chunks into (polygon, point
 //      position) pairs.
@@ -81,6 +82,8 @@ struct PipArgs {
   uint32_t const* ring_edges;
   Edge const* edges;
   EdgeB const* edges_b;
+  LineEdge const* lines;   // per edge: the fast loop's form
+  double const* ring_amax;  // per ring: largest |vertex coordinate|
   uint32_t const* chunk_start;
   uint64_t num_chunks;
   uint64_t num_nodes;
@@ -92,6 +95,8 @@ struct PipArgs {
   // non-null: point_indices hold indices local to each node's job, and
   // point_base[node] is the job's first point (the batched forest)
   uint32_t const* point_base = nullptr;
+  // non-null: each polygon's vertex box, for the test's point filter
+  PolyBox const* poly_box = nullptr;
 };
 
 // Segments of at most max(m, pairs >> s) pairs (m = 2, s = 24 by default; both
@@ -111,66 +116,75 @@ inline uint32_t pairs_per_wave(uint64_t num_pairs)
 // One wave per segment of at most pairs_per_wave pairs that share a leaf (the join
 // orders pairs by leaf offset, so such pairs are adjacent; segments start at a run's
 // first pair and at every multiple of pairs_per_wave inside a run): the segment's
-// first wave gathers the leaf's points up to 4 chunks (256 points, U per lane) at a
-// time and tests them against every polygon of the segment; the other waves return
-// at once. Masks are stored per (pair, chunk) in pair order: chunk j of pair k is
+// first wave gathers the leaf's points up to kSlots chunks (128 points) at a time
+// and tests them against every polygon of the segment; the other waves return at
+// once. Masks are stored per (pair, chunk) in pair order: chunk j of pair k is
 // chunk_start[k] + j.
 //
 // Certain-sign filter. The reference decides each (point, edge) by the sign of
 // lhs - rhs, lhs = (px - ax) * rise and rhs = run * (py - ay), rounded as written,
-// and calls the point on the edge when the two are within 4 ULP. Here each edge
-// is evaluated in the line form t = px * rise - py * run + c, c = ay * run -
-// ax * rise (c per edge, two fmas per point), and t decides only when it is far
-// from zero. With u = 2^-53, s >= |px|, |py|, |ax|, |ay| for the wave's points and
-// the batch's edges, and n = |run| + |rise|, every term is at most M = s * n:
-//   |t - (L - R)| <= 7.1 u M  (c, the inner and the outer fma; L - R exact),
-//   |lhs - L| + |rhs - R| <= 2.01 u M  (two roundings each),
-//   4 ULP of lhs or rhs <= 8.01 u M, plus 2^-1070 where values are subnormal.
-// So |t| > tol = 2^-48 * s * n + 2^-1000 (over 31.9 u M after rounding) gives
+// and calls the point on the edge when the two are within 4 ULP. Here each edge is
+// evaluated in its line form (LineEdge, built with the edge table): the edge
+// scaled exactly by 2^-k, the power of two just above n = |run| + |rise|, so that
+// n' = |run'| + |rise'| <= 1, with c' = ay * run' - ax * rise' (one product, one
+// fma) and t = px * rise' - py * run' + c' (two fmas per point). t decides only
+// when it is far from zero. With u = 2^-53, s = (largest |px|, |py| of the wave's
+// points) + (largest |ax|, |ay| of the ring), L - R = (px - ax) rise - run (py - ay)
+// and L' - R' = (L - R) / 2^k (exact):
+//   |t - (L' - R')| <= 7.1 u s n'  (c', the inner and the outer fma),
+//   |lhs - L| + |rhs - R| <= 2.01 u s n  (two roundings each),
+//   4 ULP of lhs or rhs <= 8.01 u s n, plus 2^-1070 where values are subnormal,
+// and dividing the last two by 2^k >= n (>= 2^-69) gives at most 10.02 u s + 2^-1001.
+// So |t| > tol = 2^-48 * s + 2^-1000 (over 31.9 u s after rounding) gives
 // sign(lhs - rhs) = sign(t) and rules out the 4-ULP collinearity; every other
 // (point, edge), NaN and infinity included (their comparisons are false), marks the
 // point "suspect". Suspect points of the polygon are retested with the reference's
-// arithmetic in a second, rare pass. s and n are capped at 2^400 (beyond, tol is
-// infinite and every point is suspect), so no product overflows.
+// arithmetic in a second, rare pass. s is capped at 2^400 (beyond, tol is infinite
+// and every point is suspect) and n at [2^-70, 2^400] (outside, the line form's
+// c' is NaN), so no product overflows.
 //
 // Lane state is kept as wave-uniform 64-bit masks (one bit per lane and point
 // slot), so the loop's bookkeeping is a handful of scalar ops per edge.
-// Ring staging: a wave copies up to kStageEdges edges of the ring into its own LDS
+// Ring staging: a wave copies up to S line forms of the ring into its own LDS
 // slot with one coalesced vector load per lane (one memory latency per batch),
-// computes each edge's c and tol there (one edge per lane), then walks them with
-// broadcast LDS reads. Walking them from global memory took a dependent scalar
-// load (and its latency) every two edges, which dominated when the edge table is
-// not in L2: a small join right after the table is built, or the batched join's
-// 6.8 GB table.
+// then walks them with broadcast LDS reads. Walking them from global memory took
+// a dependent scalar load (and its latency) every two edges, which dominated when
+// the edge table is not in L2: a small join right after the table is built, or
+// the batched join's multi-GB table.
 // The depth S (edges per batch, 64 or 128) and the waves per workgroup are runtime
 // tuning parameters (pip_stage_edges, pip_waves_per_block).
 constexpr int kStageEdges = kWave;  // the default depth, and the bitmask kernel's
 
-// An edge as the test's fast loop reads it from LDS.
-struct alignas(32) LineEdge {
-  double ay, run, rise, c;
-};
+// A wave's staged batch of line forms (LineEdge, built with the edge table).
 template <int S>
 struct PipStage {
   LineEdge e[S];
-  double tol[S];
 };
 
-// Largest |coordinate| of the wave's points (0 for invalid slots; NaN ignored).
-__device__ __forceinline__ double wave_max_abs(double const (&px)[4], double const (&py)[4])
+// Point slots a wave holds at a time: lane l of slot u is point u * 64 + l of
+// the wave's current group (two slots keep the per-slot masks in few scalar
+// registers at eight waves per SIMD).
+constexpr int kSlots = 2;
+
+// Largest |coordinate| of the wave's points in slots [0, nu) (NaN ignored).
+__device__ __forceinline__ double wave_max_abs(double const (&px)[kSlots],
+                                               double const (&py)[kSlots], uint32_t nu)
 {
-  double m = 0.0;
+  double m = fmax(fabs(px[0]), fabs(py[0]));
 #pragma unroll
-  for (int u = 0; u < 4; ++u) m = fmax(m, fmax(fabs(px[u]), fabs(py[u])));
+  for (int u = 1; u < kSlots; ++u)
+    if (static_cast<uint32_t>(u) < nu) m = fmax(m, fmax(fabs(px[u]), fabs(py[u])));
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
   return m;
 }
 
 template <int U, int S = kStageEdges>
-__device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, double const (&px)[4],
-                                             double const (&py)[4], uint64_t const (&valid_m)[4],
-                                             double pmax, uint64_t (&hit_m)[4], bool& err,
+__device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly,
+                                             double const (&px)[kSlots],
+                                             double const (&py)[kSlots],
+                                             uint64_t const (&valid_m)[kSlots], double pmax,
+                                             uint64_t (&hit_m)[kSlots], bool& err,
                                              PipStage<S>* stage)
 {
   uint64_t inside_m[U], suspect_m[U];
@@ -189,17 +203,20 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
     uint32_t const e0 = uniform(a.ring_offsets[r]);
     uint32_t const ne = uniform(a.ring_edges[r]);
     if (ne == 0) continue;
-    Edge const* E = a.edges + e0;
+    LineEdge const* L = a.lines + e0;
+    // one tolerance for the ring: s bounds |px| + |ax| and |py| + |ay| (the wave's
+    // points, the ring's vertices); the line forms are scaled to n' <= 1
+    double const s   = pmax + a.ring_amax[r];
+    double const tol = s < 0x1p400 ? fma(s, 0x1p-48, 0x1p-1000) : __builtin_inf();
     // y0 of the first edge is its `b` (the ring's last vertex); afterwards the
     // previous edge's `a` (is_point_in_polygon.cuh:339, :375-376).
     double const by0 = a.edges_b[e0].by;
     uint64_t y0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) y0[u] = ballot(by0 > py[u]);
-    // The reference toggles on (y1 != y0) && ((lhs < rhs) != y1). Where the edge
-    // straddles the point (y1 != y0), y1 = (ay > by) = (rise < 0): the staging
-    // negates such edges' line form, so that t < 0 is (lhs < rhs) != y1 there.
-    auto edge = [&](LineEdge const& ed, double tol) {
+    // The reference toggles on (y1 != y0) && ((lhs < rhs) != y1); the line form is
+    // oriented so that t < 0 is (lhs < rhs) != y1 wherever the edge straddles.
+    auto edge = [&](LineEdge const& ed) {
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         double const t    = fma(px[u], ed.rise, fma(-py[u], ed.run, ed.c));
@@ -217,40 +234,19 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      double amax = 0.0, nrm[S / kWave];
-#pragma unroll
-      for (int i = 0; i < S; i += kWave) {
-        nrm[i / kWave] = 0.0;
-        if (static_cast<uint32_t>(i + lane) < n) {
-          Edge const ed = E[base + i + lane];
-          double const ax_rise = ed.ax * ed.rise;
-          double const c       = fma(ed.ay, ed.run, -ax_rise);
-          stage->e[i + lane]   = ed.rise < 0.0 ? LineEdge{ed.ay, -ed.run, -ed.rise, -c}
-                                               : LineEdge{ed.ay, ed.run, ed.rise, c};
-          amax                 = fmax(amax, fmax(fabs(ed.ax), fabs(ed.ay)));
-          nrm[i / kWave]       = fabs(ed.run) + fabs(ed.rise);
-        }
-      }
-#pragma unroll
-      for (int o = kWave / 2; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o, kWave));
-      double const s = pmax + amax;
 #pragma unroll
       for (int i = 0; i < S; i += kWave)
-        if (static_cast<uint32_t>(i + lane) < n)
-          stage->tol[i + lane] = s < 0x1p400 && nrm[i / kWave] < 0x1p400
-                                   ? fma(s * nrm[i / kWave], 0x1p-48, 0x1p-1000)
-                                   : __builtin_inf();
+        if (static_cast<uint32_t>(i + lane) < n) stage->e[i + lane] = L[base + i + lane];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       uint32_t e = 0;
-      if (U <= 2) {  // unrolled so that the loop's scalar bookkeeping is amortised
-        for (; e + 2 <= n; e += 2) {
-          edge(stage->e[e], stage->tol[e]);
-          edge(stage->e[e + 1], stage->tol[e + 1]);
-        }
+      // unrolled so that the loop's scalar bookkeeping is amortised
+      for (; e + 2 <= n; e += 2) {
+        edge(stage->e[e]);
+        edge(stage->e[e + 1]);
       }
-      for (; e < n; ++e) edge(stage->e[e], stage->tol[e]);
+      if (e < n) edge(stage->e[e]);
     }
   }
   // Suspect points: the reference's test over the whole polygon
@@ -291,6 +287,88 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly, do
   }
 }
 
+// Point filter. A point strictly outside its polygon's vertex box (PolyBox) is
+// outside the polygon in the reference's arithmetic too, so it needs no test:
+//  * py < miny or py > maxy: every vertex is on one side of the point's y, so no
+//    edge straddles it (y1 == y0 for every edge) and nothing toggles.
+//  * px < minx - d or px > maxx + d, d = 2^-48 (|px| + |py| + B) + 2^-300, B the
+//    box's largest |coordinate| (at most 2^400 and every non-zero rise at least
+//    2^-600 in magnitude, else the box is infinite and filters nothing), and
+//    |px| + |py| + B < 2^402: on a straddling edge (a, b) the reference's
+//    lhs = fl(fl(px - ax) * rise) and rhs = fl(run * fl(py - ay)) have fixed
+//    signs or, where the signs agree, |fl(px - ax)| >= |run| (1 + 3u) and
+//    |fl(py - ay)| <= |rise| by monotone rounding (the gap d exceeds 12 u |run|),
+//    so |lhs| > |rhs| with lhs a normal number (|px - ax| >= 2^-301,
+//    |rise| >= 2^-600). Hence left of the box every straddling edge toggles
+//    (an even number in a closed ring), right of it none does; the on-edge test
+//    needs px within an edge's x range, so it never fires either.
+//  NaN coordinates fail every comparison, so such points are always tested.
+__device__ __forceinline__ bool box_excludes(PolyBox const& b, double px, double py)
+{
+  double const bm = fmax(fmax(fabs(b.minx), fabs(b.maxx)), fmax(fabs(b.miny), fabs(b.maxy)));
+  double const s3 = fabs(px) + fabs(py) + bm;
+  double const d  = fma(s3, 0x1p-48, 0x1p-300);
+  bool const out_x = s3 < 0x1p402 && (px < b.minx - d || px > b.maxx + d);
+  return py < b.miny || py > b.maxy || out_x;
+}
+
+// Moves the kept points (keep_m, c of them) to slots 0 .. ceil(c / 64) - 1 in
+// keep order, through the wave's LDS staging area; test_m becomes their masks.
+__device__ __forceinline__ void pack_points(uint64_t const (&keep_m)[kSlots],
+                                            double (&px)[kSlots], double (&py)[kSlots],
+                                            uint32_t c, uint64_t (&test_m)[kSlots], double2* buf)
+{
+  int const lane = lane_id();
+  // the previous polygon's edges have been read by every lane
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint32_t pre = 0;
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) {
+    if ((keep_m[u] >> lane) & 1ull)
+      buf[pre + __popcll(keep_m[u] & lanemask_lt())] = make_double2(px[u], py[u]);
+    pre += static_cast<uint32_t>(__popcll(keep_m[u]));
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) {
+    uint32_t const slot = static_cast<uint32_t>(u * kWave + lane);
+    bool const in       = slot < c;
+    if (in) {
+      double2 const p = buf[slot];
+      px[u]           = p.x;
+      py[u]           = p.y;
+    }
+    test_m[u] = ballot(in);
+  }
+}
+
+// Hits of packed slots back to the points' own slots: kept point j (in keep
+// order) is bit j % 64 of packed slot j / 64.
+__device__ __forceinline__ void unpack_hits(uint64_t const (&keep_m)[kSlots],
+                                            uint64_t (&hit_m)[kSlots])
+{
+  int const lane = lane_id();
+  uint64_t h_in[kSlots];
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) h_in[u] = hit_m[u];
+  uint32_t pre = 0;
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) {
+    uint32_t const j = pre + static_cast<uint32_t>(__popcll(keep_m[u] & lanemask_lt()));
+    uint64_t h       = h_in[0];
+#pragma unroll
+    for (int v = 1; v < kSlots; ++v)
+      if (j >= static_cast<uint32_t>(v * kWave)) h = h_in[v];
+    bool const hit = ((keep_m[u] >> lane) & 1ull) && ((h >> (j % kWave)) & 1ull);
+    pre += static_cast<uint32_t>(__popcll(keep_m[u]));
+    hit_m[u] = ballot(hit);
+  }
+}
+
 // Pairs [base, base + waves of the grid) per launch: the host launches slices of
 // kPipPairsPerLaunch pairs, so a grid never exceeds 2^28 work-items.
 constexpr uint64_t kPipPairsPerLaunch = uint64_t{1} << 22;
@@ -299,15 +377,29 @@ constexpr uint64_t kPipPairsPerLaunch = uint64_t{1} << 22;
 // registers (spilling some of the per-polygon state to vector-register lanes,
 // none inside the edge loops); the extra wave hides more of the point gathers'
 // latency (batched join: 10.2 -> 10.0 ms per launch).
+// Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8), and each
+// XCD has its own L2. Adjacent pairs usually share a leaf, whose points a wave
+// finds in L2 only if the wave that gathered them ran on the same XCD. So block b
+// takes logical block (b / 8m) * 8m + (b % 8) * m + (b / 8) % m: each XCD works
+// through runs of m consecutive logical blocks (m = pip_xcd_run); blocks past the
+// last whole group of 8m keep their own index.
+__device__ __forceinline__ uint64_t xcd_block(uint64_t b, uint64_t grid, uint32_t m)
+{
+  uint64_t const g = 8ull * m;
+  if (m <= 1 || b >= grid / g * g) return b;
+  return (b / g) * g + (b % 8) * m + (b / 8) % m;
+}
+
 template <int W, int S>  // W waves per workgroup, S edges per staged batch
 __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(8)))
-  pip_test_kernel(PipArgs a, uint64_t base, uint64_t* __restrict__ masks,
+  pip_test_kernel(PipArgs a, uint64_t base, uint32_t xcd_run, uint64_t* __restrict__ masks,
                   uint32_t* __restrict__ hit_counts, uint2* __restrict__ chunk_info,
                   uint32_t* __restrict__ bad)
 {
   __shared__ PipStage<S> s_stage[W];
   PipStage<S>* const stage = &s_stage[threadIdx.x / kWave];
-  uint64_t const kk = base + (blockIdx.x * static_cast<uint64_t>(W) + threadIdx.x / kWave);
+  uint64_t const blk = xcd_block(blockIdx.x, gridDim.x, xcd_run);
+  uint64_t const kk  = base + (blk * static_cast<uint64_t>(W) + threadIdx.x / kWave);
   uint64_t const K  = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
   if (kk >= K) return;
   uint32_t const k0 = uniform(static_cast<uint32_t>(kk));
@@ -330,12 +422,15 @@ __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(
   uint32_t const chunks = (len + kWave - 1) / kWave;
   uint64_t const slot0  = uniform(a.chunk_start[k0]);
   int const lane        = lane_id();
-  for (uint32_t j0 = 0; j0 < chunks; j0 += 4) {
-    uint32_t const nu = chunks - j0 < 4 ? chunks - j0 : 4;
-    double px[4] = {0.0, 0.0, 0.0, 0.0}, py[4] = {0.0, 0.0, 0.0, 0.0};
-    uint64_t valid_m[4];
+  for (uint32_t j0 = 0; j0 < chunks; j0 += kSlots) {
+    uint32_t const nu = chunks - j0 < kSlots ? chunks - j0 : kSlots;
+    double px[kSlots], py[kSlots];
+    uint64_t valid_m[kSlots];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < kSlots; ++u) {
+      px[u] = py[u] = 0.0;
+      valid_m[u]    = 0;
+      if (static_cast<uint32_t>(u) >= nu) continue;  // wave-uniform
       uint32_t const local = (j0 + u) * kWave + lane;
       bool valid           = local < len;
       uint32_t const pos   = first + local;
@@ -365,22 +460,53 @@ __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(
       }
       valid_m[u] = ballot(valid);
     }
-    double const pmax = wave_max_abs(px, py);
+    double const pmax = wave_max_abs(px, py, nu);
     for (uint32_t k = k0; k < K; ++k) {
       if (k > k0 && (k % G == 0 || uniform(a.pair_quad[k]) != quad)) break;
       uint32_t const poly = uniform(a.pair_poly[k]);
-      uint64_t hit_m[4]   = {0, 0, 0, 0};
-      switch (nu) {
-        case 1: test_polygon<1, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
-        case 2: test_polygon<2, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
-        case 3: test_polygon<3, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
-        default: test_polygon<4, S>(a, poly, px, py, valid_m, pmax, hit_m, err, stage); break;
+      uint64_t hit_m[kSlots];
+      // the slots the test runs over, and their points
+      uint32_t ut = nu;
+      uint64_t test_m[kSlots], keep_m[kSlots];
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        hit_m[u]  = 0;
+        test_m[u] = valid_m[u];
+        keep_m[u] = 0;
       }
+      bool packed = false;
+      if (a.poly_box && poly < a.num_polys) {
+        PolyBox const fb = a.poly_box[poly];
+        uint32_t c       = 0;
+#pragma unroll
+        for (int u = 0; u < kSlots; ++u) {
+          if (valid_m[u])  // wave-uniform
+            keep_m[u] = ballot(((valid_m[u] >> lane) & 1ull) && !box_excludes(fb, px[u], py[u]));
+          c += static_cast<uint32_t>(__popcll(keep_m[u]));
+          test_m[u] = keep_m[u];
+        }
+        uint32_t const need = (c + kWave - 1) / kWave;
+        // Packing overwrites the leaf's points, so only the wave's last pair packs.
+        bool const last = k + 1 >= K || (k + 1) % G == 0 || uniform(a.pair_quad[k + 1]) != quad;
+        if (c == 0) {
+          ut = 0;
+        } else if (need < nu && last && c * sizeof(double2) <= sizeof(PipStage<S>)) {
+          pack_points(keep_m, px, py, c, test_m, reinterpret_cast<double2*>(stage));
+          ut     = need;
+          packed = true;
+        }
+      }
+      static_assert(kSlots == 2, "the switch below covers 1 and 2 slots");
+      if (ut == 1)
+        test_polygon<1, S>(a, poly, px, py, test_m, pmax, hit_m, err, stage);
+      else if (ut == 2)
+        test_polygon<2, S>(a, poly, px, py, test_m, pmax, hit_m, err, stage);
+      if (packed) unpack_hits(keep_m, hit_m);
       uint64_t const slot = slot0 + static_cast<uint64_t>(k - k0) * chunks + j0;
       if (slot + nu > a.chunk_cap) {
         cap_err = true;  // only with a caller-sized chunk capacity
       } else if (lane < static_cast<int>(nu)) {
-        uint64_t const m = lane == 0 ? hit_m[0] : lane == 1 ? hit_m[1] : lane == 2 ? hit_m[2] : hit_m[3];
+        uint64_t const m        = lane == 0 ? hit_m[0] : hit_m[1];
         masks[slot + lane]      = m;
         hit_counts[slot + lane] = static_cast<uint32_t>(__popcll(m));
         // what the write kernel needs: the polygon and the chunk's first point position
@@ -461,17 +587,20 @@ template <int W, int S>
 void launch_pip_test_ws(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
                         uint2* chunk_info, uint32_t* bad, hipStream_t stream)
 {
+  uint32_t const xcd_run = static_cast<uint32_t>(tuning(Tuning::PipXcdRun));
   for (uint64_t b = 0; b < pairs; b += kPipPairsPerLaunch) {
     uint64_t const n = std::min<uint64_t>(pairs - b, kPipPairsPerLaunch);
     pip_test_kernel<W, S><<<static_cast<unsigned>((n + W - 1) / W), W * kWave, 0, stream>>>(
-      a, b, masks, hit_counts, chunk_info, bad);
+      a, b, xcd_run, masks, hit_counts, chunk_info, bad);
     QPIP_LAUNCH_CHECK();
   }
 }
 
-void launch_pip_test(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
+void launch_pip_test(PipArgs const& args, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
                      uint2* chunk_info, uint32_t* bad, hipStream_t stream)
 {
+  PipArgs a = args;
+  if (tuning(Tuning::PipBoxFilter) == 0) a.poly_box = nullptr;
   int64_t const w = tuning(Tuning::PipWavesPerBlock);
   bool const deep = tuning(Tuning::PipStageEdges) > 64;
 #define QPIP_PIP_TEST(W)                                                                       \
@@ -498,8 +627,8 @@ __global__ void clamp_count_kernel(uint64_t const* in, uint64_t cap, uint64_t* o
 
 // --- brute force: every point against every polygon ------------------------------
 // point_in_polygon (detail/point_in_polygon.cuh:42-102): bit j of the result of point
-// i is point i inside polygon j, for at most 31 polygons. One wave per 256 points
-// (4 per lane); each polygon's edges are wave-uniform, as in the quadtree test.
+// i is point i inside polygon j, for at most 31 polygons. One wave per 128 points
+// (2 per lane); each polygon's edges are wave-uniform, as in the quadtree test.
 __device__ __forceinline__ void load_point(PipArgs const& a, uint64_t i, double& px, double& py)
 {
   if (a.xy) {
@@ -523,36 +652,39 @@ __global__ void __launch_bounds__(256)
   __shared__ PipStage<kStageEdges> s_stage[256 / kWave];
   PipStage<kStageEdges>* const stage = &s_stage[threadIdx.x / kWave];
   uint64_t const w  = static_cast<uint64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-  uint64_t const p0 = w * 4 * kWave;
+  uint64_t const p0 = w * kSlots * kWave;
   if (p0 >= n) return;
   int const lane    = lane_id();
   uint64_t const left = (n - p0 + kWave - 1) / kWave;
-  uint32_t const nu   = left < 4 ? static_cast<uint32_t>(left) : 4u;
-  double px[4] = {0.0, 0.0, 0.0, 0.0}, py[4] = {0.0, 0.0, 0.0, 0.0};
-  uint64_t valid_m[4];
+  uint32_t const nu   = left < kSlots ? static_cast<uint32_t>(left) : kSlots;
+  double px[kSlots], py[kSlots];
+  uint64_t valid_m[kSlots];
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < kSlots; ++u) {
     uint64_t const i = p0 + static_cast<uint64_t>(u) * kWave + lane;
     bool const valid = i < n;
+    px[u] = py[u] = 0.0;
     if (valid) load_point(a, i, px[u], py[u]);
     valid_m[u] = ballot(valid);
   }
-  double const pmax = wave_max_abs(px, py);
-  uint32_t bits[4] = {0, 0, 0, 0};
-  bool err         = false;
-  for (uint32_t j = 0; j < a.num_polys; ++j) {
-    uint64_t hit_m[4] = {0, 0, 0, 0};
-    switch (nu) {
-      case 1: test_polygon<1>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
-      case 2: test_polygon<2>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
-      case 3: test_polygon<3>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
-      default: test_polygon<4>(a, j, px, py, valid_m, pmax, hit_m, err, stage); break;
-    }
+  double const pmax = wave_max_abs(px, py, nu);
+  uint32_t bits[kSlots];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) bits[u] |= static_cast<uint32_t>((hit_m[u] >> lane) & 1ull) << j;
+  for (int u = 0; u < kSlots; ++u) bits[u] = 0;
+  bool err = false;
+  for (uint32_t j = 0; j < a.num_polys; ++j) {
+    uint64_t hit_m[kSlots];
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) hit_m[u] = 0;
+    if (nu == 1)
+      test_polygon<1>(a, j, px, py, valid_m, pmax, hit_m, err, stage);
+    else
+      test_polygon<2>(a, j, px, py, valid_m, pmax, hit_m, err, stage);
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) bits[u] |= static_cast<uint32_t>((hit_m[u] >> lane) & 1ull) << j;
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
+  for (int u = 0; u < kSlots; ++u) {
     uint64_t const i = p0 + static_cast<uint64_t>(u) * kWave + lane;
     if (i < n) out[i] = static_cast<int32_t>(bits[u]);
   }
@@ -615,9 +747,10 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
   QPIP_LAUNCH_CHECK();
   PipArgs args{d.pair_poly, d.pair_quad, d.pair_cap, d.length, d.offset, d.point_indices,
                d.num_points, nullptr, nullptr, d.poly_offsets, d.num_polys, d.num_rings,
-               d.ring_offsets, d.ring_edges, d.edges, d.edges_b, d.chunk_start, d.chunk_cap,
+               d.ring_offsets, d.ring_edges, d.edges, d.edges_b, d.lines, d.ring_amax,
+               d.chunk_start, d.chunk_cap,
                d.num_nodes, d.xy, reinterpret_cast<uintptr_t>(d.xy) % 16 == 0, d.pairs_per_wave,
-               n_pairs, d.chunk_cap, d.point_base};
+               n_pairs, d.chunk_cap, d.point_base, d.poly_box};
   {
     KernelTimer kt("pip_test", stream);
     launch_pip_test(args, d.pair_cap, d.masks, d.hit_counts, d.chunk_info, flags, stream);
@@ -704,12 +837,24 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
     dbuf<EdgeB> edges_b(std::max<int64_t>(num_poly_x, 1));
     dbuf<uint32_t> ring_edges(std::max<uint64_t>(num_rings, 1));
-    if (num_rings > 0) {
+    dbuf<LineEdge> lines(std::max<int64_t>(num_poly_x, 1));
+    dbuf<double> ring_amax(std::max<uint64_t>(num_rings, 1));
+    uint64_t const num_polys = static_cast<uint64_t>(num_poly_offsets - 1);
+    dbuf<PolyBox> ring_box(std::max<uint64_t>(num_rings, 1));
+    dbuf<PolyBox> poly_box(std::max<uint64_t>(num_polys, 1));
+    {
       KernelTimer kt("pip_edges", stream);
-      build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
-        ring_offsets, static_cast<int64_t>(num_rings), poly_x, poly_y, num_poly_x, edges.get(),
-        edges_b.get(), ring_edges.get());
-      QPIP_LAUNCH_CHECK();
+      if (num_rings > 0) {
+        build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
+          ring_offsets, static_cast<int64_t>(num_rings), poly_x, poly_y, num_poly_x, edges.get(),
+          edges_b.get(), ring_edges.get(), lines.get(), ring_amax.get(), ring_box.get());
+        QPIP_LAUNCH_CHECK();
+      }
+      if (num_polys > 0) {
+        poly_box_kernel<<<grid_for(num_polys, 256), 256, 0, stream>>>(
+          poly_offsets, num_polys, num_rings, ring_box.get(), poly_box.get());
+        QPIP_LAUNCH_CHECK();
+      }
     }
     // 2. chunks: pair k's first chunk
     dbuf<uint32_t> chunk_start(K + 1);
@@ -752,12 +897,12 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
         continue;
       }
       PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
-                   n_pts, x, y, poly_offsets,
-                   static_cast<uint64_t>(num_poly_offsets - 1), num_rings, ring_offsets,
-                   ring_edges.get(), edges.get(), edges_b.get(), chunk_start.get(), chunk_cap,
+                   n_pts, x, y, poly_offsets, num_polys, num_rings, ring_offsets,
+                   ring_edges.get(), edges.get(), edges_b.get(), lines.get(), ring_amax.get(),
+                   chunk_start.get(), chunk_cap,
                    static_cast<uint64_t>(num_nodes), xy,
                    xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
-                   pairs_per_wave(K), nullptr, chunk_cap};
+                   pairs_per_wave(K), nullptr, chunk_cap, nullptr, poly_box.get()};
       {
         KernelTimer kt("pip_test", stream);
         launch_pip_test(args, K, masks.get(), hit_counts.get(), chunk_info.get(), bad.get() + 1,
@@ -932,11 +1077,14 @@ int point_in_polygon_entry(const double* x,
     dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
     dbuf<EdgeB> edges_b(std::max<int64_t>(num_poly_x, 1));
     dbuf<uint32_t> ring_edges(std::max<uint64_t>(num_rings, 1));
+    dbuf<LineEdge> lines(pairwise ? 1 : std::max<int64_t>(num_poly_x, 1));
+    dbuf<double> ring_amax(pairwise ? 1 : std::max<uint64_t>(num_rings, 1));
     if (num_rings > 0) {
       KernelTimer kt("pip_edges", stream);
       build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
         reinterpret_cast<uint32_t const*>(ring_offsets), static_cast<int64_t>(num_rings), poly_x,
-        poly_y, num_poly_x, edges.get(), edges_b.get(), ring_edges.get());
+        poly_y, num_poly_x, edges.get(), edges_b.get(), ring_edges.get(),
+        pairwise ? nullptr : lines.get(), pairwise ? nullptr : ring_amax.get(), nullptr);
       QPIP_LAUNCH_CHECK();
     }
     PipArgs a{};
@@ -952,6 +1100,8 @@ int point_in_polygon_entry(const double* x,
     a.ring_edges   = ring_edges.get();
     a.edges        = edges.get();
     a.edges_b      = edges_b.get();
+    a.lines        = lines.get();
+    a.ring_amax    = ring_amax.get();
     OutputAllocator oa(mr, stream);
     if (pairwise) {
       uint8_t* out = oa.make<uint8_t>(n);
@@ -962,7 +1112,7 @@ int point_in_polygon_entry(const double* x,
     } else {
       int32_t* out = oa.make<int32_t>(n);
       KernelTimer kt("pip_bitmask", stream);
-      uint64_t const waves = (n + 4 * kWave - 1) / (4 * kWave);
+      uint64_t const waves = (n + kSlots * kWave - 1) / (kSlots * kWave);
       pip_bitmask_kernel<<<static_cast<unsigned>((waves + 3) / 4), 256, 0, stream>>>(a, n, out);
       QPIP_LAUNCH_CHECK();
       *result = out;
@@ -1026,15 +1176,20 @@ extern "C" int qpip_pairwise_point_in_polygon(const double* x,
 // --- allocation-free bitmask point-in-polygon ---------------------------------------
 // qpip_point_in_polygon_noalloc runs the kernels of qpip_point_in_polygon with the
 // edge table in `scratch`, a fixed device array of this file, instead of pool
-// memory: edges (32 B per vertex), their cold halves (16 B per vertex) and the
-// edge count of each ring (4 B per ring). 12.25 MiB per device, held by every
-// device the library runs on until the process ends.
+// memory: edges (32 B per vertex), their cold halves (16 B per vertex), their
+// line forms (32 B per vertex), and each ring's largest |coordinate| and edge
+// count (12 B per ring). 20.75 MiB per device, held by every device the library
+// runs on until the process ends.
 namespace {
 
 constexpr size_t kNoAllocEdgesOff  = 0;
 constexpr size_t kNoAllocEdgesBOff = kNoAllocEdgesOff + sizeof(qpip::Edge) * QPIP_NOALLOC_MAX_VERTICES;
-constexpr size_t kNoAllocRingsOff  = kNoAllocEdgesBOff + sizeof(qpip::EdgeB) * QPIP_NOALLOC_MAX_VERTICES;
+constexpr size_t kNoAllocLinesOff  = kNoAllocEdgesBOff + sizeof(qpip::EdgeB) * QPIP_NOALLOC_MAX_VERTICES;
+constexpr size_t kNoAllocAmaxOff   = kNoAllocLinesOff + sizeof(qpip::LineEdge) * QPIP_NOALLOC_MAX_VERTICES;
+constexpr size_t kNoAllocRingsOff  = kNoAllocAmaxOff + sizeof(double) * QPIP_NOALLOC_MAX_RINGS;
 constexpr size_t kNoAllocBytes     = kNoAllocRingsOff + sizeof(uint32_t) * QPIP_NOALLOC_MAX_RINGS;
+static_assert(kNoAllocLinesOff % alignof(qpip::LineEdge) == 0 && kNoAllocAmaxOff % 8 == 0,
+              "scratch layout");
 static_assert(kNoAllocEdgesBOff % alignof(qpip::EdgeB) == 0 && kNoAllocRingsOff % 4 == 0,
               "scratch carving keeps each table aligned");
 
@@ -1101,13 +1256,15 @@ extern "C" int qpip_point_in_polygon_noalloc(const double* x,
     Edge* const edges         = reinterpret_cast<Edge*>(base + kNoAllocEdgesOff);
     EdgeB* const edges_b      = reinterpret_cast<EdgeB*>(base + kNoAllocEdgesBOff);
     uint32_t* const ring_edges = reinterpret_cast<uint32_t*>(base + kNoAllocRingsOff);
+    LineEdge* const lines      = reinterpret_cast<LineEdge*>(base + kNoAllocLinesOff);
+    double* const ring_amax    = reinterpret_cast<double*>(base + kNoAllocAmaxOff);
     // Every ring's entry and every edge a ring keeps are written below before the
     // test reads them; the test reads nothing else of the scratch, so nothing a
     // previous call left there needs clearing.
     if (num_rings > 0) {
       build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
         reinterpret_cast<uint32_t const*>(ring_offsets), static_cast<int64_t>(num_rings), poly_x,
-        poly_y, num_poly_x, edges, edges_b, ring_edges);
+        poly_y, num_poly_x, edges, edges_b, ring_edges, lines, ring_amax, nullptr);
       QPIP_LAUNCH_CHECK();
     }
     uint64_t const n = static_cast<uint64_t>(num_x);
@@ -1122,7 +1279,9 @@ extern "C" int qpip_point_in_polygon_noalloc(const double* x,
     a.ring_edges   = ring_edges;
     a.edges        = edges;
     a.edges_b      = edges_b;
-    uint64_t const waves = (n + 4 * kWave - 1) / (4 * kWave);
+    a.lines        = lines;
+    a.ring_amax    = ring_amax;
+    uint64_t const waves = (n + kSlots * kWave - 1) / (kSlots * kWave);
     pip_bitmask_kernel<<<static_cast<unsigned>((waves + 3) / 4), 256, 0, stream>>>(a, n, result);
     QPIP_LAUNCH_CHECK();
     return QPIP_SUCCESS;

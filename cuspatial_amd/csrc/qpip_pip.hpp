@@ -41,6 +41,9 @@ struct PipDevice {
   uint32_t const* ring_edges;
   Edge const* edges;
   EdgeB const* edges_b;
+  LineEdge const* lines;    // per edge: the test's line form
+  double const* ring_amax;  // per ring: largest |vertex coordinate|
+  PolyBox const* poly_box;  // per polygon (null: the test filters no points)
   uint32_t pairs_per_wave;
   // workspace
   uint32_t* chunk_start;  // pair_cap + 1

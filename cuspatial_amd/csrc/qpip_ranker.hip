@@ -980,7 +980,7 @@ extern "C" int qpip_ranker_locate(const qpip_ranker* model,
       KernelTimer kt("ranker_edges", stream);
       build_edges_kernel<<<grid_for(static_cast<uint64_t>(rings), 256 / kWave), 256, 0, stream>>>(
         reinterpret_cast<uint32_t const*>(ring_offsets), rings, poly_x, poly_y, num_vertices,
-        edges.get(), edges_b.get(), ring_edges.get());
+        edges.get(), edges_b.get(), ring_edges.get(), nullptr, nullptr, nullptr);
       QPIP_LAUNCH_CHECK();
     }
     LocateArgs a{};

@@ -466,6 +466,8 @@ TuningParam g_tuning[kNumTuning] = {
   {"pip_pairs_shift", {24}, 0, 40},
   {"pip_waves_per_block", {2}, 1, 16},  // PIP test workgroup: 2, 4 or 8 waves (rounded up)
   {"pip_stage_edges", {64}, 1, 128},    // PIP ring staging depth: 64 or 128 edges
+  {"pip_xcd_run", {16}, 1, 256},        // PIP test workgroups per XCD run (1: dispatch order)
+  {"pip_box_filter", {1}, 0, 1},        // PIP test skips points outside the polygon's box
 };
 }  // namespace
 

@@ -91,6 +91,11 @@ QPIP_API void qpip_memory_info(size_t* live_bytes, size_t* cached_bytes);
  *                           round up to one of these; default 2)
  *   pip_stage_edges         ring edges a wave stages in LDS per batch: 64 or 128
  *                           (values above 64 select 128; default 64)
+ *   pip_xcd_run             consecutive workgroups of that test each XCD takes
+ *                           in turn (1: the dispatch order; default 16)
+ *   pip_box_filter          1: that test skips points outside a polygon's vertex
+ *                           box and packs the rest into fewer lanes; 0: it tests
+ *                           every candidate point (default 1)
  * Results do not depend on them, only the time. Errors: unknown name, value out
  * of range. */
 QPIP_API int qpip_tuning_set(const char* name, int64_t value);
@@ -344,7 +349,7 @@ QPIP_API int qpip_point_in_polygon(const double* x,
  * HIP graph capture, serving loops that must not call hipMalloc, harnesses that
  * forbid it). Same checks, test and output as qpip_point_in_polygon, but the caller
  * passes the output, `result` (num_x INT32 in device memory), and the edge table
- * lives in a fixed device array of the library (12.25 MiB per device, kept until
+ * lives in a fixed device array of the library (20.75 MiB per device, kept until
  * the process ends). The call allocates nothing, queues its two kernels on `stream`
  * and returns without waiting.
  * Limits: at most QPIP_NOALLOC_MAX_VERTICES polygon vertices and

@@ -352,6 +352,10 @@ def test_tuning_parameters(capi):
     assert autotune.get("pip_pairs_shift") == 24
     assert autotune.get("pip_waves_per_block") == 2
     assert autotune.get("pip_stage_edges") == 64
+    assert autotune.get("pip_xcd_run") == 16
+    assert autotune.get("pip_box_filter") == 1
+    with pytest.raises(RuntimeError, match="out of range"):
+        autotune.set("pip_box_filter", 2)
     with pytest.raises(RuntimeError, match="out of range"):
         autotune.set("pip_stage_edges", 129)
     autotune.set("pip_min_pairs_per_wave", 64)

@@ -127,11 +127,13 @@ def test_boundaries_match_bitmask(cs, oracle_mod):
     assert np.array_equal(bits, np.asarray(ob).astype(np.int64))
 
 
-@pytest.mark.parametrize("waves,stage,min_pairs", [(2, 64, 2), (8, 64, 2), (4, 128, 2),
-                                                   (8, 128, 1), (2, 128, 16)])
-def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs):
-    """The PIP test's workgroup size, staging depth and segment length change only
-    the time: every setting gives the oracle's pairs on the boundary scene."""
+@pytest.mark.parametrize("waves,stage,min_pairs,xcd_run,box", [
+    (2, 64, 2, 16, 1), (8, 64, 2, 1, 0), (4, 128, 2, 4, 1), (8, 128, 1, 64, 1),
+    (2, 128, 16, 1, 0), (2, 128, 1, 256, 1)])
+def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs, xcd_run, box):
+    """The PIP test's workgroup size, staging depth, segment length, XCD run and
+    box filter change only the time: every setting gives the oracle's pairs on the
+    boundary scene."""
     from cuspatial_amd import autotune
 
     x, y, polys = scene(21)
@@ -139,7 +141,8 @@ def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs)
     saved = autotune.current()
     try:
         autotune.apply({"pip_waves_per_block": waves, "pip_stage_edges": stage,
-                        "pip_min_pairs_per_wave": min_pairs})
+                        "pip_min_pairs_per_wave": min_pairs, "pip_xcd_run": xcd_run,
+                        "pip_box_filter": box})
         g = run_gpu(cs, x, y, bbox, 64.0 / 64, 6, 8, polys)
     finally:
         autotune.apply(saved)
@@ -188,3 +191,43 @@ def test_hole_vertical_lines_match_oracle(cs, oracle_mod):
         hit[g["pi"][g["point_index"]]] = True
         assert not hit[want_out].any()
         assert hit[want_in].all()
+
+
+@pytest.mark.parametrize("k", [-20, 0, 30])
+def test_box_filter_margins_match_oracle(cs, oracle_mod, k):
+    """Points around each polygon's vertex box (a few ULPs out, around the box
+    filter's margin and beyond, on all four sides), including points left of the
+    box that the reference's rounding calls inside (test_pip_filter.py): the hits
+    equal the oracle's bit for bit, and those points are among them."""
+    from test_pip_filter import _probe_points, _shapes, ring_box
+
+    rng = np.random.default_rng(40 + k)
+    s = 2.0 ** k
+    shapes = _shapes(rng, s)
+    part, ring, vx, vy, xs, ys = [0], [0], [], [], [], []
+    for sx, sy in shapes:
+        vx.extend(sx)
+        vy.extend(sy)
+        ring.append(ring[-1] + len(sx))
+        part.append(part[-1] + 1)
+        px, py = _probe_points(ring_box(sx, sy), sx, sy, rng, n_rand=40)
+        xs.extend(px)
+        ys.extend(py)
+    quirk = (np.nextafter(0.5 * s, 0.0), np.nextafter(0.5 * s, 0.0))
+    xs.append(quirk[0])
+    ys.append(quirk[1])
+    xs.extend(rng.uniform(-2 * s, 2 * s, 2000))
+    ys.extend(rng.uniform(-2 * s, 2 * s, 2000))
+    x, y = np.array(xs), np.array(ys)
+    keep = (x >= -2 * s) & (x < 2 * s) & (y >= -2 * s) & (y < 2 * s)
+    x, y = x[keep], y[keep]
+    polys = (np.array(part, np.int32), np.array(ring, np.int32), np.array(vx), np.array(vy))
+    bbox = (-2 * s, 2 * s, -2 * s, 2 * s)
+    for d, max_size in ((6, 32), (3, 256)):
+        scale = 4 * s / (1 << d)
+        g = run_gpu(cs, x, y, bbox, scale, d, max_size, polys)
+        o = run_oracle(oracle_mod, x, y, bbox, scale, d, max_size, polys)
+        assert_same(g, o)
+        hits = set(zip(g["polygon_index"].tolist(), g["pi"][g["point_index"]].tolist()))
+        qi = int(np.flatnonzero((x == quirk[0]) & (y == quirk[1]))[0])
+        assert (0, qi) in hits  # polygon 0 is the quirk triangle
