@@ -34,6 +34,7 @@ PARAMETERS = {
     "pip_stage_edges": ([64, 128], ("pip_test",)),
     "pip_xcd_run": ([1, 4, 16, 64], ("pip_test",)),
     "pip_box_filter": ([0, 1], ("pip_test",)),
+    "pip_polygon_group": ([0, 4, 16, 64], ("pip_test", "pip_order")),
 }
 
 

@@ -1164,8 +1164,11 @@ struct PlanGroup {
   dbuf<uint32_t> out_node, out_bbox, fa_node, fa_bbox, fb_node, fb_bbox;
   dbuf<uint32_t> ka, kb, va, vb, pb, pq, pair_counts, pair_sums;
   // point-in-polygon (capacity-sized)
-  dbuf<uint32_t> chunk_start, hit_counts, hit_poly, hit_pos;
+  dbuf<uint32_t> chunk_start, hit_poly, hit_pos;
   dbuf<uint64_t> pip_partials, pip_totals, masks, hit_start;
+  // the polygon-major PIP test's pair order (used when pip_polygon_group > 0)
+  dbuf<uint32_t> pm_count, pm_start, pm_cursor, pm_order;
+  dbuf<uint64_t> pm_partials, pm_total;
   dbuf<uint2> chunk_info;
   uint32_t pairs_per_wave = 16;
 };
@@ -1247,8 +1250,8 @@ void plan_alloc(PlanGroup& g)
   g.pair_sums.reset(sort_sums_elems(c.pairs));
   g.chunk_start.reset(c.pairs + 1);
   g.masks.reset(c.chunks);
-  g.hit_counts.reset(c.chunks);
   g.chunk_info.reset(c.chunks);
+  g.pm_order.reset(c.pairs);
   g.hit_start.reset(c.chunks);
   g.hit_poly.reset(c.hits);
   g.hit_pos.reset(c.hits);
@@ -1392,7 +1395,12 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.partials       = g.pip_partials.get();
   d.totals         = g.pip_totals.get();
   d.masks          = g.masks.get();
-  d.hit_counts     = g.hit_counts.get();
+  d.pm_count       = g.pm_count.get();
+  d.pm_start       = g.pm_start.get();
+  d.pm_cursor      = g.pm_cursor.get();
+  d.pm_order       = g.pm_order.get();
+  d.pm_partials    = g.pm_partials.get();
+  d.pm_total       = g.pm_total.get();
   d.chunk_info     = g.chunk_info.get();
   d.hit_start      = g.hit_start.get();
   d.chunk_cap      = g.cap.chunks;
@@ -1492,6 +1500,11 @@ void plan_group_setup(BatchPlan& P, PlanGroup& g, hipStream_t s)
   g.jmeta.reset(1);
   g.pip_partials.reset(kScanBlocks + 1);
   g.pip_totals.reset(2);
+  g.pm_count.reset(std::max<uint64_t>(g.npg, 1));
+  g.pm_start.reset(std::max<uint64_t>(g.npg, 1));
+  g.pm_cursor.reset(std::max<uint64_t>(g.npg, 1));
+  g.pm_partials.reset(kScanBlocks + 1);
+  g.pm_total.reset(1);
 }
 
 // True sizes of a group's last run (valid where no earlier stage overflowed).

@@ -107,9 +107,10 @@ enum class Tuning : int {
   PipWavesPerBlock   = 2,
   PipStageEdges      = 3,
   PipXcdRun          = 4,
-  PipBoxFilter       = 5
+  PipBoxFilter       = 5,
+  PipPolygonGroup    = 6
 };
-constexpr int kNumTuning = 6;
+constexpr int kNumTuning = 7;
 int64_t tuning(Tuning t);
 
 // ----------------------------------------------------------------------------

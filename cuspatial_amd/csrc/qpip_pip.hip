@@ -91,12 +91,20 @@ struct PipArgs {
   bool xy_vec;       // xy is 16-byte aligned
   uint32_t pairs_per_wave;  // longest segment of a leaf's pairs one wave takes
   uint64_t const* num_pairs_dev;  // non-null: the pair count lives on the device
-  uint64_t chunk_cap;             // entries of masks / hit_counts / chunk_info
+  uint64_t chunk_cap;             // entries of masks / chunk_info
   // non-null: point_indices hold indices local to each node's job, and
   // point_base[node] is the job's first point (the batched forest)
   uint32_t const* point_base = nullptr;
   // non-null: each polygon's vertex box, for the test's point filter
   PolyBox const* poly_box = nullptr;
+  // polygon-major order (pip_test_pm_kernel): the valid pairs grouped by
+  // polygon, polygon p's at pm_order[pm_start[p] .. + pm_count[p]), *pm_total of
+  // them; a wave takes up to pm_group of one polygon's pairs
+  uint32_t const* pm_order = nullptr;
+  uint32_t const* pm_start = nullptr;
+  uint32_t const* pm_count = nullptr;
+  uint64_t const* pm_total = nullptr;
+  uint32_t pm_group        = 0;
 };
 
 // Segments of at most max(m, pairs >> s) pairs (m = 2, s = 24 by default; both
@@ -393,8 +401,7 @@ __device__ __forceinline__ uint64_t xcd_block(uint64_t b, uint64_t grid, uint32_
 template <int W, int S>  // W waves per workgroup, S edges per staged batch
 __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(8)))
   pip_test_kernel(PipArgs a, uint64_t base, uint32_t xcd_run, uint64_t* __restrict__ masks,
-                  uint32_t* __restrict__ hit_counts, uint2* __restrict__ chunk_info,
-                  uint32_t* __restrict__ bad)
+                  uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
 {
   __shared__ PipStage<S> s_stage[W];
   PipStage<S>* const stage = &s_stage[threadIdx.x / kWave];
@@ -506,9 +513,7 @@ __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(
       if (slot + nu > a.chunk_cap) {
         cap_err = true;  // only with a caller-sized chunk capacity
       } else if (lane < static_cast<int>(nu)) {
-        uint64_t const m        = lane == 0 ? hit_m[0] : hit_m[1];
-        masks[slot + lane]      = m;
-        hit_counts[slot + lane] = static_cast<uint32_t>(__popcll(m));
+        masks[slot + lane]      = lane == 0 ? hit_m[0] : hit_m[1];
         // what the write kernel needs: the polygon and the chunk's first point position
         chunk_info[slot + lane] = make_uint2(poly, first + (j0 + lane) * kWave);
       }
@@ -518,11 +523,295 @@ __global__ void __launch_bounds__(W * kWave) __attribute__((amdgpu_waves_per_eu(
   if (cap_err && lane == 0) atomicOr(bad, 2u);
 }
 
+// --- 3b. polygon-major test ---------------------------------------------------------
+// The leaf-major test above runs the edge loop once per pair: with small leaves and
+// a box filter that keeps few points per pair (the batched workload: about 2.6
+// leaves per polygon, 17% of their points in the polygon's box), most lanes idle
+// and each pair pays the wave's fixed costs. Here the pairs are grouped by polygon
+// (a counting sort: pm_count_kernel, a scan, pm_scatter_kernel) and one wave takes
+// up to pm_group pairs of one polygon: it gathers each pair's leaf chunk by chunk,
+// keeps the points inside the polygon's box, and queues them in its lanes; every
+// 64 queued points (and at the end) it runs the polygon's edges over them
+// (test_polygon, one slot) and ORs each hit into its pair's chunk mask
+// (zeroed beforehand). The wave also writes its pairs' chunk records. The results
+// are the leaf-major kernel's: each (point, polygon) is decided the same way, and
+// the order of the pairs inside a polygon (set by atomics) changes nothing.
+__global__ void __launch_bounds__(256)
+  pm_count_kernel(PipArgs a, uint32_t* __restrict__ count, uint32_t* __restrict__ bad)
+{
+  uint64_t const K = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
+  for (uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < K;
+       k += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t const p = a.pair_poly[k], q = a.pair_quad[k];
+    if (p >= a.num_polys || q >= a.num_nodes) {
+      atomicOr(bad, 1u);
+      continue;
+    }
+    atomicAdd(&count[p], 1u);
+  }
+}
+
+struct PmScanOp {
+  uint32_t const* counts;
+  uint32_t* start;
+  uint32_t* cursor;
+  __device__ uint64_t count(uint64_t p) const { return counts[p]; }
+  __device__ void emit(uint64_t p, uint64_t prefix) const
+  {
+    start[p]  = static_cast<uint32_t>(prefix);
+    cursor[p] = static_cast<uint32_t>(prefix);
+  }
+};
+
+__global__ void __launch_bounds__(256)
+  pm_scatter_kernel(PipArgs a, uint32_t* __restrict__ cursor, uint32_t* __restrict__ order)
+{
+  uint64_t const K = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
+  for (uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < K;
+       k += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    uint32_t const p = a.pair_poly[k], q = a.pair_quad[k];
+    if (p >= a.num_polys || q >= a.num_nodes) continue;
+    order[atomicAdd(&cursor[p], 1u)] = static_cast<uint32_t>(k);
+  }
+}
+
+constexpr uint32_t kNoPoint = 0xFFFFFFFFu;  // a lane with no point to fetch
+
+// A wave's queue transfer slot: kept points land here in queue order.
+struct PmQueue {
+  double2 p[kWave];
+  uint64_t tag[kWave];  // chunk slot << 6 | lane of the point in its chunk
+};
+
+// No occupancy bound: held to the 7 or 8 waves per SIMD its LDS would admit, the
+// compiler spills vector registers to scratch; unbounded it keeps 5 waves without
+// spills, which measured faster (batched workload: 7.12 -> 6.34 ms).
+template <int W, int S>
+__global__ void __launch_bounds__(W * kWave)
+  pip_test_pm_kernel(PipArgs a, uint64_t base, uint32_t xcd_run, uint64_t* __restrict__ masks,
+                     uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
+{
+  __shared__ PipStage<S> s_stage[W];
+  __shared__ PmQueue s_queue[W];
+  PipStage<S>* const stage = &s_stage[threadIdx.x / kWave];
+  PmQueue* const tq        = &s_queue[threadIdx.x / kWave];
+  uint64_t const blk = xcd_block(blockIdx.x, gridDim.x, xcd_run);
+  uint64_t const i   = base + (blk * static_cast<uint64_t>(W) + threadIdx.x / kWave);
+  uint64_t const T   = *a.pm_total;
+  if (i >= T) return;
+  uint32_t const k0   = uniform(a.pm_order[i]);
+  uint32_t const poly = uniform(a.pair_poly[k0]);
+  uint32_t const s0   = uniform(a.pm_start[poly]);
+  uint32_t const n    = uniform(a.pm_count[poly]);
+  uint32_t const G    = a.pm_group;
+  uint32_t const off  = static_cast<uint32_t>(i) - s0;
+  if (off % G != 0) return;
+  uint32_t const end = s0 + (n - off < G ? n : off + G);
+  int const lane     = lane_id();
+  bool err = false, cap_err = false;
+  bool const filter = a.poly_box != nullptr;
+  PolyBox fb{};
+  if (filter) fb = a.poly_box[poly];
+  // the queue: lanes [0, qn) hold queued points
+  double qx[kSlots], qy[kSlots];
+  uint64_t qtag = 0;
+  uint32_t qn   = 0;
+  double amax_p = 0.0;  // largest |coordinate| of the points queued since the last flush
+#pragma unroll
+  for (int u = 0; u < kSlots; ++u) qx[u] = qy[u] = 0.0;
+  auto flush = [&]() {
+    uint64_t test_m[kSlots], hit_m[kSlots];
+#pragma unroll
+    for (int u = 0; u < kSlots; ++u) test_m[u] = hit_m[u] = 0;
+    test_m[0] = qn >= kWave ? ~0ull : ((1ull << qn) - 1ull);
+    double m = amax_p;
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
+    test_polygon<1, S>(a, poly, qx, qy, test_m, m, hit_m, err, stage);
+    if ((hit_m[0] >> lane) & 1ull)
+      atomicOr(reinterpret_cast<unsigned long long*>(&masks[qtag >> 6]), 1ull << (qtag & 63u));
+    qn     = 0;
+    amax_p = 0.0;
+  };
+  // the unit's pairs, one per lane (their loads in flight together): leaf range,
+  // job base and first chunk slot
+  uint32_t const m = end - static_cast<uint32_t>(i);
+  uint32_t u_len = 0, u_first = 0, u_base = 0, u_slo = 0, u_shi = 0;
+  auto rl = [](uint32_t v, uint32_t j) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(j)));
+  };
+  for (uint32_t j0 = 0; j0 < m; j0 += kWave) {
+    uint32_t const mj = m - j0 < kWave ? m - j0 : kWave;
+    if (static_cast<uint32_t>(lane) < mj) {
+      uint32_t const k    = a.pm_order[i + j0 + lane];
+      uint32_t const quad = a.pair_quad[k];
+      u_len               = a.length[quad];
+      u_first             = a.offset[quad];
+      u_base              = a.point_base ? a.point_base[quad] : 0u;
+      uint64_t const sl   = a.chunk_start[k];
+      u_slo               = static_cast<uint32_t>(sl);
+      u_shi               = static_cast<uint32_t>(sl >> 32);
+      // a pair whose chunks pass the capacity is not tested (the caller sees bit 2)
+      if (sl + (u_len + kWave - 1) / kWave > a.chunk_cap) {
+        cap_err = true;
+        u_len   = 0;
+      }
+    }
+    auto slot_of = [&](uint32_t j) {
+      return (static_cast<uint64_t>(rl(u_shi, j)) << 32) | rl(u_slo, j);
+    };
+    // the pairs' chunk records
+    for (uint32_t j = 0; j < mj; ++j) {
+      uint32_t const len   = rl(u_len, j);
+      uint32_t const first = rl(u_first, j);
+      uint64_t const slot0 = slot_of(j);
+      for (uint32_t c = lane; c * kWave < len; c += kWave)
+        chunk_info[slot0 + c] = make_uint2(poly, first + c * kWave);
+    }
+    // The items (pair j, chunk c) in order, software-pipelined: while item A is
+    // filtered and queued, the coordinates of item B and the point index of item
+    // C are in flight.
+    auto next = [&](uint32_t& j, uint32_t& c) {  // the item after (j, c); j == mj: none
+      if (j >= mj) return;
+      ++c;
+      while (j < mj && c * kWave >= rl(u_len, j)) {
+        ++j;
+        c = 0;
+      }
+    };
+    // Reads of another lane's metadata (readlane) happen here, with every lane
+    // active, before the per-lane branch: a value the compiler reloads from scratch
+    // inside a branch is restored in the branch's active lanes only.
+    auto fetch_index = [&](uint32_t j, uint32_t c, uint32_t& idx) {  // kNoPoint: none
+      idx = kNoPoint;
+      if (j >= mj) return;
+      uint32_t const len   = rl(u_len, j);
+      uint32_t const first = rl(u_first, j);
+      uint32_t const pb    = rl(u_base, j);
+      uint32_t const local = c * kWave + lane;
+      if (local < len) {
+        uint32_t const pos_p = first + local;
+        if (pos_p >= a.num_points) {
+          err = true;
+        } else {
+          uint32_t const v = a.point_indices[pos_p] + pb;
+          if (v >= a.num_points)
+            err = true;
+          else
+            idx = v;
+        }
+      }
+    };
+    auto fetch_point = [&](uint32_t idx, double& px, double& py) {
+      px = py = 0.0;
+      if (idx == kNoPoint) return;
+      if (a.xy) {
+        if (a.xy_vec) {
+          double2 const p = reinterpret_cast<double2 const*>(a.xy)[idx];
+          px              = p.x;
+          py              = p.y;
+        } else {
+          px = a.xy[2 * static_cast<uint64_t>(idx)];
+          py = a.xy[2 * static_cast<uint64_t>(idx) + 1];
+        }
+      } else {
+        px = a.x[idx];
+        py = a.y[idx];
+      }
+    };
+    // the first item: chunk 0 of the first pair with points
+    uint32_t ja = 0, ca = 0;
+    while (ja < mj && rl(u_len, ja) == 0) ++ja;
+    uint32_t jb = ja, cb = ca;
+    next(jb, cb);
+    uint32_t idx_a, idx_b;
+    fetch_index(ja, ca, idx_a);
+    fetch_index(jb, cb, idx_b);
+    double ax_, ay_;
+    fetch_point(idx_a, ax_, ay_);
+    while (ja < mj) {
+      uint32_t jc = jb, cc = cb;
+      next(jc, cc);
+      uint32_t idx_c;
+      fetch_index(jc, cc, idx_c);
+      double bx_, by_;
+      fetch_point(idx_b, bx_, by_);
+      // item A
+      bool keep = idx_a != kNoPoint;
+      if (keep && filter) keep = !box_excludes(fb, ax_, ay_);
+      uint64_t const km = ballot(keep);
+      if (km != 0) {
+        uint32_t const nk   = static_cast<uint32_t>(__popcll(km));
+        uint32_t const rank = qn + static_cast<uint32_t>(__popcll(km & lanemask_lt()));
+        uint64_t const tag  = ((slot_of(ja) + ca) << 6) | static_cast<uint64_t>(lane);
+        // first part: the kept points that fit in the queue
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (keep && rank < kWave) {
+          tq->p[rank]   = make_double2(ax_, ay_);
+          tq->tag[rank] = tag;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint32_t const fill = qn + nk < kWave ? qn + nk : kWave;
+        if (static_cast<uint32_t>(lane) >= qn && static_cast<uint32_t>(lane) < fill) {
+          double2 const q = tq->p[lane];
+          qx[0]           = q.x;
+          qy[0]           = q.y;
+          qtag            = tq->tag[lane];
+          amax_p          = fmax(amax_p, fmax(fabs(q.x), fabs(q.y)));
+        }
+        uint32_t const total = qn + nk;
+        qn                   = fill;
+        if (total >= kWave) {
+          flush();
+          // second part: the rest starts the next queue
+          uint32_t const rest = total - kWave;
+          if (rest > 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (keep && rank >= kWave) {
+              tq->p[rank - kWave]   = make_double2(ax_, ay_);
+              tq->tag[rank - kWave] = tag;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (static_cast<uint32_t>(lane) < rest) {
+              double2 const q = tq->p[lane];
+              qx[0]           = q.x;
+              qy[0]           = q.y;
+              qtag            = tq->tag[lane];
+              amax_p          = fmax(amax_p, fmax(fabs(q.x), fabs(q.y)));
+            }
+            qn = rest;
+          }
+        }
+      }
+      // shift the pipeline
+      ja    = jb;
+      ca    = cb;
+      jb    = jc;
+      cb    = cc;
+      idx_a = idx_b;
+      idx_b = idx_c;
+      ax_   = bx_;
+      ay_   = by_;
+    }
+  }
+  if (qn > 0) flush();
+  if (err) atomicOr(bad, 1u);
+  if (cap_err && lane == 0) atomicOr(bad, 2u);
+}
+
 // --- 4. scan of hit counts ----------------------------------------------------------
 struct HitScanOp {
-  uint32_t const* hit_counts;
+  uint64_t const* masks;
   uint64_t* hit_start;
-  __device__ uint64_t count(uint64_t c) const { return hit_counts[c]; }
+  __device__ uint64_t count(uint64_t c) const { return static_cast<uint64_t>(__popcll(masks[c])); }
   __device__ void emit(uint64_t c, uint64_t prefix) const { hit_start[c] = prefix; }
 };
 
@@ -566,13 +855,16 @@ __global__ void __launch_bounds__(256)
     while (pending) {  // wave-uniform
       int const j = __builtin_ctzll(pending);
       pending &= pending - 1;
-      uint64_t const m = readlane64(mask, j);
+      // every cross-lane read before the per-lane branch (all lanes active)
+      uint64_t const m     = readlane64(mask, j);
+      uint64_t const s0    = readlane64(start, j);
+      uint32_t const poly  = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(info.x), j));
+      uint32_t const first = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(info.y), j));
       if ((m >> l) & 1ull) {
-        uint64_t const o = readlane64(start, j) + static_cast<uint64_t>(__popcll(m & lt));
+        uint64_t const o = s0 + static_cast<uint64_t>(__popcll(m & lt));
         if (o < hit_cap) {
-          out_poly[o]  = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(info.x), j));
-          out_point[o] = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(info.y), j)) +
-                         static_cast<uint32_t>(l);
+          out_poly[o]  = poly;
+          out_point[o] = first + static_cast<uint32_t>(l);
         } else {
           atomicOr(bad, 2u);
         }
@@ -581,31 +873,57 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Launches pip_test_kernel over pairs [0, pairs) in slices, with the tuned
-// workgroup size (2, 4 or 8 waves) and staging depth (64 or 128 edges).
+// Launches the test over pairs [0, pairs) in slices, with the tuned workgroup
+// size (2, 4 or 8 waves) and staging depth (64 or 128 edges): the leaf-major
+// kernel, or with pip_polygon_group > 0 (and `pm` given) the polygon-major one.
 template <int W, int S>
-void launch_pip_test_ws(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
-                        uint2* chunk_info, uint32_t* bad, hipStream_t stream)
+void launch_pip_test_ws(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint2* chunk_info,
+                        uint32_t* bad, bool pm, hipStream_t stream)
 {
   uint32_t const xcd_run = static_cast<uint32_t>(tuning(Tuning::PipXcdRun));
   for (uint64_t b = 0; b < pairs; b += kPipPairsPerLaunch) {
     uint64_t const n = std::min<uint64_t>(pairs - b, kPipPairsPerLaunch);
-    pip_test_kernel<W, S><<<static_cast<unsigned>((n + W - 1) / W), W * kWave, 0, stream>>>(
-      a, b, xcd_run, masks, hit_counts, chunk_info, bad);
+    unsigned const grid = static_cast<unsigned>((n + W - 1) / W);
+    if (pm)
+      pip_test_pm_kernel<W, S><<<grid, W * kWave, 0, stream>>>(a, b, xcd_run, masks, chunk_info, bad);
+    else
+      pip_test_kernel<W, S><<<grid, W * kWave, 0, stream>>>(a, b, xcd_run, masks, chunk_info, bad);
     QPIP_LAUNCH_CHECK();
   }
 }
 
-void launch_pip_test(PipArgs const& args, uint64_t pairs, uint64_t* masks, uint32_t* hit_counts,
-                     uint2* chunk_info, uint32_t* bad, hipStream_t stream)
+void launch_pip_test(PipArgs const& args, uint64_t pairs, uint64_t* masks, uint2* chunk_info,
+                     uint32_t* bad, PmBuffers const* pmb, hipStream_t stream)
 {
   PipArgs a = args;
   if (tuning(Tuning::PipBoxFilter) == 0) a.poly_box = nullptr;
+  int64_t const group = tuning(Tuning::PipPolygonGroup);
+  bool const pm       = group > 0 && pmb != nullptr && a.num_polys > 0;
+  if (pm) {
+    // polygon-major order of the valid pairs; masks start at zero (hits are ORed in)
+    KernelTimer kt("pip_order", stream);
+    fill_u32(pmb->count, a.num_polys, 0u, stream);
+    fill_u32(reinterpret_cast<uint32_t*>(masks), 2 * a.chunk_cap, 0u, stream);
+    pm_count_kernel<<<grid_for(pairs, 256, 4096), 256, 0, stream>>>(a, pmb->count, bad);
+    QPIP_LAUNCH_CHECK();
+    PmScanOp op{pmb->count, pmb->start, pmb->cursor};
+    count_scan_emit<PmScanOp, uint64_t, uint64_t>(op, nullptr, a.num_polys, pmb->partials,
+                                                  pmb->total, stream,
+                                                  scan_blocks_for(a.num_polys), "pip_order_scan");
+    pm_scatter_kernel<<<grid_for(pairs, 256, 4096), 256, 0, stream>>>(a, pmb->cursor, pmb->order);
+    QPIP_LAUNCH_CHECK();
+    a.pm_order = pmb->order;
+    a.pm_start = pmb->start;
+    a.pm_count = pmb->count;
+    a.pm_total = pmb->total;
+    a.pm_group = static_cast<uint32_t>(group);
+  }
+  KernelTimer kt("pip_test", stream);
   int64_t const w = tuning(Tuning::PipWavesPerBlock);
   bool const deep = tuning(Tuning::PipStageEdges) > 64;
-#define QPIP_PIP_TEST(W)                                                                       \
-  (deep ? launch_pip_test_ws<W, 2 * kWave>(a, pairs, masks, hit_counts, chunk_info, bad, stream) \
-        : launch_pip_test_ws<W, kWave>(a, pairs, masks, hit_counts, chunk_info, bad, stream))
+#define QPIP_PIP_TEST(W)                                                                    \
+  (deep ? launch_pip_test_ws<W, 2 * kWave>(a, pairs, masks, chunk_info, bad, pm, stream) \
+        : launch_pip_test_ws<W, kWave>(a, pairs, masks, chunk_info, bad, pm, stream))
   if (w <= 2)
     QPIP_PIP_TEST(2);
   else if (w <= 4)
@@ -751,11 +1069,10 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
                d.chunk_start, d.chunk_cap,
                d.num_nodes, d.xy, reinterpret_cast<uintptr_t>(d.xy) % 16 == 0, d.pairs_per_wave,
                n_pairs, d.chunk_cap, d.point_base, d.poly_box};
-  {
-    KernelTimer kt("pip_test", stream);
-    launch_pip_test(args, d.pair_cap, d.masks, d.hit_counts, d.chunk_info, flags, stream);
-  }
-  HitScanOp hop{d.hit_counts, d.hit_start};
+  PmBuffers const pmb{d.pm_count, d.pm_start, d.pm_cursor, d.pm_order, d.pm_partials, d.pm_total};
+  launch_pip_test(args, d.pair_cap, d.masks, d.chunk_info, flags, d.pm_order ? &pmb : nullptr,
+                  stream);
+  HitScanOp hop{d.masks, d.hit_start};
   count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, n_chunks, 0, d.partials, &d.totals[1],
                                                  stream, kScanBlocks, "pip_scan");
   clamp_count_kernel<<<1, 64, 0, stream>>>(&d.totals[1], d.hit_cap, &d.counts->hits, flags);
@@ -872,20 +1189,24 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     uint64_t chunk_cap   = K + (n_pts >> 4) + 4096;
     bool exact           = false;
     dbuf<uint64_t> masks, hit_start;
-    dbuf<uint32_t> hit_counts;
     dbuf<uint2> chunk_info;
+    // the polygon-major test's order (pip_polygon_group > 0)
+    bool const pm_on = tuning(Tuning::PipPolygonGroup) > 0 && num_polys > 0;
+    dbuf<uint32_t> pm_count(pm_on ? num_polys : 1), pm_start(pm_on ? num_polys : 1),
+      pm_cursor(pm_on ? num_polys : 1), pm_order(pm_on ? K : 1);
+    dbuf<uint64_t> pm_partials(kScanBlocks + 1), pm_total(1);
+    PmBuffers const pmb{pm_count.get(), pm_start.get(), pm_cursor.get(), pm_order.get(),
+                        pm_partials.get(), pm_total.get()};
     uint64_t h_tot[2] = {0, 0};
     uint32_t h_bad[2] = {0, 0};
     for (;;) {
       try {
         masks.reset(chunk_cap);
-        hit_counts.reset(chunk_cap);
         chunk_info.reset(chunk_cap);
         hit_start.reset(chunk_cap);
       } catch (oom_error const&) {
         if (exact) throw;
         masks.release();
-        hit_counts.release();
         chunk_info.release();
         hit_start.release();
         QPIP_HIP(hipMemcpyAsync(h_tot, totals.get(), sizeof(uint64_t), hipMemcpyDeviceToHost,
@@ -903,17 +1224,14 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                    static_cast<uint64_t>(num_nodes), xy,
                    xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
                    pairs_per_wave(K), nullptr, chunk_cap, nullptr, poly_box.get()};
-      {
-        KernelTimer kt("pip_test", stream);
-        launch_pip_test(args, K, masks.get(), hit_counts.get(), chunk_info.get(), bad.get() + 1,
-                        stream);
-      }
+      launch_pip_test(args, K, masks.get(), chunk_info.get(), bad.get() + 1,
+                      pm_on ? &pmb : nullptr, stream);
       // chunks past the capacity were not tested: the scan covers those that were
       dbuf<uint64_t> n_chunks(1);
       clamp_count_kernel<<<1, 64, 0, stream>>>(totals.get(), chunk_cap, n_chunks.get(),
                                                bad.get() + 1);
       QPIP_LAUNCH_CHECK();
-      HitScanOp hop{hit_counts.get(), hit_start.get()};
+      HitScanOp hop{masks.get(), hit_start.get()};
       count_scan_emit<HitScanOp, uint64_t, uint64_t>(hop, n_chunks.get(), 0, partials.get(),
                                                      totals.get() + 1, stream, kScanBlocks,
                                                      "pip_scan");

@@ -50,15 +50,31 @@ struct PipDevice {
   uint64_t* partials;     // kScanBlocks + 1
   uint64_t* totals;       // 2: chunk and hit totals before clamping
   uint64_t* masks;        // chunk_cap
-  uint32_t* hit_counts;   // chunk_cap
   uint2* chunk_info;      // chunk_cap
   uint64_t* hit_start;    // chunk_cap
   uint64_t chunk_cap;
+  // the polygon-major test's order (null pm_order: the leaf-major test)
+  uint32_t* pm_count;     // num_polys
+  uint32_t* pm_start;     // num_polys
+  uint32_t* pm_cursor;    // num_polys
+  uint32_t* pm_order;     // pair_cap
+  uint64_t* pm_partials;  // kScanBlocks + 1
+  uint64_t* pm_total;     // 1
   // output: (polygon, point position) hits, at most hit_cap
   uint32_t* out_poly;
   uint32_t* out_pos;
   uint64_t hit_cap;
   DeviceCounts* counts;
+};
+
+// Buffers of the polygon-major test's pair order.
+struct PmBuffers {
+  uint32_t* count;
+  uint32_t* start;
+  uint32_t* cursor;
+  uint32_t* order;
+  uint64_t* partials;
+  uint64_t* total;
 };
 
 // Enqueues the stage on `stream`: chunks, test, scan, write.

@@ -468,6 +468,7 @@ TuningParam g_tuning[kNumTuning] = {
   {"pip_stage_edges", {64}, 1, 128},    // PIP ring staging depth: 64 or 128 edges
   {"pip_xcd_run", {16}, 1, 256},        // PIP test workgroups per XCD run (1: dispatch order)
   {"pip_box_filter", {1}, 0, 1},        // PIP test skips points outside the polygon's box
+  {"pip_polygon_group", {0}, 0, 4096},  // > 0: polygon-major PIP test, pairs per wave
 };
 }  // namespace
 

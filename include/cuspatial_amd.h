@@ -96,6 +96,10 @@ QPIP_API void qpip_memory_info(size_t* live_bytes, size_t* cached_bytes);
  *   pip_box_filter          1: that test skips points outside a polygon's vertex
  *                           box and packs the rest into fewer lanes; 0: it tests
  *                           every candidate point (default 1)
+ *   pip_polygon_group       0: the test walks the pairs leaf by leaf (default);
+ *                           n > 0: it groups them by polygon, one wave taking up
+ *                           to n pairs of one polygon and packing their kept
+ *                           points from several leaves into full waves
  * Results do not depend on them, only the time. Errors: unknown name, value out
  * of range. */
 QPIP_API int qpip_tuning_set(const char* name, int64_t value);

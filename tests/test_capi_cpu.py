@@ -354,6 +354,7 @@ def test_tuning_parameters(capi):
     assert autotune.get("pip_stage_edges") == 64
     assert autotune.get("pip_xcd_run") == 16
     assert autotune.get("pip_box_filter") == 1
+    assert autotune.get("pip_polygon_group") == 0
     with pytest.raises(RuntimeError, match="out of range"):
         autotune.set("pip_box_filter", 2)
     with pytest.raises(RuntimeError, match="out of range"):

@@ -163,6 +163,10 @@ def test_plan_under_each_saved_tuning_matches_eager(cs, tmp_path):
          "pip_stage_edges": 128, "pip_xcd_run": 1},
         {"pip_min_pairs_per_wave": 64, "pip_pairs_shift": 16, "pip_waves_per_block": 4,
          "pip_stage_edges": 64, "pip_xcd_run": 4},
+        {"pip_min_pairs_per_wave": 1, "pip_pairs_shift": 24, "pip_waves_per_block": 2,
+         "pip_stage_edges": 128, "pip_polygon_group": 16},
+        {"pip_min_pairs_per_wave": 1, "pip_pairs_shift": 24, "pip_waves_per_block": 8,
+         "pip_stage_edges": 64, "pip_polygon_group": 2, "pip_box_filter": 0},
     ]
     if __import__("os").path.exists(autotune.DEFAULT_TABLE):
         configs += [dict(b["config"]) for b in autotune.TuningTable(autotune.DEFAULT_TABLE)

@@ -127,13 +127,15 @@ def test_boundaries_match_bitmask(cs, oracle_mod):
     assert np.array_equal(bits, np.asarray(ob).astype(np.int64))
 
 
-@pytest.mark.parametrize("waves,stage,min_pairs,xcd_run,box", [
-    (2, 64, 2, 16, 1), (8, 64, 2, 1, 0), (4, 128, 2, 4, 1), (8, 128, 1, 64, 1),
-    (2, 128, 16, 1, 0), (2, 128, 1, 256, 1)])
-def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs, xcd_run, box):
-    """The PIP test's workgroup size, staging depth, segment length, XCD run and
-    box filter change only the time: every setting gives the oracle's pairs on the
-    boundary scene."""
+@pytest.mark.parametrize("waves,stage,min_pairs,xcd_run,box,group", [
+    (2, 64, 2, 16, 1, 0), (8, 64, 2, 1, 0, 0), (4, 128, 2, 4, 1, 0), (8, 128, 1, 64, 1, 0),
+    (2, 128, 16, 1, 0, 0), (2, 128, 1, 256, 1, 0), (2, 128, 1, 16, 1, 1), (2, 64, 1, 16, 1, 3),
+    (4, 128, 1, 16, 0, 16), (8, 64, 1, 16, 1, 4096)])
+def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs, xcd_run, box,
+                                        group):
+    """The PIP test's workgroup size, staging depth, segment length, XCD run, box
+    filter and polygon-major grouping change only the time: every setting gives
+    the oracle's pairs on the boundary scene."""
     from cuspatial_amd import autotune
 
     x, y, polys = scene(21)
@@ -142,7 +144,7 @@ def test_tuning_does_not_change_results(cs, oracle_mod, waves, stage, min_pairs,
     try:
         autotune.apply({"pip_waves_per_block": waves, "pip_stage_edges": stage,
                         "pip_min_pairs_per_wave": min_pairs, "pip_xcd_run": xcd_run,
-                        "pip_box_filter": box})
+                        "pip_box_filter": box, "pip_polygon_group": group})
         g = run_gpu(cs, x, y, bbox, 64.0 / 64, 6, 8, polys)
     finally:
         autotune.apply(saved)
