@@ -1167,8 +1167,8 @@ struct PlanGroup {
   dbuf<uint32_t> chunk_start, hit_poly, hit_pos;
   dbuf<uint64_t> pip_partials, pip_totals, masks, hit_start;
   // the polygon-major PIP test's pair order (used when pip_polygon_group > 0)
-  dbuf<uint32_t> pm_count, pm_start, pm_cursor, pm_order;
-  dbuf<uint64_t> pm_partials, pm_total;
+  dbuf<uint32_t> pm_count, pm_start, pm_cursor, pm_order, pm_units;
+  dbuf<uint64_t> pm_partials, pm_total, pm_num_units;
   dbuf<uint2> chunk_info;
   uint32_t pairs_per_wave = 16;
 };
@@ -1252,6 +1252,7 @@ void plan_alloc(PlanGroup& g)
   g.masks.reset(c.chunks);
   g.chunk_info.reset(c.chunks);
   g.pm_order.reset(c.pairs);
+  g.pm_units.reset(c.pairs);
   g.hit_start.reset(c.chunks);
   g.hit_poly.reset(c.hits);
   g.hit_pos.reset(c.hits);
@@ -1399,8 +1400,10 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.pm_start       = g.pm_start.get();
   d.pm_cursor      = g.pm_cursor.get();
   d.pm_order       = g.pm_order.get();
+  d.pm_units       = g.pm_units.get();
   d.pm_partials    = g.pm_partials.get();
   d.pm_total       = g.pm_total.get();
+  d.pm_num_units   = g.pm_num_units.get();
   d.chunk_info     = g.chunk_info.get();
   d.hit_start      = g.hit_start.get();
   d.chunk_cap      = g.cap.chunks;
@@ -1505,6 +1508,7 @@ void plan_group_setup(BatchPlan& P, PlanGroup& g, hipStream_t s)
   g.pm_cursor.reset(std::max<uint64_t>(g.npg, 1));
   g.pm_partials.reset(kScanBlocks + 1);
   g.pm_total.reset(1);
+  g.pm_num_units.reset(1);
 }
 
 // True sizes of a group's last run (valid where no earlier stage overflowed).

@@ -98,12 +98,14 @@ struct PipArgs {
   // non-null: each polygon's vertex box, for the test's point filter
   PolyBox const* poly_box = nullptr;
   // polygon-major order (pip_test_pm_kernel): the valid pairs grouped by
-  // polygon, polygon p's at pm_order[pm_start[p] .. + pm_count[p]), *pm_total of
-  // them; a wave takes up to pm_group of one polygon's pairs
+  // polygon, polygon p's at pm_order[pm_start[p] .. + pm_count[p]); a unit is up
+  // to pm_group of one polygon's pairs, *pm_num_units of them, listed by their
+  // first position in pm_order
   uint32_t const* pm_order = nullptr;
   uint32_t const* pm_start = nullptr;
   uint32_t const* pm_count = nullptr;
-  uint64_t const* pm_total = nullptr;
+  uint32_t const* pm_units = nullptr;  // each wave unit's first order position
+  uint64_t const* pm_num_units = nullptr;
   uint32_t pm_group        = 0;
 };
 
@@ -563,6 +565,20 @@ struct PmScanOp {
   }
 };
 
+// Units: polygon p's pairs [start, start + count) in runs of at most G.
+struct PmUnitOp {
+  uint32_t const* counts;
+  uint32_t const* start;
+  uint32_t* units;
+  uint32_t G;
+  __device__ uint64_t count(uint64_t p) const { return (counts[p] + G - 1) / G; }
+  __device__ void emit(uint64_t p, uint64_t prefix) const
+  {
+    uint32_t const n = (counts[p] + G - 1) / G;
+    for (uint32_t t = 0; t < n; ++t) units[prefix + t] = start[p] + t * G;
+  }
+};
+
 __global__ void __launch_bounds__(256)
   pm_scatter_kernel(PipArgs a, uint32_t* __restrict__ cursor, uint32_t* __restrict__ order)
 {
@@ -583,32 +599,20 @@ struct PmQueue {
   uint64_t tag[kWave];  // chunk slot << 6 | lane of the point in its chunk
 };
 
-// No occupancy bound: held to the 7 or 8 waves per SIMD its LDS would admit, the
-// compiler spills vector registers to scratch; unbounded it keeps 5 waves without
-// spills, which measured faster (batched workload: 7.12 -> 6.34 ms).
-template <int W, int S>
-__global__ void __launch_bounds__(W * kWave)
-  pip_test_pm_kernel(PipArgs a, uint64_t base, uint32_t xcd_run, uint64_t* __restrict__ masks,
-                     uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
+// One unit: the pairs at order positions [i, i + G) of one polygon (fewer at its end).
+template <int S>
+__device__ __forceinline__ void pm_unit(PipArgs const& a, uint32_t i, uint64_t* __restrict__ masks,
+                                        uint2* __restrict__ chunk_info, PipStage<S>* stage,
+                                        PmQueue* tq, bool& err, bool& cap_err)
 {
-  __shared__ PipStage<S> s_stage[W];
-  __shared__ PmQueue s_queue[W];
-  PipStage<S>* const stage = &s_stage[threadIdx.x / kWave];
-  PmQueue* const tq        = &s_queue[threadIdx.x / kWave];
-  uint64_t const blk = xcd_block(blockIdx.x, gridDim.x, xcd_run);
-  uint64_t const i   = base + (blk * static_cast<uint64_t>(W) + threadIdx.x / kWave);
-  uint64_t const T   = *a.pm_total;
-  if (i >= T) return;
   uint32_t const k0   = uniform(a.pm_order[i]);
   uint32_t const poly = uniform(a.pair_poly[k0]);
   uint32_t const s0   = uniform(a.pm_start[poly]);
   uint32_t const n    = uniform(a.pm_count[poly]);
   uint32_t const G    = a.pm_group;
-  uint32_t const off  = static_cast<uint32_t>(i) - s0;
-  if (off % G != 0) return;
+  uint32_t const off  = i - s0;
   uint32_t const end = s0 + (n - off < G ? n : off + G);
   int const lane     = lane_id();
-  bool err = false, cap_err = false;
   bool const filter = a.poly_box != nullptr;
   PolyBox fb{};
   if (filter) fb = a.poly_box[poly];
@@ -803,6 +807,29 @@ __global__ void __launch_bounds__(W * kWave)
     }
   }
   if (qn > 0) flush();
+}
+
+// One wave per unit of the unit list, in the XCD-run workgroup order (a
+// polygon's units are adjacent in the list, so its edges are fetched by one XCD);
+// the grid covers the most units the pairs can form, and waves past the count
+// return at once. No occupancy bound: held to the 7 or 8 waves per SIMD its LDS
+// would admit, the compiler spills vector registers to scratch; unbounded it
+// keeps 5 or 6 waves without spills, which measured faster.
+template <int W, int S>
+__global__ void __launch_bounds__(W * kWave)
+  pip_test_pm_kernel(PipArgs a, uint64_t base, uint32_t xcd_run, uint64_t* __restrict__ masks,
+                     uint2* __restrict__ chunk_info, uint32_t* __restrict__ bad)
+{
+  __shared__ PipStage<S> s_stage[W];
+  __shared__ PmQueue s_queue[W];
+  PipStage<S>* const stage = &s_stage[threadIdx.x / kWave];
+  PmQueue* const tq        = &s_queue[threadIdx.x / kWave];
+  int const lane = lane_id();
+  uint64_t const blk = xcd_block(blockIdx.x, gridDim.x, xcd_run);
+  uint64_t const u   = base + blk * static_cast<uint64_t>(W) + threadIdx.x / kWave;
+  if (u >= *a.pm_num_units) return;
+  bool err = false, cap_err = false;
+  pm_unit(a, uniform(a.pm_units[u]), masks, chunk_info, stage, tq, err, cap_err);
   if (err) atomicOr(bad, 1u);
   if (cap_err && lane == 0) atomicOr(bad, 2u);
 }
@@ -881,13 +908,23 @@ void launch_pip_test_ws(PipArgs const& a, uint64_t pairs, uint64_t* masks, uint2
                         uint32_t* bad, bool pm, hipStream_t stream)
 {
   uint32_t const xcd_run = static_cast<uint32_t>(tuning(Tuning::PipXcdRun));
+  if (pm) {
+    // a polygon with c pairs forms ceil(c / G) <= c / G + 1 units
+    uint64_t const units = std::min<uint64_t>(pairs, (pairs + a.pm_group - 1) / a.pm_group +
+                                                       a.num_polys);
+    for (uint64_t b = 0; b < units; b += kPipPairsPerLaunch) {
+      uint64_t const n = std::min<uint64_t>(units - b, kPipPairsPerLaunch);
+      unsigned const grid = static_cast<unsigned>((n + W - 1) / W);
+      pip_test_pm_kernel<W, S><<<grid, W * kWave, 0, stream>>>(a, b, xcd_run, masks, chunk_info,
+                                                               bad);
+      QPIP_LAUNCH_CHECK();
+    }
+    return;
+  }
   for (uint64_t b = 0; b < pairs; b += kPipPairsPerLaunch) {
     uint64_t const n = std::min<uint64_t>(pairs - b, kPipPairsPerLaunch);
     unsigned const grid = static_cast<unsigned>((n + W - 1) / W);
-    if (pm)
-      pip_test_pm_kernel<W, S><<<grid, W * kWave, 0, stream>>>(a, b, xcd_run, masks, chunk_info, bad);
-    else
-      pip_test_kernel<W, S><<<grid, W * kWave, 0, stream>>>(a, b, xcd_run, masks, chunk_info, bad);
+    pip_test_kernel<W, S><<<grid, W * kWave, 0, stream>>>(a, b, xcd_run, masks, chunk_info, bad);
     QPIP_LAUNCH_CHECK();
   }
 }
@@ -910,13 +947,18 @@ void launch_pip_test(PipArgs const& args, uint64_t pairs, uint64_t* masks, uint2
     count_scan_emit<PmScanOp, uint64_t, uint64_t>(op, nullptr, a.num_polys, pmb->partials,
                                                   pmb->total, stream,
                                                   scan_blocks_for(a.num_polys), "pip_order_scan");
+    PmUnitOp uop{pmb->count, pmb->start, pmb->units, static_cast<uint32_t>(group)};
+    count_scan_emit<PmUnitOp, uint64_t, uint64_t>(uop, nullptr, a.num_polys, pmb->partials,
+                                                  pmb->num_units, stream,
+                                                  scan_blocks_for(a.num_polys), "pip_order_scan");
     pm_scatter_kernel<<<grid_for(pairs, 256, 4096), 256, 0, stream>>>(a, pmb->cursor, pmb->order);
     QPIP_LAUNCH_CHECK();
-    a.pm_order = pmb->order;
-    a.pm_start = pmb->start;
-    a.pm_count = pmb->count;
-    a.pm_total = pmb->total;
-    a.pm_group = static_cast<uint32_t>(group);
+    a.pm_order     = pmb->order;
+    a.pm_start     = pmb->start;
+    a.pm_count     = pmb->count;
+    a.pm_units     = pmb->units;
+    a.pm_num_units = pmb->num_units;
+    a.pm_group     = static_cast<uint32_t>(group);
   }
   KernelTimer kt("pip_test", stream);
   int64_t const w = tuning(Tuning::PipWavesPerBlock);
@@ -1069,7 +1111,8 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
                d.chunk_start, d.chunk_cap,
                d.num_nodes, d.xy, reinterpret_cast<uintptr_t>(d.xy) % 16 == 0, d.pairs_per_wave,
                n_pairs, d.chunk_cap, d.point_base, d.poly_box};
-  PmBuffers const pmb{d.pm_count, d.pm_start, d.pm_cursor, d.pm_order, d.pm_partials, d.pm_total};
+  PmBuffers const pmb{d.pm_count,    d.pm_start, d.pm_cursor, d.pm_order,
+                      d.pm_units,    d.pm_partials, d.pm_total, d.pm_num_units};
   launch_pip_test(args, d.pair_cap, d.masks, d.chunk_info, flags, d.pm_order ? &pmb : nullptr,
                   stream);
   HitScanOp hop{d.masks, d.hit_start};
@@ -1193,10 +1236,10 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     // the polygon-major test's order (pip_polygon_group > 0)
     bool const pm_on = tuning(Tuning::PipPolygonGroup) > 0 && num_polys > 0;
     dbuf<uint32_t> pm_count(pm_on ? num_polys : 1), pm_start(pm_on ? num_polys : 1),
-      pm_cursor(pm_on ? num_polys : 1), pm_order(pm_on ? K : 1);
-    dbuf<uint64_t> pm_partials(kScanBlocks + 1), pm_total(1);
-    PmBuffers const pmb{pm_count.get(), pm_start.get(), pm_cursor.get(), pm_order.get(),
-                        pm_partials.get(), pm_total.get()};
+      pm_cursor(pm_on ? num_polys : 1), pm_order(pm_on ? K : 1), pm_units(pm_on ? K : 1);
+    dbuf<uint64_t> pm_partials(kScanBlocks + 1), pm_total(1), pm_num_units(1);
+    PmBuffers const pmb{pm_count.get(), pm_start.get(),    pm_cursor.get(), pm_order.get(),
+                        pm_units.get(), pm_partials.get(), pm_total.get(),  pm_num_units.get()};
     uint64_t h_tot[2] = {0, 0};
     uint32_t h_bad[2] = {0, 0};
     for (;;) {

@@ -58,8 +58,10 @@ struct PipDevice {
   uint32_t* pm_start;     // num_polys
   uint32_t* pm_cursor;    // num_polys
   uint32_t* pm_order;     // pair_cap
+  uint32_t* pm_units;     // pair_cap
   uint64_t* pm_partials;  // kScanBlocks + 1
   uint64_t* pm_total;     // 1
+  uint64_t* pm_num_units; // 1
   // output: (polygon, point position) hits, at most hit_cap
   uint32_t* out_poly;
   uint32_t* out_pos;
@@ -69,12 +71,14 @@ struct PipDevice {
 
 // Buffers of the polygon-major test's pair order.
 struct PmBuffers {
-  uint32_t* count;
-  uint32_t* start;
-  uint32_t* cursor;
-  uint32_t* order;
-  uint64_t* partials;
-  uint64_t* total;
+  uint32_t* count;      // per polygon
+  uint32_t* start;      // per polygon
+  uint32_t* cursor;     // per polygon
+  uint32_t* order;      // per pair
+  uint32_t* units;      // per pair (at most one unit per pair)
+  uint64_t* partials;   // kScanBlocks + 1
+  uint64_t* total;      // 1
+  uint64_t* num_units;  // 1
 };
 
 // Enqueues the stage on `stream`: chunks, test, scan, write.
