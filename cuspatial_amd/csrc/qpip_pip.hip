@@ -189,13 +189,25 @@ __device__ __forceinline__ double wave_max_abs(double const (&px)[kSlots],
   return m;
 }
 
+// Lane groups (lg > 0, one slot only): the wave holds G = 2^lg copies of q <= 64 / G
+// points, lane l holding point l % (64 / G) of copy g = l / (64 / G), and step e
+// gives copy g the ring's edge e * G + g: the G copies read G adjacent line forms.
+// A crossing toggles on the two ends of its own edge only: y1 from its first
+// vertex, y0 from the vertex before it, which is the previous edge's first vertex
+// and so copy g - 1's y1 of the same step (64 / G lanes down the ballot), or, for
+// copy 0, the last copy's y1 of the step before. The ring's parity is the XOR of
+// the copies' parities (the suspect marks their OR), so a flush of few points
+// takes G times fewer edge steps. The ring's last batch is padded to a multiple
+// of G with edges that neither toggle (their vertex repeats the ring's last one)
+// nor mark (t = +inf for finite points; for a point that is not finite t is NaN,
+// a mark, as on every real edge).
 template <int U, int S = kStageEdges>
 __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly,
                                              double const (&px)[kSlots],
                                              double const (&py)[kSlots],
                                              uint64_t const (&valid_m)[kSlots], double pmax,
                                              uint64_t (&hit_m)[kSlots], bool& err,
-                                             PipStage<S>* stage)
+                                             PipStage<S>* stage, uint32_t lg = 0)
 {
   uint64_t inside_m[U], suspect_m[U];
 #pragma unroll
@@ -237,6 +249,48 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly,
         y0[u] = y1;
       }
     };
+    if (U == 1 && lg != 0) {
+      uint32_t const G  = 1u << lg;
+      uint32_t const Q  = static_cast<uint32_t>(kWave) >> lg;  // lanes per copy
+      uint32_t const gq = static_cast<uint32_t>(lane) >> (6 - lg);
+      // y1 of the previous step; before edge 0 every copy holds the ring's b vertex
+      uint64_t y1p = ballot(by0 > py[0]);
+      // one step: copy g takes edge e + g; the vertex before it is the previous
+      // copy's edge (Q lanes down), or for copy 0 the last copy's previous step
+      auto step = [&](LineEdge const& ed) {
+        double const t    = fma(px[0], ed.rise, fma(-py[0], ed.run, ed.c));
+        uint64_t const y1 = ballot(ed.ay > py[0]);
+        uint64_t const cr = ballot(t < 0.0);
+        suspect_m[0] |= ballot(!(fabs(t) > tol));
+        uint64_t const y0 = (y1 << Q) | (y1p >> (kWave - Q));
+        inside_m[0] ^= (y1 ^ y0) & cr;
+        y1p = y1;
+      };
+      for (uint32_t base = 0; base < ne; base += S) {
+        uint32_t const n   = ne - base < S ? ne - base : S;
+        uint32_t const np  = (n + G - 1) & ~(G - 1);  // <= S: S is a multiple of G
+        double const ylast = L[base + n - 1].ay;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int i = 0; i < S; i += kWave)
+          if (static_cast<uint32_t>(i + lane) < n) stage->e[i + lane] = L[base + i + lane];
+        if (static_cast<uint32_t>(lane) < np - n)
+          stage->e[n + lane] = LineEdge{ylast, 0.0, 0.0, __builtin_inf()};
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        LineEdge const* E = &stage->e[gq];
+        uint32_t e        = 0;
+        for (; e + 2 * G <= np; e += 2 * G) {
+          step(E[e]);
+          step(E[e + G]);
+        }
+        if (e < np) step(E[e]);
+      }
+      continue;
+    }
     for (uint32_t base = 0; base < ne; base += S) {
       uint32_t const n = ne - base < S ? ne - base : S;
       // the previous batch has been read by every lane before it is overwritten,
@@ -257,6 +311,13 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly,
         edge(stage->e[e + 1]);
       }
       if (e < n) edge(stage->e[e]);
+    }
+  }
+  if (U == 1 && lg != 0) {
+    // the copies' parities and marks onto the first copy's lanes
+    for (uint32_t sh = kWave / 2; sh >= (static_cast<uint32_t>(kWave) >> lg); sh >>= 1) {
+      inside_m[0] ^= inside_m[0] >> sh;
+      suspect_m[0] |= suspect_m[0] >> sh;
     }
   }
   // Suspect points: the reference's test over the whole polygon
@@ -631,7 +692,16 @@ __device__ __forceinline__ void pm_unit(PipArgs const& a, uint32_t i, uint64_t* 
     double m = amax_p;
 #pragma unroll
     for (int o = kWave / 2; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, kWave));
-    test_polygon<1, S>(a, poly, qx, qy, test_m, m, hit_m, err, stage);
+    // at most half a wave of points: copies of them in lane groups
+    uint32_t lg = 0;
+    if (qn <= kWave / 2) {
+      uint32_t const q = qn > 1 ? 32u - static_cast<uint32_t>(__builtin_clz(qn - 1)) : 0u;
+      lg               = 6 - q;  // 2^q >= qn lanes per copy
+      int const src    = lane & ((1 << q) - 1);
+      qx[0]            = __shfl(qx[0], src, kWave);
+      qy[0]            = __shfl(qy[0], src, kWave);
+    }
+    test_polygon<1, S>(a, poly, qx, qy, test_m, m, hit_m, err, stage, lg);
     if ((hit_m[0] >> lane) & 1ull)
       atomicOr(reinterpret_cast<unsigned long long*>(&masks[qtag >> 6]), 1ull << (qtag & 63u));
     qn     = 0;

@@ -233,3 +233,76 @@ def test_box_filter_margins_match_oracle(cs, oracle_mod, k):
         hits = set(zip(g["polygon_index"].tolist(), g["pi"][g["point_index"]].tolist()))
         qi = int(np.flatnonzero((x == quirk[0]) & (y == quirk[1]))[0])
         assert (0, qi) in hits  # polygon 0 is the quirk triangle
+
+
+def long_ring_scene(seed):
+    """Star polygons with long rings (65 to 1,000 edges, some with a 70-edge hole)
+    on a grid of disjoint cells, each cell holding a chosen number of points (1 to
+    100): on vertices, a few ULPs beside them, on the vertical lines through them,
+    and uniform in the cell. With the box filter a polygon's flush then tests
+    exactly its own cell's points, so every flush size occurs."""
+    rng = np.random.default_rng(seed)
+    part, ring, vx, vy, xs, ys = [0], [0], [], [], [], []
+
+    def add_ring(xr, yr):
+        vx.extend(list(xr) + [xr[0]])
+        vy.extend(list(yr) + [yr[0]])
+        ring.append(ring[-1] + len(xr) + 1)
+
+    sizes = [65, 66, 127, 129, 130, 200, 513, 1000]
+    counts = [1, 2, 3, 5, 8, 16, 17, 31, 32, 33, 63, 64, 65, 100]
+    cell = 8.0
+    for c in range(24):
+        cx, cy = cell * (c % 6) + cell / 2, cell * (c // 6) + cell / 2
+        k = sizes[c % len(sizes)]
+        a = np.sort(rng.uniform(0, 2 * np.pi, k))
+        r = 3.0 * rng.uniform(0.7, 1.0, k)
+        ox, oy = cx + r * np.cos(a), cy + r * np.sin(a)
+        add_ring(ox, oy)
+        nr = 1
+        if c % 3 == 0:  # a hole, clockwise
+            h = np.sort(rng.uniform(0, 2 * np.pi, 69))[::-1]
+            add_ring(cx + 0.8 * np.cos(h), cy + 0.8 * np.sin(h))
+            nr = 2
+        part.append(part[-1] + nr)
+        m = counts[c % len(counts)]
+        vi = rng.integers(0, k, m)
+        kind = rng.integers(0, 4, m)
+        for v, t in zip(vi, kind):
+            if t == 0:
+                xs.append(ox[v])
+                ys.append(ulps(oy[v], int(rng.integers(-3, 4))))
+            elif t == 1:
+                xs.append(ulps(ox[v], int(rng.integers(-2, 3))))
+                ys.append(oy[v])
+            elif t == 2:  # the vertical line through the vertex, inside the cell
+                xs.append(ox[v])
+                ys.append(cy + rng.uniform(-3.5, 3.5))
+            else:
+                xs.append(cx + rng.uniform(-3.5, 3.5))
+                ys.append(cy + rng.uniform(-3.5, 3.5))
+    polys = (np.array(part, np.int32), np.array(ring, np.int32), np.array(vx), np.array(vy))
+    return np.array(xs), np.array(ys), polys
+
+
+@pytest.mark.parametrize("stage,group,box", [(128, 1, 1), (128, 16, 1), (64, 16, 1),
+                                             (128, 4096, 0), (128, 0, 1)])
+def test_long_rings_few_points_match_oracle(cs, oracle_mod, stage, group, box):
+    """Polygon-major flushes of 1 to 64 points, whose edge walks split over copies
+    of the points (rings longer than one staged batch, holes, points on vertices
+    and on the vertical lines through them): the oracle's pairs bit for bit."""
+    from cuspatial_amd import autotune
+
+    x, y, polys = long_ring_scene(31)
+    bbox = (0.0, 48.0, 0.0, 48.0)
+    saved = autotune.current()
+    for d, max_size in ((4, 64), (2, 256)):
+        try:
+            autotune.apply({"pip_stage_edges": stage, "pip_polygon_group": group,
+                            "pip_box_filter": box, "pip_waves_per_block": 2})
+            g = run_gpu(cs, x, y, bbox, 48.0 / (1 << d), d, max_size, polys)
+        finally:
+            autotune.apply(saved)
+        o = run_oracle(oracle_mod, x, y, bbox, 48.0 / (1 << d), d, max_size, polys)
+        assert_same(g, o)
+        assert len(o["polygon_index"]) > 0
