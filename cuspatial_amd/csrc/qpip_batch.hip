@@ -816,9 +816,9 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   for (uint32_t k = 0; k < J; ++k) job_hits[j0 + k] = 0;
   if (n == 0) return;
 
-  // 1-2. keys and the sort, each job's points on their own (values: group-local
-  // point index)
-  dbuf<uint32_t> keys_a(n), keys_b(n), vals(n), vals_tmp(n), samples((n + 31) / 32);
+  // 1-2. keys and the sort, each job's points on their own; the last pass writes
+  // the job-local point_indices output (the PIP test adds each node's job base)
+  dbuf<uint32_t> keys_a(n), keys_b(n), vals_tmp(n), samples((n + 31) / 32);
   dbuf<uint32_t> counts0(tile_tab.size() * kRadix);
   {
     KernelTimer kt("batch_keygen", stream);
@@ -826,11 +826,11 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
       points_xy + 2 * P0, stiles.get(), jobs.get(), keys_a.get(), counts0.get());
     QPIP_LAUNCH_CHECK();
   }
-  // the last pass also writes the job-local point_indices output
-  SortResult sorted = segmented_radix_sort_pairs(keys_a.get(), keys_b.get(), vals.get(),
+  SortResult sorted = segmented_radix_sort_pairs(keys_a.get(), keys_b.get(), out_pi,
                                                  vals_tmp.get(), n, key_bits, stiles.get(),
-                                                 tile_tab.size(), ssegs.get(), seg_tab.size(), counts0.get(),
-                                                 out_pi, samples.get(), false, stream, "batch_sort");
+                                                 tile_tab.size(), ssegs.get(), seg_tab.size(),
+                                                 counts0.get(), nullptr, samples.get(), true,
+                                                 stream, "batch_sort");
   vals_tmp.release();
 
   // 3. forest levels
@@ -874,12 +874,13 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
                                                           trees.get());
   QPIP_LAUNCH_CHECK();
   dbuf<uint8_t> f_internal(std::max<uint64_t>(M, 1));
-  dbuf<uint32_t> f_length(std::max<uint64_t>(M, 1)), f_offset(std::max<uint64_t>(M, 1));
+  dbuf<uint32_t> f_length(std::max<uint64_t>(M, 1)), f_offset(std::max<uint64_t>(M, 1)),
+    f_base(std::max<uint64_t>(M, 1));
   if (M > 0) {
     batch_finalize_kernel<<<grid_for(M, 256), 256, 0, stream>>>(
       nkey.get(), nstart.get(), ncount.get(), nlevel.get(), njob.get(), cfirst.get(), ccount.get(),
       jobs.get(), trees.get(), M, nullptr, 0, f_internal.get(), f_length.get(), f_offset.get(),
-      nullptr);
+      f_base.get());
     QPIP_LAUNCH_CHECK();
   }
   if (M == 0 || npg == 0) {
@@ -981,12 +982,13 @@ void run_group(std::vector<HostJob> const& hj, int64_t j0, int64_t j1, int64_t c
   // 6. point-in-polygon on the forest
   uint32_t *hp = nullptr, *ht = nullptr;
   int64_t H = 0;
-  int rc = qpip_quadtree_point_in_polygon_xy(
+  int rc = quadtree_point_in_polygon_entry(
     pb.get(), pq.get(), static_cast<int64_t>(K), nullptr, nullptr, nullptr, f_length.get(),
-    f_offset.get(), static_cast<int64_t>(M), vals.get(), static_cast<int64_t>(n),
-    points_xy + 2 * P0, static_cast<int64_t>(n), reinterpret_cast<uint32_t const*>(poly_offsets + PO0),
-    static_cast<int64_t>(npg) + 1, reinterpret_cast<uint32_t const*>(ring_offsets),
-    num_ring_offsets, poly_x, poly_y, num_vertices, num_vertices, nullptr, stream, &hp, &ht, &H);
+    f_offset.get(), static_cast<int64_t>(M), out_pi, static_cast<int64_t>(n), nullptr, nullptr,
+    points_xy + 2 * P0, static_cast<int64_t>(n), static_cast<int64_t>(n),
+    reinterpret_cast<uint32_t const*>(poly_offsets + PO0), static_cast<int64_t>(npg) + 1,
+    reinterpret_cast<uint32_t const*>(ring_offsets), num_ring_offsets, poly_x, poly_y,
+    num_vertices, num_vertices, nullptr, stream, &hp, &ht, &H, f_base.get());
   if (rc != QPIP_SUCCESS) throw logic_error(qpip_last_error_message());
   struct Hits {
     uint32_t* p[2];

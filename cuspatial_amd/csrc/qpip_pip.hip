@@ -1201,10 +1201,8 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
   }
 }
 
-}  // namespace qpip
-
-namespace {
-
+// The plain test's entry, also used by the batched eager call (point_base non-null:
+// point_indices are job-local and point_base[node] is the node's job's first point).
 int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                                               const uint32_t* quad_index,
                                               int64_t num_pairs,
@@ -1233,9 +1231,9 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                                               void* stream_,
                                               uint32_t** polygon_index,
                                               uint32_t** point_index,
-                                              int64_t* num_hits)
+                                              int64_t* num_hits,
+                                              const uint32_t* point_base)
 {
-  using namespace qpip;
   (void)key;
   (void)level;
   (void)is_internal_node;
@@ -1336,7 +1334,7 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
                    chunk_start.get(), chunk_cap,
                    static_cast<uint64_t>(num_nodes), xy,
                    xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
-                   pairs_per_wave(K), nullptr, chunk_cap, nullptr, poly_box.get()};
+                   pairs_per_wave(K), nullptr, chunk_cap, point_base, poly_box.get()};
       launch_pip_test(args, K, masks.get(), chunk_info.get(), bad.get() + 1,
                       pm_on ? &pmb : nullptr, stream);
       // chunks past the capacity were not tested: the scan covers those that were
@@ -1388,7 +1386,7 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
   }
 }
 
-}  // namespace
+}  // namespace qpip
 
 extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
                                               const uint32_t* quad_index,
@@ -1419,11 +1417,11 @@ extern "C" int qpip_quadtree_point_in_polygon(const uint32_t* poly_index,
                                               uint32_t** point_index,
                                               int64_t* num_hits)
 {
-  return quadtree_point_in_polygon_entry(
+  return qpip::quadtree_point_in_polygon_entry(
     poly_index, quad_index, num_pairs, key, level, is_internal_node, length, offset, num_nodes,
     point_indices, num_point_indices, x, y, nullptr, num_x, num_y, poly_offsets,
     num_poly_offsets, ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x, num_poly_y, mr,
-    stream, polygon_index, point_index, num_hits);
+    stream, polygon_index, point_index, num_hits, nullptr);
 }
 
 extern "C" int qpip_quadtree_point_in_polygon_xy(const uint32_t* poly_index,
@@ -1453,11 +1451,11 @@ extern "C" int qpip_quadtree_point_in_polygon_xy(const uint32_t* poly_index,
                                                  uint32_t** point_index,
                                                  int64_t* num_hits)
 {
-  return quadtree_point_in_polygon_entry(
+  return qpip::quadtree_point_in_polygon_entry(
     poly_index, quad_index, num_pairs, key, level, is_internal_node, length, offset, num_nodes,
     point_indices, num_point_indices, nullptr, nullptr, points_xy, num_points, num_points,
     poly_offsets, num_poly_offsets, ring_offsets, num_ring_offsets, poly_x, poly_y, num_poly_x,
-    num_poly_y, mr, stream, polygon_index, point_index, num_hits);
+    num_poly_y, mr, stream, polygon_index, point_index, num_hits, nullptr);
 }
 
 namespace {

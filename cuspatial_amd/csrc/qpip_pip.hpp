@@ -84,6 +84,24 @@ struct PmBuffers {
 // Enqueues the stage on `stream`: chunks, test, scan, write.
 void launch_pip_device(PipDevice const& d, hipStream_t stream);
 
+// The quadtree PIP join over host-visible sizes (qpip_quadtree_point_in_polygon and
+// its _xy form; x/y or xy). With point_base non-null, point_indices are local to
+// each node's job and point_base[node] is that job's first point (the batched
+// eager call); the C entry points pass null.
+int quadtree_point_in_polygon_entry(const uint32_t* poly_index, const uint32_t* quad_index,
+                                    int64_t num_pairs, const uint32_t* key, const uint8_t* level,
+                                    const uint8_t* is_internal_node, const uint32_t* length,
+                                    const uint32_t* offset, int64_t num_nodes,
+                                    const uint32_t* point_indices, int64_t num_point_indices,
+                                    const double* x, const double* y, const double* xy,
+                                    int64_t num_x, int64_t num_y, const uint32_t* poly_offsets,
+                                    int64_t num_poly_offsets, const uint32_t* ring_offsets,
+                                    int64_t num_ring_offsets, const double* poly_x,
+                                    const double* poly_y, int64_t num_poly_x, int64_t num_poly_y,
+                                    const qpip_resource* mr, void* stream,
+                                    uint32_t** polygon_index, uint32_t** point_index,
+                                    int64_t* num_hits, const uint32_t* point_base);
+
 // The test kernel's pairs-per-wave segment length for a join of num_pairs pairs
 // (the runtime tuning parameters pip_min_pairs_per_wave, pip_pairs_shift).
 uint32_t pip_pairs_per_wave(uint64_t num_pairs);
