@@ -145,6 +145,41 @@ def test_batch_edge_cases_parity(cs, oracle_mod):
     assert check_all(cs, oracle_mod, w) > 0
 
 
+@pytest.mark.parametrize("group", [1, 16, 64])
+def test_batch_polygon_major_parity(cs, oracle_mod, group):
+    """The batched call with the PIP test in polygon-major order (pip_polygon_group)
+    and the box filter on, on jobs of several depths (one of depth 15, in a sort
+    group of its own) with points on the quarter-cell lines of each job's deepest
+    cells (a few ULPs either side) and on the polygons' vertices: the oracle's
+    pairs."""
+    from cuspatial_amd import autotune
+
+    rng = np.random.default_rng(12)
+    jobs = [star_job(rng, 6000, 30), star_job(rng, 20000, 20, d=15, max_size=5),
+            star_job(rng, 3000, 15, ext=2000.0, d=3, max_size=2),
+            star_job(rng, 60000, 40, d=9, max_size=64)]
+    for jb in jobs:
+        n, d = len(jb["x"]), jb["d"]
+        x0, x1, y0, y1 = jb["bbox"]
+        sc = jb["scale"]
+        k = n // 4
+        ex = x0 + np.floor(rng.uniform(0, 4 << d, k)) * 0.25 * sc
+        ey = y0 + np.floor(rng.uniform(0, 4 << d, k)) * 0.25 * sc
+        jb["x"][:k] = np.nextafter(ex, np.where(rng.uniform(size=k) < 0.5, -np.inf, np.inf))
+        jb["y"][:k] = np.nextafter(ey, np.where(rng.uniform(size=k) < 0.5, -np.inf, np.inf))
+        vx, vy = np.asarray(jb["vx"]), np.asarray(jb["vy"])
+        if len(vx):
+            m = min(n - k, 2 * len(vx))
+            i = rng.integers(0, len(vx), m)
+            jb["x"][k:k + m] = np.where(rng.uniform(size=m) < 0.5, vx[i], jb["x"][k:k + m])
+            jb["y"][k:k + m] = np.where(rng.uniform(size=m) < 0.5, vy[i], jb["y"][k:k + m])
+        jb["x"] = np.clip(jb["x"], min(x0, x1), max(x0, x1))
+        jb["y"] = np.clip(jb["y"], min(y0, y1), max(y0, y1))
+    w = concat_jobs(jobs)
+    with autotune.applied({"pip_polygon_group": group, "pip_box_filter": 1}):
+        assert check_all(cs, oracle_mod, w) > 0
+
+
 def test_batch_matches_single_job_calls(cs):
     """The batched result equals the four single-job GPU calls, job by job."""
     import torch
