@@ -104,7 +104,7 @@ struct PipArgs {
   uint32_t const* pm_order = nullptr;
   uint32_t const* pm_start = nullptr;
   uint32_t const* pm_count = nullptr;
-  uint32_t const* pm_units = nullptr;  // each wave unit's first order position
+  uint4 const* pm_units = nullptr;  // per unit: first order position, polygon, pairs
   uint64_t const* pm_num_units = nullptr;
   uint32_t pm_group        = 0;
 };
@@ -630,13 +630,16 @@ struct PmScanOp {
 struct PmUnitOp {
   uint32_t const* counts;
   uint32_t const* start;
-  uint32_t* units;
+  uint4* units;
   uint32_t G;
   __device__ uint64_t count(uint64_t p) const { return (counts[p] + G - 1) / G; }
+  // one record per unit, so that a wave finds its polygon and pairs in one load
   __device__ void emit(uint64_t p, uint64_t prefix) const
   {
-    uint32_t const n = (counts[p] + G - 1) / G;
-    for (uint32_t t = 0; t < n; ++t) units[prefix + t] = start[p] + t * G;
+    uint32_t const c = counts[p], n = (c + G - 1) / G;
+    for (uint32_t t = 0; t < n; ++t)
+      units[prefix + t] = make_uint4(start[p] + t * G, static_cast<uint32_t>(p),
+                                     c - t * G < G ? c - t * G : G, 0u);
   }
 };
 
@@ -660,19 +663,14 @@ struct PmQueue {
   uint64_t tag[kWave];  // chunk slot << 6 | lane of the point in its chunk
 };
 
-// One unit: the pairs at order positions [i, i + G) of one polygon (fewer at its end).
+// One unit: the cnt pairs of polygon `poly` at order positions [i, i + cnt).
 template <int S>
-__device__ __forceinline__ void pm_unit(PipArgs const& a, uint32_t i, uint64_t* __restrict__ masks,
+__device__ __forceinline__ void pm_unit(PipArgs const& a, uint32_t i, uint32_t poly, uint32_t cnt,
+                                        uint64_t* __restrict__ masks,
                                         uint2* __restrict__ chunk_info, PipStage<S>* stage,
                                         PmQueue* tq, bool& err, bool& cap_err)
 {
-  uint32_t const k0   = uniform(a.pm_order[i]);
-  uint32_t const poly = uniform(a.pair_poly[k0]);
-  uint32_t const s0   = uniform(a.pm_start[poly]);
-  uint32_t const n    = uniform(a.pm_count[poly]);
-  uint32_t const G    = a.pm_group;
-  uint32_t const off  = i - s0;
-  uint32_t const end = s0 + (n - off < G ? n : off + G);
+  uint32_t const end = i + cnt;
   int const lane     = lane_id();
   bool const filter = a.poly_box != nullptr;
   PolyBox fb{};
@@ -899,7 +897,13 @@ __global__ void __launch_bounds__(W * kWave)
   uint64_t const u   = base + blk * static_cast<uint64_t>(W) + threadIdx.x / kWave;
   if (u >= *a.pm_num_units) return;
   bool err = false, cap_err = false;
-  pm_unit(a, uniform(a.pm_units[u]), masks, chunk_info, stage, tq, err, cap_err);
+  uint4 const un = a.pm_units[u];
+  if (uniform(un.y) >= a.num_polys) {
+    atomicOr(bad, 1u);
+    return;
+  }
+  pm_unit(a, uniform(un.x), uniform(un.y), uniform(un.z), masks, chunk_info, stage, tq, err,
+          cap_err);
   if (err) atomicOr(bad, 1u);
   if (cap_err && lane == 0) atomicOr(bad, 2u);
 }
@@ -1304,7 +1308,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     // the polygon-major test's order (pip_polygon_group > 0)
     bool const pm_on = tuning(Tuning::PipPolygonGroup) > 0 && num_polys > 0;
     dbuf<uint32_t> pm_count(pm_on ? num_polys : 1), pm_start(pm_on ? num_polys : 1),
-      pm_cursor(pm_on ? num_polys : 1), pm_order(pm_on ? K : 1), pm_units(pm_on ? K : 1);
+      pm_cursor(pm_on ? num_polys : 1), pm_order(pm_on ? K : 1);
+    dbuf<uint4> pm_units(pm_on ? K : 1);
     dbuf<uint64_t> pm_partials(kScanBlocks + 1), pm_total(1), pm_num_units(1);
     PmBuffers const pmb{pm_count.get(), pm_start.get(),    pm_cursor.get(), pm_order.get(),
                         pm_units.get(), pm_partials.get(), pm_total.get(),  pm_num_units.get()};

@@ -58,7 +58,7 @@ struct PipDevice {
   uint32_t* pm_start;     // num_polys
   uint32_t* pm_cursor;    // num_polys
   uint32_t* pm_order;     // pair_cap
-  uint32_t* pm_units;     // pair_cap
+  uint4* pm_units;        // pair_cap
   uint64_t* pm_partials;  // kScanBlocks + 1
   uint64_t* pm_total;     // 1
   uint64_t* pm_num_units; // 1
@@ -75,7 +75,7 @@ struct PmBuffers {
   uint32_t* start;      // per polygon
   uint32_t* cursor;     // per polygon
   uint32_t* order;      // per pair
-  uint32_t* units;      // per pair (at most one unit per pair)
+  uint4* units;         // per pair (at most one unit per pair)
   uint64_t* partials;   // kScanBlocks + 1
   uint64_t* total;      // 1
   uint64_t* num_units;  // 1
