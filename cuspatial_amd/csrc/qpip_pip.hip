@@ -201,38 +201,66 @@ __device__ __forceinline__ double wave_max_abs(double const (&px)[kSlots],
 // of G with edges that neither toggle (their vertex repeats the ring's last one)
 // nor mark (t = +inf for finite points; for a point that is not finite t is NaN,
 // a mark, as on every real edge).
+// A polygon's ring range and its first ring's table entries, loaded ahead of its
+// test: a polygon-major wave loads them when it starts its unit, so that their
+// dependent loads overlap the point gathers instead of starting each test.
+struct RingHint {
+  uint32_t r0, r1, e0, ne;
+  double amax, by0;
+};
+
+__device__ __forceinline__ RingHint ring_hint(PipArgs const& a, uint32_t poly)
+{
+  RingHint h{0u, 0u, 0u, 0u, 0.0, 0.0};
+  h.r0 = uniform(a.poly_offsets[poly]);
+  h.r1 = uniform(a.poly_offsets[poly + 1]);
+  if (h.r1 > a.num_rings) h.r1 = static_cast<uint32_t>(a.num_rings);
+  if (h.r0 < h.r1) {
+    h.e0   = uniform(a.ring_offsets[h.r0]);
+    h.ne   = uniform(a.ring_edges[h.r0]);
+    h.amax = a.ring_amax[h.r0];
+    if (h.ne > 0) h.by0 = a.edges_b[h.e0].by;
+  }
+  return h;
+}
+
 template <int U, int S = kStageEdges>
 __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly,
                                              double const (&px)[kSlots],
                                              double const (&py)[kSlots],
                                              uint64_t const (&valid_m)[kSlots], double pmax,
                                              uint64_t (&hit_m)[kSlots], bool& err,
-                                             PipStage<S>* stage, uint32_t lg = 0)
+                                             PipStage<S>* stage, uint32_t lg = 0,
+                                             RingHint const* hint = nullptr)
 {
   uint64_t inside_m[U], suspect_m[U];
 #pragma unroll
   for (int u = 0; u < U; ++u) inside_m[u] = suspect_m[u] = 0;
   uint32_t r0 = 0, r1 = 0;
-  if (poly < a.num_polys) {
+  if (poly >= a.num_polys) {
+    err = true;
+  } else if (hint) {
+    r0 = hint->r0;
+    r1 = hint->r1;
+  } else {
     r0 = uniform(a.poly_offsets[poly]);
     r1 = uniform(a.poly_offsets[poly + 1]);
     if (r1 > a.num_rings) r1 = static_cast<uint32_t>(a.num_rings);
-  } else {
-    err = true;
   }
   int const lane = lane_id();
   for (uint32_t r = r0; r < r1; ++r) {
-    uint32_t const e0 = uniform(a.ring_offsets[r]);
-    uint32_t const ne = uniform(a.ring_edges[r]);
+    bool const hinted = hint && r == r0;
+    uint32_t const e0 = hinted ? hint->e0 : uniform(a.ring_offsets[r]);
+    uint32_t const ne = hinted ? hint->ne : uniform(a.ring_edges[r]);
     if (ne == 0) continue;
     LineEdge const* L = a.lines + e0;
     // one tolerance for the ring: s bounds |px| + |ax| and |py| + |ay| (the wave's
     // points, the ring's vertices); the line forms are scaled to n' <= 1
-    double const s   = pmax + a.ring_amax[r];
+    double const s   = pmax + (hinted ? hint->amax : a.ring_amax[r]);
     double const tol = s < 0x1p400 ? fma(s, 0x1p-48, 0x1p-1000) : __builtin_inf();
     // y0 of the first edge is its `b` (the ring's last vertex); afterwards the
     // previous edge's `a` (is_point_in_polygon.cuh:339, :375-376).
-    double const by0 = a.edges_b[e0].by;
+    double const by0 = hinted ? hint->by0 : a.edges_b[e0].by;
     uint64_t y0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) y0[u] = ballot(by0 > py[u]);
@@ -671,6 +699,7 @@ __device__ __forceinline__ void pm_unit(PipArgs const& a, uint32_t i, uint32_t p
                                         PmQueue* tq, bool& err, bool& cap_err)
 {
   uint32_t const end = i + cnt;
+  RingHint const rh  = ring_hint(a, poly);
   int const lane     = lane_id();
   bool const filter = a.poly_box != nullptr;
   PolyBox fb{};
@@ -699,7 +728,7 @@ __device__ __forceinline__ void pm_unit(PipArgs const& a, uint32_t i, uint32_t p
       qx[0]            = __shfl(qx[0], src, kWave);
       qy[0]            = __shfl(qy[0], src, kWave);
     }
-    test_polygon<1, S>(a, poly, qx, qy, test_m, m, hit_m, err, stage, lg);
+    test_polygon<1, S>(a, poly, qx, qy, test_m, m, hit_m, err, stage, lg, &rh);
     if ((hit_m[0] >> lane) & 1ull)
       atomicOr(reinterpret_cast<unsigned long long*>(&masks[qtag >> 6]), 1ull << (qtag & 63u));
     qn     = 0;
