@@ -1169,8 +1169,8 @@ struct PlanGroup {
   dbuf<uint32_t> chunk_start, hit_poly, hit_pos;
   dbuf<uint64_t> pip_partials, pip_totals, masks, hit_start;
   // the polygon-major PIP test's pair order (used when pip_polygon_group > 0)
-  dbuf<uint32_t> pm_count, pm_start, pm_cursor, pm_order;
-  dbuf<uint4> pm_units;
+  dbuf<uint32_t> pm_count, pm_start, pm_cursor;
+  dbuf<uint4> pm_order, pm_units;
   dbuf<uint64_t> pm_partials, pm_total, pm_num_units;
   dbuf<uint2> chunk_info;
   uint32_t pairs_per_wave = 16;

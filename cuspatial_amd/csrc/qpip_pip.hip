@@ -98,10 +98,10 @@ struct PipArgs {
   // non-null: each polygon's vertex box, for the test's point filter
   PolyBox const* poly_box = nullptr;
   // polygon-major order (pip_test_pm_kernel): the valid pairs grouped by
-  // polygon, polygon p's at pm_order[pm_start[p] .. + pm_count[p]); a unit is up
-  // to pm_group of one polygon's pairs, *pm_num_units of them, listed by their
-  // first position in pm_order
-  uint32_t const* pm_order = nullptr;
+  // polygon, polygon p's at pm_order[pm_start[p] .. + pm_count[p]), each as the
+  // record the test needs (PmPair); a unit is up to pm_group of one polygon's
+  // pairs, *pm_num_units of them, listed by their first position in pm_order
+  uint4 const* pm_order = nullptr;
   uint32_t const* pm_start = nullptr;
   uint32_t const* pm_count = nullptr;
   uint4 const* pm_units = nullptr;  // per unit: first order position, polygon, pairs
@@ -671,15 +671,28 @@ struct PmUnitOp {
   }
 };
 
+// A pair in the polygon-major order, as its wave reads it in one load: the leaf's
+// point count and first sorted position, the node's job base (0 without one) and
+// the pair's first chunk slot; the count is kPmOver for a pair whose chunks pass
+// the chunk capacity (not tested; the caller sees bit 2). Slots fit 32 bits: the
+// order is used only with a chunk capacity below 2^32.
+constexpr uint32_t kPmOver = 0xFFFFFFFFu;
+
 __global__ void __launch_bounds__(256)
-  pm_scatter_kernel(PipArgs a, uint32_t* __restrict__ cursor, uint32_t* __restrict__ order)
+  pm_scatter_kernel(PipArgs a, uint32_t* __restrict__ cursor, uint4* __restrict__ order)
 {
   uint64_t const K = a.num_pairs_dev ? *a.num_pairs_dev : a.num_pairs;
   for (uint64_t k = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; k < K;
        k += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
     uint32_t const p = a.pair_poly[k], q = a.pair_quad[k];
     if (p >= a.num_polys || q >= a.num_nodes) continue;
-    order[atomicAdd(&cursor[p], 1u)] = static_cast<uint32_t>(k);
+    uint32_t const len   = a.length[q];
+    uint32_t const first = a.offset[q];
+    uint32_t const base  = a.point_base ? a.point_base[q] : 0u;
+    uint64_t const sl    = a.chunk_start[k];
+    bool const over      = sl + (len + kWave - 1) / kWave > a.chunk_cap;
+    order[atomicAdd(&cursor[p], 1u)] =
+      make_uint4(over ? kPmOver : len, first, base, static_cast<uint32_t>(sl));
   }
 }
 
@@ -737,30 +750,25 @@ __device__ __forceinline__ void pm_unit(PipArgs const& a, uint32_t i, uint32_t p
   // the unit's pairs, one per lane (their loads in flight together): leaf range,
   // job base and first chunk slot
   uint32_t const m = end - static_cast<uint32_t>(i);
-  uint32_t u_len = 0, u_first = 0, u_base = 0, u_slo = 0, u_shi = 0;
+  uint32_t u_len = 0, u_first = 0, u_base = 0, u_slot = 0;
   auto rl = [](uint32_t v, uint32_t j) {
     return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(j)));
   };
   for (uint32_t j0 = 0; j0 < m; j0 += kWave) {
     uint32_t const mj = m - j0 < kWave ? m - j0 : kWave;
     if (static_cast<uint32_t>(lane) < mj) {
-      uint32_t const k    = a.pm_order[i + j0 + lane];
-      uint32_t const quad = a.pair_quad[k];
-      u_len               = a.length[quad];
-      u_first             = a.offset[quad];
-      u_base              = a.point_base ? a.point_base[quad] : 0u;
-      uint64_t const sl   = a.chunk_start[k];
-      u_slo               = static_cast<uint32_t>(sl);
-      u_shi               = static_cast<uint32_t>(sl >> 32);
+      uint4 const r = a.pm_order[i + j0 + lane];
+      u_len         = r.x;
+      u_first       = r.y;
+      u_base        = r.z;
+      u_slot        = r.w;
       // a pair whose chunks pass the capacity is not tested (the caller sees bit 2)
-      if (sl + (u_len + kWave - 1) / kWave > a.chunk_cap) {
+      if (u_len == kPmOver) {
         cap_err = true;
         u_len   = 0;
       }
     }
-    auto slot_of = [&](uint32_t j) {
-      return (static_cast<uint64_t>(rl(u_shi, j)) << 32) | rl(u_slo, j);
-    };
+    auto slot_of = [&](uint32_t j) { return static_cast<uint64_t>(rl(u_slot, j)); };
     // the pairs' chunk records
     for (uint32_t j = 0; j < mj; ++j) {
       uint32_t const len   = rl(u_len, j);
@@ -1038,7 +1046,8 @@ void launch_pip_test(PipArgs const& args, uint64_t pairs, uint64_t* masks, uint2
   PipArgs a = args;
   if (tuning(Tuning::PipBoxFilter) == 0) a.poly_box = nullptr;
   int64_t const group = tuning(Tuning::PipPolygonGroup);
-  bool const pm       = group > 0 && pmb != nullptr && a.num_polys > 0;
+  bool const pm = group > 0 && pmb != nullptr && a.num_polys > 0 &&
+                  a.chunk_cap < (uint64_t{1} << 32);  // slots in 32 bits (PmPair)
   if (pm) {
     // polygon-major order of the valid pairs; masks start at zero (hits are ORed in)
     KernelTimer kt("pip_order", stream);
@@ -1337,8 +1346,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     // the polygon-major test's order (pip_polygon_group > 0)
     bool const pm_on = tuning(Tuning::PipPolygonGroup) > 0 && num_polys > 0;
     dbuf<uint32_t> pm_count(pm_on ? num_polys : 1), pm_start(pm_on ? num_polys : 1),
-      pm_cursor(pm_on ? num_polys : 1), pm_order(pm_on ? K : 1);
-    dbuf<uint4> pm_units(pm_on ? K : 1);
+      pm_cursor(pm_on ? num_polys : 1);
+    dbuf<uint4> pm_order(pm_on ? K : 1), pm_units(pm_on ? K : 1);
     dbuf<uint64_t> pm_partials(kScanBlocks + 1), pm_total(1), pm_num_units(1);
     PmBuffers const pmb{pm_count.get(), pm_start.get(),    pm_cursor.get(), pm_order.get(),
                         pm_units.get(), pm_partials.get(), pm_total.get(),  pm_num_units.get()};

@@ -57,7 +57,7 @@ struct PipDevice {
   uint32_t* pm_count;     // num_polys
   uint32_t* pm_start;     // num_polys
   uint32_t* pm_cursor;    // num_polys
-  uint32_t* pm_order;     // pair_cap
+  uint4* pm_order;        // pair_cap: per order position, the pair's record
   uint4* pm_units;        // pair_cap
   uint64_t* pm_partials;  // kScanBlocks + 1
   uint64_t* pm_total;     // 1
@@ -74,7 +74,7 @@ struct PmBuffers {
   uint32_t* count;      // per polygon
   uint32_t* start;      // per polygon
   uint32_t* cursor;     // per polygon
-  uint32_t* order;      // per pair
+  uint4* order;         // per pair: its record (leaf count, first, base, slot)
   uint4* units;         // per pair (at most one unit per pair)
   uint64_t* partials;   // kScanBlocks + 1
   uint64_t* total;      // 1
