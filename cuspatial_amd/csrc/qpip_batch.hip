@@ -1389,10 +1389,13 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.ring_offsets   = reinterpret_cast<uint32_t const*>(P.ring_offsets);
   d.num_rings      = static_cast<uint64_t>(P.num_ring_offsets - 1);
   d.ring_edges     = P.ring_edges.get();
-  d.edges          = P.edges.get();
-  d.edges_b        = P.edges_b.get();
+  d.edges          = nullptr;  // the exact retest walks the vertices
+  d.edges_b        = nullptr;
   d.lines          = P.lines.get();
   d.ring_amax      = P.ring_amax.get();
+  d.poly_x         = P.poly_x;
+  d.poly_y         = P.poly_y;
+  d.num_vertices   = static_cast<uint64_t>(P.num_vertices);
   d.poly_box       = P.poly_box.get() + g.PO0;
   d.pairs_per_wave = g.pairs_per_wave;
   d.chunk_start    = g.chunk_start.get();
@@ -1689,8 +1692,6 @@ extern "C" int qpip_batch_plan_create(int64_t num_jobs,
     P->poly_x           = poly_x;
     P->poly_y           = poly_y;
     int64_t const rings = num_ring_offsets > 0 ? num_ring_offsets - 1 : 0;
-    P->edges.reset(std::max<int64_t>(num_poly_x, 1));
-    P->edges_b.reset(std::max<int64_t>(num_poly_x, 1));
     P->ring_edges.reset(std::max<int64_t>(rings, 1));
     P->lines.reset(std::max<int64_t>(num_poly_x, 1));
     P->ring_amax.reset(std::max<int64_t>(rings, 1));
@@ -1701,7 +1702,7 @@ extern "C" int qpip_batch_plan_create(int64_t num_jobs,
       if (rings > 0) {
         build_edges_kernel<<<grid_for(static_cast<uint64_t>(rings), 256 / kWave), 256, 0, s>>>(
           reinterpret_cast<uint32_t const*>(ring_offsets), rings, poly_x, poly_y, num_poly_x,
-          P->edges.get(), P->edges_b.get(), P->ring_edges.get(), P->lines.get(),
+          nullptr, nullptr, P->ring_edges.get(), P->lines.get(),
           P->ring_amax.get(), ring_box.get());
         QPIP_LAUNCH_CHECK();
       }

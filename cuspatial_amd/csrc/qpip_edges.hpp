@@ -180,8 +180,8 @@ __device__ __forceinline__ void build_ring_edges(uint32_t const* __restrict__ ri
       uint64_t const km = ballot(keep);
       if (keep) {
         uint32_t const slot = base + static_cast<uint32_t>(__popcll(km & lanemask_lt()));
-        edges[vb + slot]    = Edge{ax, ay, bx - ax, by - ay};
-        edges_b[vb + slot]  = EdgeB{bx, by};
+        if (edges) edges[vb + slot] = Edge{ax, ay, bx - ax, by - ay};
+        if (edges_b) edges_b[vb + slot] = EdgeB{bx, by};
         if (lines) lines[vb + slot] = line_of(ax, ay, bx - ax, by - ay);
       }
       base += static_cast<uint32_t>(__popcll(km));
@@ -204,8 +204,8 @@ __device__ __forceinline__ void build_ring_edges(uint32_t const* __restrict__ ri
       double const run = bx - ax, rise = by - ay;
       if (float_equal(run, 0.0) && float_equal(rise, 0.0)) continue;
       ok               = ok && box_rise_ok(rise);
-      edges[vb + ne]   = Edge{ax, ay, run, rise};
-      edges_b[vb + ne] = EdgeB{bx, by};
+      if (edges) edges[vb + ne] = Edge{ax, ay, run, rise};
+      if (edges_b) edges_b[vb + ne] = EdgeB{bx, by};
       if (lines) lines[vb + ne] = line_of(ax, ay, run, rise);
       ++ne;
       bx = ax;

@@ -106,6 +106,10 @@ struct PipArgs {
   uint32_t const* pm_count = nullptr;
   uint4 const* pm_units = nullptr;  // per unit: first order position, polygon, pairs
   uint64_t const* pm_num_units = nullptr;
+  // the polygons' vertices (num_vertices of them): the exact retest walks them
+  double const* poly_x  = nullptr;
+  double const* poly_y  = nullptr;
+  uint64_t num_vertices = 0;
   uint32_t pm_group        = 0;
 };
 
@@ -201,6 +205,53 @@ __device__ __forceinline__ double wave_max_abs(double const (&px)[kSlots],
 // of G with edges that neither toggle (their vertex repeats the ring's last one)
 // nor mark (t = +inf for finite points; for a point that is not finite t is NaN,
 // a mark, as on every real edge).
+// The reference's test of one point against rings [r0, r1) of a polygon
+// (is_point_in_polygon.cuh:46-100), on the vertices: a ring's first edge ends at
+// its last vertex, an edge with run and rise both within 4 ULP of 0 is skipped
+// without moving `b`, a collinear point within an edge's x range is on the edge.
+// Vertex ranges are clamped to the vertex count, as the edge table's builder
+// clamps them. Returns inside and not on an edge.
+__device__ __forceinline__ bool exact_in_polygon(PipArgs const& a, uint32_t r0, uint32_t r1,
+                                                 double px, double py)
+{
+  bool inside = false, on_edge = false;
+  for (uint32_t r = r0; r < r1; ++r) {
+    uint64_t vb = a.ring_offsets[r], ve = a.ring_offsets[r + 1];
+    if (vb > a.num_vertices) vb = a.num_vertices;
+    if (ve > a.num_vertices) ve = a.num_vertices;
+    if (ve <= vb) continue;
+    double bx = a.poly_x[ve - 1], by = a.poly_y[ve - 1];
+    bool y0 = by > py;
+    for (uint64_t v = vb; v < ve; ++v) {
+      double const ax = a.poly_x[v], ay = a.poly_y[v];
+      double const run = bx - ax, rise = by - ay;
+      if (float_equal(run, 0.0) && float_equal(rise, 0.0)) continue;
+      double const lhs = (px - ax) * rise;  // run_to_point * rise
+      double const rhs = run * (py - ay);   // run * rise_to_point
+      if (float_equal(rhs, lhs)) {
+        double const minx = ax < bx ? ax : bx;
+        double const maxx = ax < bx ? bx : ax;
+        if (minx <= px && px <= maxx) on_edge = true;
+      }
+      bool const y1 = ay > py;
+      if (y1 != y0 && ((lhs < rhs) != y1)) inside = !inside;
+      y0 = y1;
+      bx = ax;
+      by = ay;
+    }
+  }
+  return inside && !on_edge;
+}
+
+// The y of a ring's last vertex (clamped as above): the `b` end of its first
+// effective edge, whose y starts the crossing chain.
+__device__ __forceinline__ double ring_last_y(PipArgs const& a, uint32_t r)
+{
+  uint64_t ve = a.ring_offsets[r + 1];
+  if (ve > a.num_vertices) ve = a.num_vertices;
+  return a.poly_y[ve - 1];
+}
+
 // A polygon's ring range and its first ring's table entries, loaded ahead of its
 // test: a polygon-major wave loads them when it starts its unit, so that their
 // dependent loads overlap the point gathers instead of starting each test.
@@ -219,7 +270,7 @@ __device__ __forceinline__ RingHint ring_hint(PipArgs const& a, uint32_t poly)
     h.e0   = uniform(a.ring_offsets[h.r0]);
     h.ne   = uniform(a.ring_edges[h.r0]);
     h.amax = a.ring_amax[h.r0];
-    if (h.ne > 0) h.by0 = a.edges_b[h.e0].by;
+    if (h.ne > 0) h.by0 = ring_last_y(a, h.r0);
   }
   return h;
 }
@@ -260,7 +311,7 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly,
     double const tol = s < 0x1p400 ? fma(s, 0x1p-48, 0x1p-1000) : __builtin_inf();
     // y0 of the first edge is its `b` (the ring's last vertex); afterwards the
     // previous edge's `a` (is_point_in_polygon.cuh:339, :375-376).
-    double const by0 = hinted ? hint->by0 : a.edges_b[e0].by;
+    double const by0 = hinted ? hint->by0 : ring_last_y(a, r);
     uint64_t y0[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) y0[u] = ballot(by0 > py[u]);
@@ -356,31 +407,8 @@ __device__ __forceinline__ void test_polygon(PipArgs const& a, uint32_t poly,
     uint64_t exact_m   = 0;
     if (sus) {
       bool const suspect = (sus >> lane) & 1ull;
-      bool inside = false, on_edge = false;
-      for (uint32_t r = r0; r < r1; ++r) {
-        uint32_t const e0 = uniform(a.ring_offsets[r]);
-        uint32_t const ne = uniform(a.ring_edges[r]);
-        if (ne == 0) continue;
-        bool y0 = a.edges_b[e0].by > py[u];
-        for (uint32_t e = 0; e < ne; ++e) {
-          Edge const E = a.edges[e0 + e];
-          if (suspect) {
-            double const lhs = (px[u] - E.ax) * E.rise;  // run_to_point * rise
-            double const rhs = E.run * (py[u] - E.ay);   // run * rise_to_point
-            if (float_equal(rhs, lhs)) {
-              // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
-              double const bx   = a.edges_b[e0 + e].bx;
-              double const minx = E.ax < bx ? E.ax : bx;
-              double const maxx = E.ax < bx ? bx : E.ax;
-              if (minx <= px[u] && px[u] <= maxx) on_edge = true;
-            }
-            bool const y1 = E.ay > py[u];
-            if (y1 != y0 && ((lhs < rhs) != y1)) inside = !inside;
-            y0 = y1;
-          }
-        }
-      }
-      exact_m = ballot(suspect && inside && !on_edge);
+      bool const in      = suspect && exact_in_polygon(a, r0, r1, px[u], py[u]);
+      exact_m            = ballot(in);
     }
     hit_m[u] = ((inside_m[u] & ~sus) | exact_m) & valid_m[u];
   }
@@ -1170,29 +1198,7 @@ __device__ __forceinline__ void pairwise_one(PipArgs const& a, uint64_t i, uint8
   load_point(a, i, px, py);
   uint32_t r0 = a.poly_offsets[i], r1 = a.poly_offsets[i + 1];
   if (r1 > a.num_rings) r1 = static_cast<uint32_t>(a.num_rings);
-  bool inside = false, on_edge = false;
-  for (uint32_t r = r0; r < r1; ++r) {
-    uint32_t const e0 = a.ring_offsets[r];
-    uint32_t const ne = a.ring_edges[r];
-    if (ne == 0) continue;
-    bool y0 = a.edges_b[e0].by > py;
-    for (uint32_t e = 0; e < ne; ++e) {
-      Edge const E      = a.edges[e0 + e];
-      double const lhs  = (px - E.ax) * E.rise;  // run_to_point * rise
-      double const rhs  = E.run * (py - E.ay);   // run * rise_to_point
-      if (float_equal(rhs, lhs)) {
-        // is_point_in_polygon.cuh:56-65: collinear and within the edge's x range
-        double const bx   = a.edges_b[e0 + e].bx;
-        double const minx = E.ax < bx ? E.ax : bx;
-        double const maxx = E.ax < bx ? bx : E.ax;
-        if (minx <= px && px <= maxx) on_edge = true;
-      }
-      bool const y1 = E.ay > py;
-      if (y1 != y0 && ((lhs < rhs) != y1)) inside = !inside;
-      y0 = y1;
-    }
-  }
-  out[i] = (inside && !on_edge) ? 1 : 0;
+  out[i] = exact_in_polygon(a, r0, r1, px, py) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(256)
@@ -1223,6 +1229,11 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
                d.chunk_start, d.chunk_cap,
                d.num_nodes, d.xy, reinterpret_cast<uintptr_t>(d.xy) % 16 == 0, d.pairs_per_wave,
                n_pairs, d.chunk_cap, d.point_base, d.poly_box};
+  args.poly_x       = d.poly_x;
+  args.poly_y       = d.poly_y;
+  args.num_vertices = d.num_vertices;
+  QPIP_EXPECTS(args.num_vertices == 0 || (args.poly_x && args.poly_y),
+               "polygon vertices must not be null");
   PmBuffers const pmb{d.pm_count,    d.pm_start, d.pm_cursor, d.pm_order,
                       d.pm_units,    d.pm_partials, d.pm_total, d.pm_num_units};
   launch_pip_test(args, d.pair_cap, d.masks, d.chunk_info, flags, d.pm_order ? &pmb : nullptr,
@@ -1304,8 +1315,6 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
     QPIP_HIP(hipMemsetAsync(bad.get(), 0, 2 * sizeof(uint32_t), stream));
 
     // 1. edges
-    dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
-    dbuf<EdgeB> edges_b(std::max<int64_t>(num_poly_x, 1));
     dbuf<uint32_t> ring_edges(std::max<uint64_t>(num_rings, 1));
     dbuf<LineEdge> lines(std::max<int64_t>(num_poly_x, 1));
     dbuf<double> ring_amax(std::max<uint64_t>(num_rings, 1));
@@ -1316,8 +1325,8 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
       KernelTimer kt("pip_edges", stream);
       if (num_rings > 0) {
         build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
-          ring_offsets, static_cast<int64_t>(num_rings), poly_x, poly_y, num_poly_x, edges.get(),
-          edges_b.get(), ring_edges.get(), lines.get(), ring_amax.get(), ring_box.get());
+          ring_offsets, static_cast<int64_t>(num_rings), poly_x, poly_y, num_poly_x, nullptr,
+          nullptr, ring_edges.get(), lines.get(), ring_amax.get(), ring_box.get());
         QPIP_LAUNCH_CHECK();
       }
       if (num_polys > 0) {
@@ -1373,11 +1382,14 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
       }
       PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
                    n_pts, x, y, poly_offsets, num_polys, num_rings, ring_offsets,
-                   ring_edges.get(), edges.get(), edges_b.get(), lines.get(), ring_amax.get(),
+                   ring_edges.get(), nullptr, nullptr, lines.get(), ring_amax.get(),
                    chunk_start.get(), chunk_cap,
                    static_cast<uint64_t>(num_nodes), xy,
                    xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,
                    pairs_per_wave(K), nullptr, chunk_cap, point_base, poly_box.get()};
+      args.poly_x       = poly_x;
+      args.poly_y       = poly_y;
+      args.num_vertices = static_cast<uint64_t>(num_poly_x);
       launch_pip_test(args, K, masks.get(), chunk_info.get(), bad.get() + 1,
                       pm_on ? &pmb : nullptr, stream);
       // chunks past the capacity were not tested: the scan covers those that were
@@ -1546,8 +1558,6 @@ int point_in_polygon_entry(const double* x,
     if (num_polys > 0) QPIP_EXPECTS(poly_offsets && ring_offsets, "offsets must not be null");
     StreamScope scope(stream);
 
-    dbuf<Edge> edges(std::max<int64_t>(num_poly_x, 1));
-    dbuf<EdgeB> edges_b(std::max<int64_t>(num_poly_x, 1));
     dbuf<uint32_t> ring_edges(std::max<uint64_t>(num_rings, 1));
     dbuf<LineEdge> lines(pairwise ? 1 : std::max<int64_t>(num_poly_x, 1));
     dbuf<double> ring_amax(pairwise ? 1 : std::max<uint64_t>(num_rings, 1));
@@ -1555,7 +1565,7 @@ int point_in_polygon_entry(const double* x,
       KernelTimer kt("pip_edges", stream);
       build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
         reinterpret_cast<uint32_t const*>(ring_offsets), static_cast<int64_t>(num_rings), poly_x,
-        poly_y, num_poly_x, edges.get(), edges_b.get(), ring_edges.get(),
+        poly_y, num_poly_x, nullptr, nullptr, ring_edges.get(),
         pairwise ? nullptr : lines.get(), pairwise ? nullptr : ring_amax.get(), nullptr);
       QPIP_LAUNCH_CHECK();
     }
@@ -1570,8 +1580,9 @@ int point_in_polygon_entry(const double* x,
     a.num_rings    = num_rings;
     a.ring_offsets = reinterpret_cast<uint32_t const*>(ring_offsets);
     a.ring_edges   = ring_edges.get();
-    a.edges        = edges.get();
-    a.edges_b      = edges_b.get();
+    a.poly_x       = poly_x;
+    a.poly_y       = poly_y;
+    a.num_vertices = static_cast<uint64_t>(num_poly_x);
     a.lines        = lines.get();
     a.ring_amax    = ring_amax.get();
     OutputAllocator oa(mr, stream);
@@ -1736,7 +1747,7 @@ extern "C" int qpip_point_in_polygon_noalloc(const double* x,
     if (num_rings > 0) {
       build_edges_kernel<<<grid_for(num_rings, 256 / kWave), 256, 0, stream>>>(
         reinterpret_cast<uint32_t const*>(ring_offsets), static_cast<int64_t>(num_rings), poly_x,
-        poly_y, num_poly_x, edges, edges_b, ring_edges, lines, ring_amax, nullptr);
+        poly_y, num_poly_x, nullptr, nullptr, ring_edges, lines, ring_amax, nullptr);
       QPIP_LAUNCH_CHECK();
     }
     uint64_t const n = static_cast<uint64_t>(num_x);
@@ -1749,8 +1760,9 @@ extern "C" int qpip_point_in_polygon_noalloc(const double* x,
     a.num_rings    = num_rings;
     a.ring_offsets = reinterpret_cast<uint32_t const*>(ring_offsets);
     a.ring_edges   = ring_edges;
-    a.edges        = edges;
-    a.edges_b      = edges_b;
+    a.poly_x       = poly_x;
+    a.poly_y       = poly_y;
+    a.num_vertices = static_cast<uint64_t>(num_poly_x);
     a.lines        = lines;
     a.ring_amax    = ring_amax;
     uint64_t const waves = (n + kSlots * kWave - 1) / (kSlots * kWave);

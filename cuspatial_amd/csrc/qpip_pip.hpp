@@ -43,6 +43,9 @@ struct PipDevice {
   EdgeB const* edges_b;
   LineEdge const* lines;    // per edge: the test's line form
   double const* ring_amax;  // per ring: largest |vertex coordinate|
+  double const* poly_x  = nullptr;  // the vertices (the exact retest walks them)
+  double const* poly_y  = nullptr;
+  uint64_t num_vertices = 0;
   PolyBox const* poly_box;  // per polygon (null: the test filters no points)
   uint32_t pairs_per_wave;
   // workspace
