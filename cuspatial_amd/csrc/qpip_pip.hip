@@ -1659,22 +1659,19 @@ extern "C" int qpip_pairwise_point_in_polygon(const double* x,
 // --- allocation-free bitmask point-in-polygon ---------------------------------------
 // qpip_point_in_polygon_noalloc runs the kernels of qpip_point_in_polygon with the
 // edge table in `scratch`, a fixed device array of this file, instead of pool
-// memory: edges (32 B per vertex), their cold halves (16 B per vertex), their
-// line forms (32 B per vertex), and each ring's largest |coordinate| and edge
-// count (12 B per ring). 20.75 MiB per device, held by every device the library
-// runs on until the process ends.
+// memory: the edges' line forms (32 B per vertex) and each ring's largest
+// |coordinate| and edge count (12 B per ring); the exact retest reads the caller's
+// vertices. 8.75 MiB per device, held by every device the library runs on until
+// the process ends.
 namespace {
 
-constexpr size_t kNoAllocEdgesOff  = 0;
-constexpr size_t kNoAllocEdgesBOff = kNoAllocEdgesOff + sizeof(qpip::Edge) * QPIP_NOALLOC_MAX_VERTICES;
-constexpr size_t kNoAllocLinesOff  = kNoAllocEdgesBOff + sizeof(qpip::EdgeB) * QPIP_NOALLOC_MAX_VERTICES;
+constexpr size_t kNoAllocLinesOff  = 0;
 constexpr size_t kNoAllocAmaxOff   = kNoAllocLinesOff + sizeof(qpip::LineEdge) * QPIP_NOALLOC_MAX_VERTICES;
 constexpr size_t kNoAllocRingsOff  = kNoAllocAmaxOff + sizeof(double) * QPIP_NOALLOC_MAX_RINGS;
 constexpr size_t kNoAllocBytes     = kNoAllocRingsOff + sizeof(uint32_t) * QPIP_NOALLOC_MAX_RINGS;
 static_assert(kNoAllocLinesOff % alignof(qpip::LineEdge) == 0 && kNoAllocAmaxOff % 8 == 0,
               "scratch layout");
-static_assert(kNoAllocEdgesBOff % alignof(qpip::EdgeB) == 0 && kNoAllocRingsOff % 4 == 0,
-              "scratch carving keeps each table aligned");
+static_assert(kNoAllocRingsOff % 4 == 0, "scratch carving keeps each table aligned");
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
@@ -1736,8 +1733,6 @@ extern "C" int qpip_point_in_polygon_noalloc(const double* x,
         " polygon vertices and " QPIP_STR(QPIP_NOALLOC_MAX_RINGS)
         " rings fit its fixed scratch; use qpip_point_in_polygon");
     unsigned char* const base = scratch_on_current_device();
-    Edge* const edges         = reinterpret_cast<Edge*>(base + kNoAllocEdgesOff);
-    EdgeB* const edges_b      = reinterpret_cast<EdgeB*>(base + kNoAllocEdgesBOff);
     uint32_t* const ring_edges = reinterpret_cast<uint32_t*>(base + kNoAllocRingsOff);
     LineEdge* const lines      = reinterpret_cast<LineEdge*>(base + kNoAllocLinesOff);
     double* const ring_amax    = reinterpret_cast<double*>(base + kNoAllocAmaxOff);

@@ -353,7 +353,7 @@ QPIP_API int qpip_point_in_polygon(const double* x,
  * HIP graph capture, serving loops that must not call hipMalloc, harnesses that
  * forbid it). Same checks, test and output as qpip_point_in_polygon, but the caller
  * passes the output, `result` (num_x INT32 in device memory), and the edge table
- * lives in a fixed device array of the library (20.75 MiB per device, kept until
+ * lives in a fixed device array of the library (8.75 MiB per device, kept until
  * the process ends). The call allocates nothing, queues its two kernels on `stream`
  * and returns without waiting.
  * Limits: at most QPIP_NOALLOC_MAX_VERTICES polygon vertices and
