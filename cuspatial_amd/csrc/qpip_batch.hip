@@ -1189,8 +1189,6 @@ struct BatchPlan {
   int32_t const* ring_offsets = nullptr;
   int64_t num_ring_offsets = 0, num_vertices = 0;
   double const *poly_x = nullptr, *poly_y = nullptr;
-  dbuf<Edge> edges;
-  dbuf<EdgeB> edges_b;
   dbuf<uint32_t> ring_edges;
   dbuf<LineEdge> lines;    // per edge: the PIP test's line form
   dbuf<double> ring_amax;  // per ring
@@ -1217,8 +1215,6 @@ struct BatchPlan {
     // every stream that used the buffers has finished: they go back idle
     CompletedScope done;
     groups.clear();
-    edges.release();
-    edges_b.release();
     ring_edges.release();
     lines.release();
     ring_amax.release();
@@ -1389,8 +1385,6 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
   d.ring_offsets   = reinterpret_cast<uint32_t const*>(P.ring_offsets);
   d.num_rings      = static_cast<uint64_t>(P.num_ring_offsets - 1);
   d.ring_edges     = P.ring_edges.get();
-  d.edges          = nullptr;  // the exact retest walks the vertices
-  d.edges_b        = nullptr;
   d.lines          = P.lines.get();
   d.ring_amax      = P.ring_amax.get();
   d.poly_x         = P.poly_x;

@@ -80,8 +80,6 @@ struct PipArgs {
   uint64_t num_rings;
   uint32_t const* ring_offsets;
   uint32_t const* ring_edges;
-  Edge const* edges;
-  EdgeB const* edges_b;
   LineEdge const* lines;   // per edge: the fast loop's form
   double const* ring_amax;  // per ring: largest |vertex coordinate|
   uint32_t const* chunk_start;
@@ -1225,7 +1223,7 @@ void launch_pip_device(PipDevice const& d, hipStream_t stream)
   QPIP_LAUNCH_CHECK();
   PipArgs args{d.pair_poly, d.pair_quad, d.pair_cap, d.length, d.offset, d.point_indices,
                d.num_points, nullptr, nullptr, d.poly_offsets, d.num_polys, d.num_rings,
-               d.ring_offsets, d.ring_edges, d.edges, d.edges_b, d.lines, d.ring_amax,
+               d.ring_offsets, d.ring_edges, d.lines, d.ring_amax,
                d.chunk_start, d.chunk_cap,
                d.num_nodes, d.xy, reinterpret_cast<uintptr_t>(d.xy) % 16 == 0, d.pairs_per_wave,
                n_pairs, d.chunk_cap, d.point_base, d.poly_box};
@@ -1382,7 +1380,7 @@ int quadtree_point_in_polygon_entry(const uint32_t* poly_index,
       }
       PipArgs args{poly_index, quad_index, K, length, offset, point_indices,
                    n_pts, x, y, poly_offsets, num_polys, num_rings, ring_offsets,
-                   ring_edges.get(), nullptr, nullptr, lines.get(), ring_amax.get(),
+                   ring_edges.get(), lines.get(), ring_amax.get(),
                    chunk_start.get(), chunk_cap,
                    static_cast<uint64_t>(num_nodes), xy,
                    xy != nullptr && reinterpret_cast<uintptr_t>(xy) % 16 == 0,

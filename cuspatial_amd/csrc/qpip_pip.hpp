@@ -39,8 +39,6 @@ struct PipDevice {
   uint32_t const* ring_offsets;
   uint64_t num_rings;
   uint32_t const* ring_edges;
-  Edge const* edges;
-  EdgeB const* edges_b;
   LineEdge const* lines;    // per edge: the test's line form
   double const* ring_amax;  // per ring: largest |vertex coordinate|
   double const* poly_x  = nullptr;  // the vertices (the exact retest walks them)
