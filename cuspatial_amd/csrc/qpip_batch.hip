@@ -1313,7 +1313,8 @@ void plan_enqueue(BatchPlan& P, PlanGroup& g, size_t gi, hipStream_t s)
                                                         g.jobs.get(), g.J, g.fmeta.get(),
                                                         g.trees.get());
   QPIP_LAUNCH_CHECK();
-  batch_finalize_kernel<<<grid_for(g.node_cap, 256), 256, 0, s>>>(
+  // a bounded grid striding over the node count (the capacity is far above it)
+  batch_finalize_kernel<<<grid_for(g.node_cap, 256, 8192), 256, 0, s>>>(
     g.nkey.get(), g.nstart.get(), g.ncount.get(), g.nlevel.get(), g.njob.get(), g.cfirst.get(),
     g.ccount.get(), g.jobs.get(), g.trees.get(), 0, g.fmeta.get(), levels, g.f_internal.get(),
     g.f_length.get(), g.f_offset.get(), g.f_base.get());
