@@ -11,23 +11,32 @@
 //      vertices whose edge to the current `b` is zero within 4 ULP (without moving
 //      `b`, :65), and tests every other edge (a, b). Which edges are skipped does
 //      not depend on the test point, so each ring is walked once here into a list
-//      of effective edges {ax, ay, run, rise} + {bx, by}. The per-point test is then
-//      order-independent: inside = parity of crossings over all effective edges,
-//      forced to false if any edge is an on-edge hit (the reference breaks out and
-//      returns false at the first such edge).
+//      of effective edges, each stored as the scaled line form the fast test reads
+//      (LineEdge). The per-point test is then order-independent: inside = parity
+//      of crossings over all effective edges, forced to false if any edge is an
+//      on-edge hit (the reference breaks out and returns false at the first such
+//      edge).
 //   2. Chunks. Pair k covers ceil(length / 64) chunks of 64 consecutive point
 //      positions; a scan gives each pair its first chunk, and a map gives each
 //      chunk its pair (get_quad_and_local_point_indices.cuh:26-45 searches for it).
-//   3. Test: one wave per segment of a leaf's pairs (they are adjacent in pair
-//      order), up to 128 points (2 chunks) at a time. The lanes read
-//      consecutive point_indices entries (coalesced) and gather the coordinates
-//      once; points outside a polygon's vertex box are dropped and the rest
-//      packed into fewer lanes; each polygon's edges are wave-uniform and are
-//      staged through the wave's LDS slot in coalesced batches. A certain-sign
-//      form of the crossing test decides almost every (point, edge); the rest
-//      are retested with the reference's own FP64 arithmetic. Each wave stores
-//      its 64-bit hit ballot and its popcount.
-//   4. Scan of hit counts -> output positions; one host sync for the total.
+//   3. Test, in one of two orders. Leaf order (pip_test_kernel): one wave per
+//      segment of a leaf's pairs (they are adjacent in pair order), up to 128
+//      points (2 chunks) at a time. Polygon order (pip_test_pm_kernel): the
+//      pairs counting-sorted by polygon, one wave per unit of up to pm_group of
+//      one polygon's pairs, with the ring's edges shared by the unit's leaves;
+//      a flush of at most 32 points runs as several lane groups that take
+//      different edges of the same batch. Either way the lanes read consecutive
+//      point_indices entries (coalesced) and gather the coordinates once; points
+//      outside a polygon's vertex box are dropped and the rest packed into fewer
+//      lanes; each polygon's edges are wave-uniform and are staged through the
+//      wave's LDS slot in coalesced batches. A certain-sign form of the crossing
+//      test decides almost every (point, edge); a point with an undecided edge
+//      is retested by walking the ring's vertices with the reference's own loop
+//      and FP64 arithmetic. The hits of a chunk end up as the bits of its 64-bit
+//      mask (a wave's ballot in leaf order, one atomic OR per hit in polygon
+//      order, where one flush packs the points of several of the unit's chunks).
+//   4. Scan of the masks' popcounts -> output positions; one host sync for the
+//      total.
 //   5. Write: each wave expands the ballots of 64 chunks into (polygon, point
 //      position) pairs.
 //   No candidate-sized buffer is ever allocated, so the reference's out-of-memory
